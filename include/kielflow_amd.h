@@ -1,0 +1,180 @@
+/*
+ * kielflow_amd.h -- C ABI of the MI355X (gfx950) implementation of kielflow's
+ * pseudo-spectral right-hand side, RK4 time step and nonlinear Poisson solve.
+ *
+ * Every entry point returns KF_OK (0) or a KF_ERR_* code; kf_last_error()
+ * gives the message of the last failure on the calling thread.  No C++
+ * exception crosses this boundary.  All field pointers are DEVICE pointers
+ * (use kf_malloc / kf_memcpy_*), laid out as the reference lays out its
+ * fields (field_base.hpp:23-24, field_imag.cpp:15-16):
+ *   real      double[Nx][Ny][Nz]            z fastest
+ *   spectral  double[Nx][Ny][Nz/2+1][2]     interleaved re/im (fftw_complex)
+ * Work is enqueued on the context's stream; entry points that return
+ * results to the host synchronise that stream.
+ *
+ * Reference citations name the kielflow file:line of the interface each
+ * entry point replaces.
+ */
+#ifndef KIELFLOW_AMD_H_
+#define KIELFLOW_AMD_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    KF_OK = 0,
+    KF_ERR_ARG = 1,             /* invalid argument or unsupported size      */
+    KF_ERR_HIP = 2,             /* HIP runtime error                          */
+    KF_ERR_NOT_CONVERGED = 3,   /* Poisson solve hit its iteration budget     */
+    KF_ERR_NAN = 4              /* NaN in the potential (solver_poisson_jacobi_nlin.cpp:78-99) */
+};
+
+enum { KF_FP64 = 0, KF_MIXED = 1, KF_FP32 = 2 };
+
+/* axis types as stored by the reference's HDF5 files (IO.cpp:153-164) */
+enum { KF_AXIS_EQST = 1, KF_AXIS_HYST = 2, KF_AXIS_SIST = 3 };
+
+/* Poisson algorithms */
+enum {
+    KF_POISSON_NEWTON_MG = 0,  /* default: Newton + red-black multigrid, FP64 residual test */
+    KF_POISSON_NLJ = 1,        /* the reference iteration (solver_poisson_jacobi_nlin.cpp:140-265) */
+    KF_POISSON_CASCADE = 2     /* the reference MG cascade (solver_poisson_multigrid.cpp:246-418,
+                                  cycle of kielflow.cpp:207-217) */
+};
+
+typedef struct kf_ctx_s *kf_ctx;
+typedef struct kf_grid_s *kf_grid;
+typedef struct kf_fft_s *kf_fft;
+typedef struct kf_poisson_s *kf_poisson;
+typedef struct kf_rhs_s *kf_rhs;
+typedef struct kf_rk4_s *kf_rk4;
+
+/* axis_CoEqSt(l0, L, N) (axis_EqSt.hpp:15), axis_CoSiSt(l0, L, N, m)
+ * (axis_SiSt.hpp), axis_CoHySt(l0, L, N, m) (axis_HySt.hpp) */
+typedef struct {
+    int type;
+    int n;
+    double l0;
+    double L;
+    double m;
+} kf_axis;
+
+/* parameters (parameters.hpp:7-21) */
+typedef struct {
+    double M, tau, theta, mu, beta;
+} kf_parameters;
+
+/* compile-time term switches of rhs_standard.hpp:22-30 made runtime:
+ * _RHS_SVISCOSITY and _RHS_PENALIZATION (the others are always on, as shipped) */
+typedef struct {
+    int sv_on;
+    int pen_on;
+} kf_rhs_switches;
+
+/* ---------------------------------------------------------------- context */
+const char *kf_last_error(void);
+int kf_ctx_create(int device, kf_ctx *out);
+int kf_ctx_destroy(kf_ctx ctx);
+int kf_ctx_synchronize(kf_ctx ctx);
+void *kf_ctx_stream(kf_ctx ctx);                 /* hipStream_t */
+int kf_malloc(kf_ctx ctx, size_t bytes, void **out);
+int kf_free(kf_ctx ctx, void *p);
+int kf_memcpy_h2d(kf_ctx ctx, void *dst, const void *src, size_t bytes);
+int kf_memcpy_d2h(kf_ctx ctx, void *dst, const void *src, size_t bytes);
+int kf_memcpy_d2d(kf_ctx ctx, void *dst, const void *src, size_t bytes);
+int kf_memset(kf_ctx ctx, void *dst, int value, size_t bytes);
+
+/* ---------------------------------------------------------------- grid
+ * grid(const axis&, const axis&, const axis&) (grid_base.hpp:9-15).
+ * Supported extents: 2^a 3^b up to 256 in x and y; Nz even with Nz/2 of
+ * that form. */
+int kf_grid_create(kf_ctx ctx, const kf_axis axes[3], kf_grid *out);
+int kf_grid_destroy(kf_grid g);
+int kf_grid_dims(kf_grid g, int *nx, int *ny, int *nz, size_t *nreal, size_t *ncplx);
+/* axis::val_at / k_val_at / dS (axis_interface.hpp:72-74, axis.hpp:44) */
+int kf_axis_values(const kf_axis *a, int index, double *val, double *k, double *dS);
+
+/* ---------------------------------------------------------------- FFT
+ * OP_FFT(grid) / OP_iFFT(grid) with operator()(in, out)
+ * (OP_FFT.hpp:13-26, OP_iFFT.hpp:13-26; free FFT/iFFT operations.hpp:48-49).
+ * r2c is unnormalised; c2r scales by 1/(Nx Ny Nz) (OP_iFFT.cpp:59-61) and
+ * transforms the Hermitian projection of the kz=0 and kz=Nz/2 planes.  The
+ * input is preserved.  precision KF_FP64 (double fields) or KF_FP32 (float). */
+int kf_fft_create(kf_grid g, int precision, kf_fft *out);
+int kf_fft_destroy(kf_fft f);
+int kf_fft_r2c(kf_fft f, const void *in_real, void *out_cplx);
+int kf_fft_c2r(kf_fft f, const void *in_cplx, void *out_real);
+
+/* ---------------------------------------------------------------- spectral operators (FP64)
+ * OP_partial_derivative(grid, dir)::operator()(in, out), dir 0/1/2 = e_x/e_y/e_z
+ * (OP_partial_derivative.cpp:10-136); in == out allowed. */
+int kf_partial_derivative(kf_grid g, int dir, const double *in, double *out);
+/* OP_dealiasing(grid, k<2/3)::operator()(field_imag&) (OP_dealiasing.hpp:12-32) */
+int kf_dealias23(kf_grid g, double *inout);
+/* dealiasing_undesignated(F, exp(-2000 k^15)) (operations.hpp:20-46, rhs_standard.cpp:220) */
+int kf_filter_phi(kf_grid g, double *inout);
+
+/* ---------------------------------------------------------------- Poisson
+ * interface_relaxation_solver (interface_poisson_solver.hpp:14-34) and
+ * solver_poisson_jacobi_nlin / solver_poisson_multigrid (their .hpp:38-56). */
+int kf_poisson_create(kf_grid g, int precision, int mode, kf_poisson *out);
+int kf_poisson_destroy(kf_poisson p);
+int kf_poisson_set_tolerance(kf_poisson p, double tol);
+int kf_poisson_set_max_iterations(kf_poisson p, int iters);
+/* solve(Phi_IO, rho): phi FP64 in/out (warm start), rho FP64.  Returns
+ * KF_ERR_NOT_CONVERGED when the budget is spent; iters/resid may be NULL. */
+int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid);
+int kf_poisson_converged(kf_poisson p);         /* converged() of the last solve */
+/* residual max|L_h phi - exp(phi) + rho| / max|rho| of the discrete equation */
+int kf_poisson_residual(kf_poisson p, const double *phi, const double *rho, double *resid);
+
+/* ---------------------------------------------------------------- RHS
+ * rhs_standard(Params, Solver, Phi, n_d, mask, grid) (rhs_standard.hpp:35-37)
+ * with interface_rhs::solve(t, FUx, FUy, FUz, Fn) (interface_rhs.hpp:11-12).
+ * precision KF_FP64 or KF_MIXED (FFT/RHS/sweeps in FP32). */
+int kf_rhs_create(kf_grid g, const kf_parameters *params, const kf_rhs_switches *sw,
+                  kf_poisson solver, int precision, kf_rhs *out);
+int kf_rhs_destroy(kf_rhs r);
+/* phi: persistent FP64 potential the RHS keeps using (held by reference, as
+ * rhs_standard::Phi); nd, mask: FP64 real fields (copied). NULL keeps the old. */
+int kf_rhs_set_fields(kf_rhs r, double *phi, const double *nd, const double *mask);
+int kf_rhs_set_parameters(kf_rhs r, const kf_parameters *params);
+/* in place: the four FP64 spectral fields become d/dt of the state */
+int kf_rhs_solve(kf_rhs r, double t, double *FUx, double *FUy, double *FUz, double *Fn);
+/* number of Poisson solves that did not converge since creation */
+int kf_rhs_poisson_failures(kf_rhs r, int *count);
+/* diagnostic: evaluate equidistant grids through the general (stretched-axis)
+ * path, which materialises every derivative as OP_partial_derivative does */
+int kf_rhs_force_general(kf_rhs r, int on);
+
+/* ---------------------------------------------------------------- RK4
+ * Runge_kutta_O4(rhs, dt) (RKO4.hpp:29) and interface_ode_solver::solve
+ * (interface_ODE_solver.hpp:11-12); FP64 state, FP64 accumulation. */
+int kf_rk4_create(kf_rhs r, double dt, kf_rk4 *out);
+int kf_rk4_destroy(kf_rk4 k);
+int kf_rk4_solve(kf_rk4 k, double *FUx, double *FUy, double *FUz, double *Fn);
+/* n steps back to back (no host synchronisation between steps) */
+int kf_rk4_steps(kf_rk4 k, int n, double *FUx, double *FUy, double *FUz, double *Fn);
+int kf_rk4_set_dt(kf_rk4 k, double dt);
+int kf_rk4_time(kf_rk4 k, double *t);
+
+/* cfl_all(dt, FUx, FUy, FUz) (CFL.cpp:29-82) */
+int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
+               double *dt_new);
+
+/* ---------------------------------------------------------------- set-up
+ * Initial state of the kielflow driver (kielflow.cpp:172-193, 227-233) on
+ * the host: seeded ion density n = 1 + amp N(0,1) (Philox4x32-10 +
+ * Box-Muller, i-major / z-fastest), state ln n; grain charge density
+ * fkt3d_Gauss(Q, a) (masks.hpp:351-366); solid mask exp(-(r/a)^2). Host
+ * arrays: ln_n (nreal), nd (nreal), mask (nreal). */
+int kf_setup_host(const kf_axis axes[3], unsigned long long seed, double amp, double Q, double a,
+                  double *ln_n, double *nd, double *mask);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,371 @@
+"""ctypes binding of libkielflow_amd.so (the C ABI in include/kielflow_amd.h).
+
+Host data is numpy; device data lives in DeviceArray buffers allocated by the
+library.  Nothing here computes: every call goes to the HIP library, and
+loading fails loudly when the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libkielflow_amd.so")
+
+KF_OK, KF_ERR_ARG, KF_ERR_HIP, KF_ERR_NOT_CONVERGED, KF_ERR_NAN = 0, 1, 2, 3, 4
+KF_FP64, KF_MIXED, KF_FP32 = 0, 1, 2
+EQST, HYST, SIST = 1, 2, 3
+POISSON_NEWTON_MG, POISSON_NLJ, POISSON_CASCADE = 0, 1, 2
+
+
+class KfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kielflow_amd error {code}: {msg}")
+        self.code = code
+
+
+class Axis(C.Structure):
+    _fields_ = [("type", C.c_int), ("n", C.c_int), ("l0", C.c_double), ("L", C.c_double),
+                ("m", C.c_double)]
+
+
+class Parameters(C.Structure):
+    _fields_ = [("M", C.c_double), ("tau", C.c_double), ("theta", C.c_double),
+                ("mu", C.c_double), ("beta", C.c_double)]
+
+
+class Switches(C.Structure):
+    _fields_ = [("sv_on", C.c_int), ("pen_on", C.c_int)]
+
+
+# name -> (restype, argtypes); every int-returning entry point is checked
+_vp, _dp, _ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+_AX3 = C.POINTER(Axis)
+SIGNATURES = {
+    "kf_last_error": (C.c_char_p, []),
+    "kf_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "kf_ctx_destroy": (C.c_int, [_vp]),
+    "kf_ctx_synchronize": (C.c_int, [_vp]),
+    "kf_ctx_stream": (_vp, [_vp]),
+    "kf_malloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "kf_free": (C.c_int, [_vp, _vp]),
+    "kf_memcpy_h2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "kf_memcpy_d2h": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "kf_memcpy_d2d": (C.c_int, [_vp, _vp, _vp, C.c_size_t]),
+    "kf_memset": (C.c_int, [_vp, _vp, C.c_int, C.c_size_t]),
+    "kf_grid_create": (C.c_int, [_vp, _AX3, C.POINTER(_vp)]),
+    "kf_grid_destroy": (C.c_int, [_vp]),
+    "kf_grid_dims": (C.c_int, [_vp, _ip, _ip, _ip, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]),
+    "kf_axis_values": (C.c_int, [_AX3, C.c_int, _dp, _dp, _dp]),
+    "kf_fft_create": (C.c_int, [_vp, C.c_int, C.POINTER(_vp)]),
+    "kf_fft_destroy": (C.c_int, [_vp]),
+    "kf_fft_r2c": (C.c_int, [_vp, _vp, _vp]),
+    "kf_fft_c2r": (C.c_int, [_vp, _vp, _vp]),
+    "kf_partial_derivative": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "kf_dealias23": (C.c_int, [_vp, _vp]),
+    "kf_filter_phi": (C.c_int, [_vp, _vp]),
+    "kf_poisson_create": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_vp)]),
+    "kf_poisson_destroy": (C.c_int, [_vp]),
+    "kf_poisson_set_tolerance": (C.c_int, [_vp, C.c_double]),
+    "kf_poisson_set_max_iterations": (C.c_int, [_vp, C.c_int]),
+    "kf_poisson_solve": (C.c_int, [_vp, _vp, _vp, _ip, _dp]),
+    "kf_poisson_converged": (C.c_int, [_vp]),
+    "kf_poisson_residual": (C.c_int, [_vp, _vp, _vp, _dp]),
+    "kf_rhs_create": (C.c_int, [_vp, C.POINTER(Parameters), C.POINTER(Switches), _vp, C.c_int,
+                                C.POINTER(_vp)]),
+    "kf_rhs_destroy": (C.c_int, [_vp]),
+    "kf_rhs_set_fields": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "kf_rhs_set_parameters": (C.c_int, [_vp, C.POINTER(Parameters)]),
+    "kf_rhs_solve": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _vp]),
+    "kf_rhs_poisson_failures": (C.c_int, [_vp, _ip]),
+    "kf_rhs_force_general": (C.c_int, [_vp, C.c_int]),
+    "kf_rk4_create": (C.c_int, [_vp, C.c_double, C.POINTER(_vp)]),
+    "kf_rk4_destroy": (C.c_int, [_vp]),
+    "kf_rk4_solve": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "kf_rk4_steps": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "kf_rk4_set_dt": (C.c_int, [_vp, C.c_double]),
+    "kf_rk4_time": (C.c_int, [_vp, _dp]),
+    "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
+    "kf_setup_host": (C.c_int, [_AX3, C.c_ulonglong, C.c_double, C.c_double, C.c_double, _vp, _vp,
+                                _vp]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load the HIP library (built by kielflow_amd.build); raise if missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run `python -m kielflow_amd.build` "
+                              "(the HIP library is the only implementation)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(code: int):
+    if code != KF_OK:
+        raise KfError(code, load().kf_last_error().decode())
+
+
+def axis(type_, n, l0, L, m=0.0) -> Axis:
+    return Axis(type_, n, l0, L, m)
+
+
+def cube(n, L=16.0, type_=EQST, m=0.0):
+    return [axis(type_, n, -0.5 * L, L, m) for _ in range(3)]
+
+
+def _ax3(axes):
+    arr = (Axis * 3)(*axes)
+    return arr
+
+
+class Context:
+    def __init__(self, device=0):
+        self.lib = load()
+        h = _vp()
+        check(self.lib.kf_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.kf_ctx_destroy(self.h)
+            self.h = None
+
+    def sync(self):
+        check(self.lib.kf_ctx_synchronize(self.h))
+
+    @property
+    def stream(self):
+        return self.lib.kf_ctx_stream(self.h)
+
+    def empty(self, shape, dtype) -> "DeviceArray":
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def zeros(self, shape, dtype) -> "DeviceArray":
+        d = DeviceArray(self, shape, dtype)
+        check(self.lib.kf_memset(self.h, d.ptr, 0, d.nbytes))
+        return d
+
+
+class DeviceArray:
+    def __init__(self, ctx: Context, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _vp()
+        check(ctx.lib.kf_malloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes, (a.shape, self.shape)
+        check(self.ctx.lib.kf_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data_as(_vp), self.nbytes))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(self.ctx.lib.kf_memcpy_d2h(self.ctx.h, out.ctypes.data_as(_vp), self.ptr, self.nbytes))
+        return out
+
+    def copy_from(self, other: "DeviceArray"):
+        assert other.nbytes == self.nbytes
+        check(self.ctx.lib.kf_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, self.nbytes))
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            self.ctx.lib.kf_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            if self.ctx.h is not None:
+                self.free()
+        except Exception:
+            pass
+
+
+class Grid:
+    def __init__(self, ctx: Context, axes):
+        self.ctx = ctx
+        self.axes = list(axes)
+        h = _vp()
+        check(ctx.lib.kf_grid_create(ctx.h, _ax3(self.axes), C.byref(h)))
+        self.h = h
+        self.n = tuple(a.n for a in self.axes)
+        self.nzc = self.n[2] // 2 + 1
+        self.rshape = self.n
+        self.cshape = (self.n[0], self.n[1], self.nzc)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.kf_grid_destroy(self.h)
+            self.h = None
+
+    def cfield(self, dtype=np.complex128):
+        return self.ctx.zeros(self.cshape, dtype)
+
+    def rfield(self, dtype=np.float64):
+        return self.ctx.zeros(self.rshape, dtype)
+
+    def deriv(self, d: int, X: DeviceArray, out: DeviceArray):
+        check(self.ctx.lib.kf_partial_derivative(self.h, d, X.ptr, out.ptr))
+
+    def dealias23(self, X: DeviceArray):
+        check(self.ctx.lib.kf_dealias23(self.h, X.ptr))
+
+    def filter_phi(self, X: DeviceArray):
+        check(self.ctx.lib.kf_filter_phi(self.h, X.ptr))
+
+    def cfl(self, dt, FUx, FUy, FUz):
+        out = C.c_double(0)
+        check(self.ctx.lib.kf_cfl_all(self.h, dt, FUx.ptr, FUy.ptr, FUz.ptr, C.byref(out)))
+        return out.value
+
+
+class FFT:
+    def __init__(self, grid: Grid, precision=KF_FP64):
+        self.grid = grid
+        self.precision = precision
+        h = _vp()
+        check(grid.ctx.lib.kf_fft_create(grid.h, precision, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.grid.ctx.lib.kf_fft_destroy(self.h)
+            self.h = None
+
+    def r2c(self, x: DeviceArray, X: DeviceArray):
+        check(self.grid.ctx.lib.kf_fft_r2c(self.h, x.ptr, X.ptr))
+
+    def c2r(self, X: DeviceArray, x: DeviceArray):
+        check(self.grid.ctx.lib.kf_fft_c2r(self.h, X.ptr, x.ptr))
+
+
+class Poisson:
+    def __init__(self, grid: Grid, precision=KF_FP64, mode=POISSON_NEWTON_MG, tol=1e-10, max_it=100):
+        self.grid = grid
+        h = _vp()
+        check(grid.ctx.lib.kf_poisson_create(grid.h, precision, mode, C.byref(h)))
+        self.h = h
+        self.set_tolerance(tol)
+        self.set_max_iterations(max_it)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.grid.ctx.lib.kf_poisson_destroy(self.h)
+            self.h = None
+
+    def set_tolerance(self, t):
+        check(self.grid.ctx.lib.kf_poisson_set_tolerance(self.h, t))
+
+    def set_max_iterations(self, m):
+        check(self.grid.ctx.lib.kf_poisson_set_max_iterations(self.h, m))
+
+    def solve(self, phi: DeviceArray, rho: DeviceArray, raise_on_error=True):
+        it = C.c_int(0)
+        res = C.c_double(0)
+        code = self.grid.ctx.lib.kf_poisson_solve(self.h, phi.ptr, rho.ptr, C.byref(it), C.byref(res))
+        if raise_on_error:
+            check(code)
+        return it.value, res.value, code
+
+    def residual(self, phi: DeviceArray, rho: DeviceArray):
+        res = C.c_double(0)
+        check(self.grid.ctx.lib.kf_poisson_residual(self.h, phi.ptr, rho.ptr, C.byref(res)))
+        return res.value
+
+
+class Rhs:
+    def __init__(self, grid: Grid, solver: Poisson, M=1.0, tau=0.1, theta=100.0, sv_on=False,
+                 pen_on=True, precision=KF_FP64):
+        self.grid = grid
+        self.solver = solver
+        self.params = Parameters(M, tau, theta, 0.0, 0.0)
+        self.switches = Switches(int(sv_on), int(pen_on))
+        h = _vp()
+        check(grid.ctx.lib.kf_rhs_create(grid.h, C.byref(self.params), C.byref(self.switches),
+                                         solver.h, precision, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.grid.ctx.lib.kf_rhs_destroy(self.h)
+            self.h = None
+
+    def set_fields(self, phi=None, nd=None, mask=None):
+        check(self.grid.ctx.lib.kf_rhs_set_fields(
+            self.h, None if phi is None else phi.ptr, None if nd is None else nd.ptr,
+            None if mask is None else mask.ptr))
+
+    def set_parameters(self, M=None, tau=None, theta=None):
+        if M is not None:
+            self.params.M = M
+        if tau is not None:
+            self.params.tau = tau
+        if theta is not None:
+            self.params.theta = theta
+        check(self.grid.ctx.lib.kf_rhs_set_parameters(self.h, C.byref(self.params)))
+
+    def solve(self, t, F):
+        check(self.grid.ctx.lib.kf_rhs_solve(self.h, t, *[f.ptr for f in F]))
+
+    def force_general(self, on=True):
+        check(self.grid.ctx.lib.kf_rhs_force_general(self.h, int(on)))
+
+    def poisson_failures(self):
+        c = C.c_int(0)
+        check(self.grid.ctx.lib.kf_rhs_poisson_failures(self.h, C.byref(c)))
+        return c.value
+
+
+class Rk4:
+    def __init__(self, rhs: Rhs, dt):
+        self.rhs = rhs
+        h = _vp()
+        check(rhs.grid.ctx.lib.kf_rk4_create(rhs.h, dt, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.rhs.grid.ctx.lib.kf_rk4_destroy(self.h)
+            self.h = None
+
+    def step(self, F, n=1):
+        check(self.rhs.grid.ctx.lib.kf_rk4_steps(self.h, n, *[f.ptr for f in F]))
+
+    def set_dt(self, dt):
+        check(self.rhs.grid.ctx.lib.kf_rk4_set_dt(self.h, dt))
+
+    @property
+    def t(self):
+        v = C.c_double(0)
+        check(self.rhs.grid.ctx.lib.kf_rk4_time(self.h, C.byref(v)))
+        return v.value
+
+
+def setup_host(axes, seed=0, amp=1e-3, Q=-0.1, a=1.0):
+    """Initial ln n, grain charge n_d and solid mask on the host (kielflow.cpp:172-229)."""
+    n = [ax.n for ax in axes]
+    ln_n = np.empty(n, np.float64)
+    nd = np.empty(n, np.float64)
+    mask = np.empty(n, np.float64)
+    check(load().kf_setup_host(_ax3(axes), seed, amp, Q, a, ln_n.ctypes.data_as(_vp),
+                               nd.ctypes.data_as(_vp), mask.ctypes.data_as(_vp)))
+    return ln_n, nd, mask

@@ -1,0 +1,660 @@
+// kf_capi.hip -- the extern "C" boundary (include/kielflow_amd.h).
+#include "../../include/kielflow_amd.h"
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "kf_fft.h"
+#include "kf_grid.h"
+#include "kf_poisson.h"
+#include "kf_rhs.h"
+
+using namespace kf;
+
+static thread_local std::string g_err;
+
+struct kf_ctx_s {
+    int device;
+    hipStream_t stream;
+};
+struct kf_grid_s {
+    kf_ctx ctx;
+    Grid *g;
+    fft::Plan<double> *p64;  // FP64 plan for the OP_* operators and CFL
+};
+struct kf_fft_s {
+    kf_grid grid;
+    int precision;
+    fft::Plan<double> p64;
+    fft::Plan<float> p32;
+};
+struct kf_poisson_s {
+    kf_grid grid;
+    Poisson *p;
+    PoissonStats last;
+    double *tmp;
+};
+struct kf_rhs_s {
+    kf_grid grid;
+    int precision;
+    kf_poisson solver;
+    Rhs<double> *r64;
+    Rhs<float> *r32;
+    float2 *conv[4];
+};
+struct kf_rk4_s {
+    kf_rhs rhs;
+    Rk4<double> *k64;
+    Rk4<float> *k32;
+};
+
+template <class F>
+static int guard(F &&f)
+{
+    try {
+        return f();
+    } catch (const kf::Error &e) {
+        g_err = e.msg;
+        return (e.msg.find("hip") != std::string::npos) ? KF_ERR_HIP : KF_ERR_ARG;
+    } catch (const std::exception &e) {
+        g_err = e.what();
+        return KF_ERR_ARG;
+    } catch (...) {
+        g_err = "unknown error";
+        return KF_ERR_ARG;
+    }
+}
+
+#define KF_REQUIRE(cond, msg)            \
+    do {                                 \
+        if (!(cond)) {                   \
+            g_err = (msg);               \
+            return KF_ERR_ARG;           \
+        }                                \
+    } while (0)
+
+static void set_dev(kf_ctx c) { KF_HIP(hipSetDevice(c->device)); }
+
+extern "C" {
+
+const char *kf_last_error(void) { return g_err.c_str(); }
+
+int kf_ctx_create(int device, kf_ctx *out)
+{
+    return guard([&] {
+        KF_REQUIRE(out, "null output");
+        int n = 0;
+        KF_HIP(hipGetDeviceCount(&n));
+        KF_REQUIRE(device >= 0 && device < n, "no such device");
+        KF_HIP(hipSetDevice(device));
+        auto *c = new kf_ctx_s{device, nullptr};
+        KF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c;
+        return KF_OK;
+    });
+}
+
+int kf_ctx_destroy(kf_ctx ctx)
+{
+    return guard([&] {
+        if (!ctx) return KF_OK;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return KF_OK;
+    });
+}
+
+int kf_ctx_synchronize(kf_ctx ctx)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx, "null context");
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        return KF_OK;
+    });
+}
+
+void *kf_ctx_stream(kf_ctx ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int kf_malloc(kf_ctx ctx, size_t bytes, void **out)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx && out, "null argument");
+        set_dev(ctx);
+        KF_HIP(hipMalloc(out, bytes ? bytes : 1));
+        return KF_OK;
+    });
+}
+
+int kf_free(kf_ctx ctx, void *p)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx, "null context");
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        KF_HIP(hipFree(p));
+        return KF_OK;
+    });
+}
+
+int kf_memcpy_h2d(kf_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx, "null context");
+        KF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        return KF_OK;
+    });
+}
+
+int kf_memcpy_d2h(kf_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx, "null context");
+        KF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        KF_HIP(hipStreamSynchronize(ctx->stream));
+        return KF_OK;
+    });
+}
+
+int kf_memcpy_d2d(kf_ctx ctx, void *dst, const void *src, size_t bytes)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx, "null context");
+        KF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        return KF_OK;
+    });
+}
+
+int kf_memset(kf_ctx ctx, void *dst, int value, size_t bytes)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx, "null context");
+        KF_HIP(hipMemsetAsync(dst, value, bytes, ctx->stream));
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ grid
+static AxisDesc to_desc(const kf_axis &a) { return AxisDesc{a.type, a.n, a.l0, a.L, a.m}; }
+
+int kf_grid_create(kf_ctx ctx, const kf_axis axes[3], kf_grid *out)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx && axes && out, "null argument");
+        set_dev(ctx);
+        AxisDesc ad[3] = {to_desc(axes[0]), to_desc(axes[1]), to_desc(axes[2])};
+        KF_REQUIRE(fft::grid_supported(ad[0].n, ad[1].n, ad[2].n),
+                   "grid extents not supported (2^a 3^b <= 256; Nz even)");
+        for (int d = 0; d < 3; ++d) {
+            KF_REQUIRE(ad[d].type >= 1 && ad[d].type <= 3, "unknown axis type");
+            KF_REQUIRE(ad[d].L > 0, "axis length must be positive");
+        }
+        std::unique_ptr<Grid> g(new Grid(ad, ctx->stream));
+        std::unique_ptr<fft::Plan<double>> p(new fft::Plan<double>());
+        fft::plan_init<double>(*p, ad[0].n, ad[1].n, ad[2].n, ctx->stream);
+        *out = new kf_grid_s{ctx, g.release(), p.release()};
+        return KF_OK;
+    });
+}
+
+int kf_grid_destroy(kf_grid g)
+{
+    return guard([&] {
+        if (!g) return KF_OK;
+        (void)hipStreamSynchronize(g->ctx->stream);
+        fft::plan_free<double>(*g->p64);
+        delete g->p64;
+        delete g->g;
+        delete g;
+        return KF_OK;
+    });
+}
+
+int kf_grid_dims(kf_grid g, int *nx, int *ny, int *nz, size_t *nreal, size_t *ncplx)
+{
+    return guard([&] {
+        KF_REQUIRE(g, "null grid");
+        if (nx) *nx = g->g->nx;
+        if (ny) *ny = g->g->ny;
+        if (nz) *nz = g->g->nz;
+        if (nreal) *nreal = (size_t)g->g->nr;
+        if (ncplx) *ncplx = (size_t)g->g->nc;
+        return KF_OK;
+    });
+}
+
+int kf_axis_values(const kf_axis *a, int index, double *val, double *k, double *dS)
+{
+    return guard([&] {
+        KF_REQUIRE(a && a->n > 0, "bad axis");
+        AxisDesc d = to_desc(*a);
+        if (val) *val = axis_val_at(d, index);
+        if (k) *k = axis_k_val_at(d, index);
+        if (dS) *dS = axis_dS(d, index);
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ FFT
+int kf_fft_create(kf_grid g, int precision, kf_fft *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && out, "null argument");
+        KF_REQUIRE(precision == KF_FP64 || precision == KF_FP32, "precision must be KF_FP64 or KF_FP32");
+        set_dev(g->ctx);
+        std::unique_ptr<kf_fft_s> f(new kf_fft_s());
+        f->grid = g;
+        f->precision = precision;
+        if (precision == KF_FP64) fft::plan_init<double>(f->p64, g->g->nx, g->g->ny, g->g->nz, g->ctx->stream);
+        else fft::plan_init<float>(f->p32, g->g->nx, g->g->ny, g->g->nz, g->ctx->stream);
+        *out = f.release();
+        return KF_OK;
+    });
+}
+
+int kf_fft_destroy(kf_fft f)
+{
+    return guard([&] {
+        if (!f) return KF_OK;
+        (void)hipStreamSynchronize(f->grid->ctx->stream);
+        if (f->precision == KF_FP64) fft::plan_free<double>(f->p64);
+        else fft::plan_free<float>(f->p32);
+        delete f;
+        return KF_OK;
+    });
+}
+
+int kf_fft_r2c(kf_fft f, const void *in, void *out)
+{
+    return guard([&] {
+        KF_REQUIRE(f && in && out, "null argument");
+        hipStream_t s = f->grid->ctx->stream;
+        if (f->precision == KF_FP64) fft::r2c<double>(f->p64, (const double *)in, (double2 *)out, s);
+        else fft::r2c<float>(f->p32, (const float *)in, (float2 *)out, s);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_fft_c2r(kf_fft f, const void *in, void *out)
+{
+    return guard([&] {
+        KF_REQUIRE(f && in && out, "null argument");
+        hipStream_t s = f->grid->ctx->stream;
+        if (f->precision == KF_FP64) fft::c2r<double>(f->p64, (const double2 *)in, (double *)out, s);
+        else fft::c2r<float>(f->p32, (const float2 *)in, (float *)out, s);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ spectral ops
+int kf_partial_derivative(kf_grid g, int dir, const double *in, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && in && out, "null argument");
+        KF_REQUIRE(dir >= 0 && dir < 3, "direction must be 0, 1 or 2");
+        op_partial_derivative(*g->g, *g->p64, dir, (const double2 *)in, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_dealias23(kf_grid g, double *inout)
+{
+    return guard([&] {
+        KF_REQUIRE(g && inout, "null argument");
+        op_dealias23(*g->g, (double2 *)inout, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_filter_phi(kf_grid g, double *inout)
+{
+    return guard([&] {
+        KF_REQUIRE(g && inout, "null argument");
+        op_filter_phi(*g->g, (double2 *)inout, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ Poisson
+int kf_poisson_create(kf_grid g, int precision, int mode, kf_poisson *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && out, "null argument");
+        KF_REQUIRE(precision == KF_FP64 || precision == KF_MIXED, "precision must be KF_FP64 or KF_MIXED");
+        KF_REQUIRE(mode >= 0 && mode <= 2, "unknown Poisson mode");
+        KF_REQUIRE(!(precision == KF_MIXED && mode != KF_POISSON_NEWTON_MG),
+                   "mixed precision needs the Newton-MG solver");
+        set_dev(g->ctx);
+        std::unique_ptr<Poisson> p(new Poisson(g->g->ax, precision == KF_MIXED, g->ctx->stream));
+        p->set_mode(mode);
+        *out = new kf_poisson_s{g, p.release(), PoissonStats{0, 0., 0}, nullptr};
+        return KF_OK;
+    });
+}
+
+int kf_poisson_destroy(kf_poisson p)
+{
+    return guard([&] {
+        if (!p) return KF_OK;
+        (void)hipStreamSynchronize(p->grid->ctx->stream);
+        delete p->p;
+        (void)hipFree(p->tmp);
+        delete p;
+        return KF_OK;
+    });
+}
+
+int kf_poisson_set_tolerance(kf_poisson p, double tol)
+{
+    return guard([&] {
+        KF_REQUIRE(p, "null solver");
+        KF_REQUIRE(tol > 0, "tolerance must be positive");
+        p->p->set_tolerance(tol);
+        return KF_OK;
+    });
+}
+
+int kf_poisson_set_max_iterations(kf_poisson p, int iters)
+{
+    return guard([&] {
+        KF_REQUIRE(p, "null solver");
+        p->p->set_max_iterations(iters);
+        return KF_OK;
+    });
+}
+
+int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid)
+{
+    return guard([&] {
+        KF_REQUIRE(p && phi && rho, "null argument");
+        hipStream_t s = p->grid->ctx->stream;
+        int rc = p->p->solve(phi, rho, nullptr, s, true);
+        // Newton-MG: resid = max|F|/max|rho|; reference modes: the last
+        // sweep's max update norm_max (solver_poisson_jacobi_nlin.cpp:204-210)
+        PoissonStats st = p->p->last_stats();
+        p->last = st;
+        if (iters) *iters = st.cycles;
+        if (resid) *resid = st.resid;
+        if (rc == -2) {
+            g_err = "NaN in the potential";
+            return KF_ERR_NAN;
+        }
+        if (p->p->mode() == POISSON_NEWTON_MG && !st.converged) {
+            g_err = "Poisson solve did not converge within the iteration budget";
+            return KF_ERR_NOT_CONVERGED;
+        }
+        return KF_OK;
+    });
+}
+
+int kf_poisson_converged(kf_poisson p) { return p ? p->last.converged : 0; }
+
+int kf_poisson_residual(kf_poisson p, const double *phi, const double *rho, double *resid)
+{
+    return guard([&] {
+        KF_REQUIRE(p && phi && rho && resid, "null argument");
+        hipStream_t s = p->grid->ctx->stream;
+        // a zero-iteration Newton solve on a copy evaluates the residual
+        const long N = p->grid->g->nr;
+        if (!p->tmp) KF_HIP(hipMalloc(&p->tmp, sizeof(double) * N));
+        KF_HIP(hipMemcpyAsync(p->tmp, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+        int mode = p->p->mode(), mi = p->p->max_iterations();
+        double tol = p->p->tolerance();
+        p->p->set_mode(POISSON_NEWTON_MG);
+        p->p->set_max_iterations(0);
+        p->p->set_tolerance(0.0);
+        p->p->solve(p->tmp, rho, nullptr, s, true);
+        PoissonStats st = p->p->last_stats();
+        p->p->set_mode(mode);
+        p->p->set_max_iterations(mi);
+        p->p->set_tolerance(tol);
+        *resid = st.resid;
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ RHS
+int kf_rhs_create(kf_grid g, const kf_parameters *params, const kf_rhs_switches *sw, kf_poisson solver,
+                  int precision, kf_rhs *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && params && sw && solver && out, "null argument");
+        KF_REQUIRE(precision == KF_FP64 || precision == KF_MIXED, "precision must be KF_FP64 or KF_MIXED");
+        KF_REQUIRE(solver->grid->g == g->g, "solver belongs to another grid");
+        KF_REQUIRE(params->theta != 0, "theta must be nonzero");
+        set_dev(g->ctx);
+        Params pa{params->M, params->tau, params->theta, params->mu, params->beta};
+        Switches s{sw->sv_on, sw->pen_on};
+        std::unique_ptr<kf_rhs_s> r(new kf_rhs_s());
+        r->grid = g;
+        r->precision = precision;
+        r->solver = solver;
+        if (precision == KF_FP64) r->r64 = new Rhs<double>(*g->g, *solver->p, pa, s, g->ctx->stream);
+        else {
+            r->r32 = new Rhs<float>(*g->g, *solver->p, pa, s, g->ctx->stream);
+            for (int f = 0; f < 4; ++f) KF_HIP(hipMalloc(&r->conv[f], sizeof(float2) * g->g->nc));
+        }
+        *out = r.release();
+        return KF_OK;
+    });
+}
+
+int kf_rhs_destroy(kf_rhs r)
+{
+    return guard([&] {
+        if (!r) return KF_OK;
+        (void)hipStreamSynchronize(r->grid->ctx->stream);
+        delete r->r64;
+        delete r->r32;
+        for (int f = 0; f < 4; ++f) (void)hipFree(r->conv[f]);
+        delete r;
+        return KF_OK;
+    });
+}
+
+int kf_rhs_set_fields(kf_rhs r, double *phi, const double *nd, const double *mask)
+{
+    return guard([&] {
+        KF_REQUIRE(r, "null rhs");
+        if (r->r64) r->r64->set_fields(phi, nd, mask);
+        else r->r32->set_fields(phi, nd, mask);
+        return KF_OK;
+    });
+}
+
+int kf_rhs_set_parameters(kf_rhs r, const kf_parameters *params)
+{
+    return guard([&] {
+        KF_REQUIRE(r && params, "null argument");
+        Params pa{params->M, params->tau, params->theta, params->mu, params->beta};
+        if (r->r64) r->r64->set_params(pa);
+        else r->r32->set_params(pa);
+        return KF_OK;
+    });
+}
+
+int kf_rhs_solve(kf_rhs r, double t, double *FUx, double *FUy, double *FUz, double *Fn)
+{
+    return guard([&] {
+        KF_REQUIRE(r && FUx && FUy && FUz && Fn, "null argument");
+        hipStream_t s = r->grid->ctx->stream;
+        double2 *F[4] = {(double2 *)FUx, (double2 *)FUy, (double2 *)FUz, (double2 *)Fn};
+        if (r->r64) {
+            r->r64->eval(t, F);
+        } else {
+            const long nc = r->grid->g->nc;
+            for (int f = 0; f < 4; ++f) to_single(F[f], r->conv[f], nc, s);
+            r->r32->eval(t, r->conv);
+            for (int f = 0; f < 4; ++f) to_double(r->conv[f], F[f], nc, s);
+        }
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_rhs_poisson_failures(kf_rhs r, int *count)
+{
+    return guard([&] {
+        KF_REQUIRE(r && count, "null argument");
+        *count = r->r64 ? r->r64->poisson_failures() : r->r32->poisson_failures();
+        return KF_OK;
+    });
+}
+
+int kf_rhs_force_general(kf_rhs r, int on)
+{
+    return guard([&] {
+        KF_REQUIRE(r, "null rhs");
+        if (r->r64) r->r64->set_force_general(on != 0);
+        else r->r32->set_force_general(on != 0);
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ RK4
+int kf_rk4_create(kf_rhs r, double dt, kf_rk4 *out)
+{
+    return guard([&] {
+        KF_REQUIRE(r && out, "null argument");
+        set_dev(r->grid->ctx);
+        std::unique_ptr<kf_rk4_s> k(new kf_rk4_s());
+        k->rhs = r;
+        if (r->r64) k->k64 = new Rk4<double>(*r->r64, dt);
+        else k->k32 = new Rk4<float>(*r->r32, dt);
+        *out = k.release();
+        return KF_OK;
+    });
+}
+
+int kf_rk4_destroy(kf_rk4 k)
+{
+    return guard([&] {
+        if (!k) return KF_OK;
+        (void)hipStreamSynchronize(k->rhs->grid->ctx->stream);
+        delete k->k64;
+        delete k->k32;
+        delete k;
+        return KF_OK;
+    });
+}
+
+int kf_rk4_steps(kf_rk4 k, int n, double *FUx, double *FUy, double *FUz, double *Fn)
+{
+    return guard([&] {
+        KF_REQUIRE(k && FUx && FUy && FUz && Fn, "null argument");
+        double2 *y[4] = {(double2 *)FUx, (double2 *)FUy, (double2 *)FUz, (double2 *)Fn};
+        for (int i = 0; i < n; ++i) {
+            if (k->k64) k->k64->step(y);
+            else k->k32->step(y);
+        }
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_rk4_solve(kf_rk4 k, double *FUx, double *FUy, double *FUz, double *Fn)
+{
+    return kf_rk4_steps(k, 1, FUx, FUy, FUz, Fn);
+}
+
+int kf_rk4_set_dt(kf_rk4 k, double dt)
+{
+    return guard([&] {
+        KF_REQUIRE(k, "null integrator");
+        if (k->k64) k->k64->set_dt(dt);
+        else k->k32->set_dt(dt);
+        return KF_OK;
+    });
+}
+
+int kf_rk4_time(kf_rk4 k, double *t)
+{
+    return guard([&] {
+        KF_REQUIRE(k && t, "null argument");
+        *t = k->k64 ? k->k64->t() : k->k32->t();
+        return KF_OK;
+    });
+}
+
+int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz, double *dt_new)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FUx && FUy && FUz && dt_new, "null argument");
+        const double2 *F[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
+        *dt_new = cfl_all(*g->g, *g->p64, dt, F, g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ set-up (host)
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+static void philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c[1] ^ k0;
+        const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = (uint32_t)p1;
+        c[2] = n2;
+        c[3] = (uint32_t)p0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    std::memcpy(out, c, sizeof(c));
+}
+
+int kf_setup_host(const kf_axis axes[3], unsigned long long seed, double amp, double Q, double a,
+                  double *ln_n, double *nd, double *mask)
+{
+    return guard([&] {
+        KF_REQUIRE(axes, "null axes");
+        AxisDesc ad[3] = {to_desc(axes[0]), to_desc(axes[1]), to_desc(axes[2])};
+        const long nx = ad[0].n, ny = ad[1].n, nz = ad[2].n, N = nx * ny * nz;
+        const double pi = acos(-1.);
+        if (ln_n) {
+            const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+            for (long b = 0; 4 * b < N; ++b) {
+                const uint32_t ctr[4] = {(uint32_t)b, (uint32_t)((uint64_t)b >> 32), 0u, 0u};
+                uint32_t o[4];
+                philox(ctr, key, o);
+                for (int h = 0; h < 2; ++h) {
+                    const double u1 = ((double)o[2 * h] + 0.5) * (1.0 / 4294967296.0);
+                    const double u2 = ((double)o[2 * h + 1] + 0.5) * (1.0 / 4294967296.0);
+                    const double r = sqrt(-2. * log(u1)), th = 2. * pi * u2;
+                    const long i0 = 4 * b + 2 * h;
+                    if (i0 < N) ln_n[i0] = log(1. + amp * (r * cos(th)));
+                    if (i0 + 1 < N) ln_n[i0 + 1] = log(1. + amp * (r * sin(th)));
+                }
+            }
+        }
+        const double A = (Q / (pow(sqrt(pi), 3.) * a * a * a));
+        for (long i = 0; i < nx; ++i) {
+            const double x = axis_val_at(ad[0], (int)i);
+            for (long j = 0; j < ny; ++j) {
+                const double y = axis_val_at(ad[1], (int)j);
+                for (long k = 0; k < nz; ++k) {
+                    const double z = axis_val_at(ad[2], (int)k);
+                    const double e = exp(-pow(x / a, 2.) - pow(y / a, 2.) - pow(z / a, 2.));
+                    const long p = (i * ny + j) * nz + k;
+                    if (nd) nd[p] = A * e;
+                    if (mask) mask[p] = e;
+                }
+            }
+        }
+        return KF_OK;
+    });
+}
+
+}  // extern "C"

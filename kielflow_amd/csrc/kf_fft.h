@@ -1,0 +1,333 @@
+// kf_fft.h -- 3D R2C / C2R transforms on kielflow's field layout, built
+// from three LDS-staged passes:
+//   forward R2C : z (real rows -> half spectrum)  ->  y  ->  x
+//   inverse C2R : x  ->  y  ->  z (half spectrum -> real rows, 1/N scale)
+// The z pass packs a real row of Nz values as Nz/2 complex values and
+// runs an Nz/2-point transform plus the split/merge step; the y and x passes
+// are strided complex transforms over CB columns per workgroup, so every
+// global access is a contiguous run of CB complex values.
+//
+// Pass kernels take load/store functors, which is where the RHS kernels
+// fuse spectral multipliers and real-space products into the transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <vector>
+
+#include "kf_common.h"
+#include "kf_fft_engine.h"
+
+namespace kf {
+namespace fft {
+
+// threads per transform for a length-n transform (about 8 elements/thread)
+__host__ __device__ constexpr int tpt_for(int n)
+{
+    return n <= 8 ? 1 : (n + 7) / 8;
+}
+
+// ------------------------------------------------------------------ functors
+// Strided pass addressing: element t of transform q sits at
+//   (q / qdiv) * qplane + (q % qdiv) + t * tstride
+struct StridedAddr {
+    long qdiv, qplane, tstride;
+    __device__ __forceinline__ long operator()(long q, int t) const
+    {
+        return (q / qdiv) * qplane + (q % qdiv) + (long)t * tstride;
+    }
+};
+
+template <typename R>
+struct LoadPlain {
+    const C2<R> *p;
+    StridedAddr a;
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const { return p[a(q, t)]; }
+};
+
+template <typename R>
+struct StorePlain {
+    C2<R> *p;
+    StridedAddr a;
+    __device__ __forceinline__ void operator()(long q, int t, C2<R> v) const { p[a(q, t)] = v; }
+};
+
+// ------------------------------------------------------------------ kernels
+// Strided complex pass: CB transforms of length N per workgroup.
+// Layout in LDS: [t][c] (ST = CB, SC = 1).
+template <typename R, int N, int CB, int DIR, class LD, class ST_>
+__global__ void __launch_bounds__(tpt_for(N) * CB)
+k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
+{
+    using C = C2<R>;
+    constexpr int TPT = tpt_for(N);
+    constexpr int WG = TPT * CB;
+    __shared__ C lds[N * CB];
+    __shared__ C tw[N];
+    const int tid = threadIdx.x;
+    const long q0 = (long)blockIdx.x * CB;
+    for (int m = tid; m < N; m += WG) tw[m] = tw_g[m];
+    for (int e = tid; e < N * CB; e += WG) {
+        const int c = e % CB, t = e / CB;
+        const long q = q0 + c;
+        lds[t * CB + c] = (q < Q) ? ld(q, t) : mk<C>((R)0, (R)0);
+    }
+    __syncthreads();
+    fft_lds<C, N, TPT, 1, CB, DIR>(lds, tw, tid % CB, tid / CB);
+    for (int e = tid; e < N * CB; e += WG) {
+        const int c = e % CB, t = e / CB;
+        const long q = q0 + c;
+        if (q < Q) st(q, t, lds[t * CB + c]);
+    }
+}
+
+// Real rows -> half-spectrum rows (forward, unnormalised).
+// in: rows of NZ = 2*M reals; out functor gets (row, k, X_k), k = 0..M.
+template <typename R>
+struct RowLoadReal {
+    const R *p;
+    int nz;
+    __device__ __forceinline__ C2<R> operator()(long row, int n) const
+    {
+        return reinterpret_cast<const C2<R> *>(p + row * nz)[n];
+    }
+};
+
+template <typename R>
+struct RowStoreC {
+    C2<R> *p;
+    int nzc;
+    __device__ __forceinline__ void operator()(long row, int k, C2<R> v) const
+    {
+        p[row * nzc + k] = v;
+    }
+};
+
+template <typename R, int M, int RB, class LD, class ST_>
+__global__ void __launch_bounds__(tpt_for(M) * RB)
+k_zr2c(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, LD ld,
+       ST_ st)
+{
+    using C = C2<R>;
+    constexpr int TPT = tpt_for(M);
+    constexpr int WG = TPT * RB;
+    constexpr int SC = M + 1;
+    __shared__ C lds[RB * SC];
+    __shared__ C tw[M];
+    __shared__ C twN[M + 1];  // exp(-2 pi i k / (2M)), k = 0..M
+    const int tid = threadIdx.x;
+    const long r0 = (long)blockIdx.x * RB;
+    for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
+    for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
+    for (int e = tid; e < RB * M; e += WG) {
+        const int c = e / M, n = e % M;
+        const long row = r0 + c;
+        lds[c * SC + n] = (row < rows) ? ld(row, n) : mk<C>((R)0, (R)0);
+    }
+    __syncthreads();
+    fft_lds<C, M, TPT, SC, 1, -1>(lds, tw, tid / TPT, tid % TPT);
+    // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k])
+    for (int e = tid; e < RB * (M + 1); e += WG) {
+        const int c = e / (M + 1), k = e % (M + 1);
+        const long row = r0 + c;
+        if (row < rows) {
+            C zk = lds[c * SC + (k == M ? 0 : k)];
+            C zm = cconj(lds[c * SC + (k == 0 ? 0 : M - k)]);
+            C ev = mk<C>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y + zm.y));
+            C od = mk<C>((R)0.5 * (zk.x - zm.x), (R)0.5 * (zk.y - zm.y));
+            // -i * W^k * od
+            C wod = cmul(twN[k], od);
+            C X = mk<C>(ev.x + wod.y, ev.y - wod.x);
+            st(row, k, X);
+        }
+    }
+}
+
+// Half-spectrum rows -> real rows (inverse, unnormalised; callers scale in
+// the store functor).  Im X_0 and Im X_M are dropped: this is C2R of the
+// Hermitian projection of the kz = 0 and kz = Nz/2 planes once the x and y
+// passes have run.
+template <typename R>
+struct RowLoadC {
+    const C2<R> *p;
+    int nzc;
+    __device__ __forceinline__ C2<R> operator()(long row, int k) const
+    {
+        return p[row * nzc + k];
+    }
+};
+
+template <typename R>
+struct RowStoreReal {
+    R *p;
+    int nz;
+    R scale;
+    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const
+    {
+        reinterpret_cast<C2<R> *>(p + row * nz)[n] = mk<C2<R>>(v.x * scale, v.y * scale);
+    }
+};
+
+template <typename R, int M, int RB, class LD, class ST_>
+__global__ void __launch_bounds__(tpt_for(M) * RB)
+k_zc2r(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, LD ld,
+       ST_ st)
+{
+    using C = C2<R>;
+    constexpr int TPT = tpt_for(M);
+    constexpr int WG = TPT * RB;
+    constexpr int SC = M + 1;
+    __shared__ C xs[RB * SC];
+    __shared__ C lds[RB * SC];
+    __shared__ C tw[M];
+    __shared__ C twN[M + 1];
+    const int tid = threadIdx.x;
+    const long r0 = (long)blockIdx.x * RB;
+    for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
+    for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
+    for (int e = tid; e < RB * (M + 1); e += WG) {
+        const int c = e / (M + 1), k = e % (M + 1);
+        const long row = r0 + c;
+        C v = (row < rows) ? ld(row, k) : mk<C>((R)0, (R)0);
+        if (k == 0 || k == M) v.y = (R)0;
+        xs[c * SC + k] = v;
+    }
+    __syncthreads();
+    // merge: Z[k] = (X[k] + conj X[M-k]) + i conj(W^k) (X[k] - conj X[M-k])
+    for (int e = tid; e < RB * M; e += WG) {
+        const int c = e / M, k = e % M;
+        C xk = xs[c * SC + k];
+        C xm = cconj(xs[c * SC + (M - k)]);
+        C ev = cadd(xk, xm);
+        C od = cmul(cconj(twN[k]), csub(xk, xm));
+        lds[c * SC + k] = mk<C>(ev.x - od.y, ev.y + od.x);
+    }
+    __syncthreads();
+    fft_lds<C, M, TPT, SC, 1, +1>(lds, tw, tid / TPT, tid % TPT);
+    for (int e = tid; e < RB * M; e += WG) {
+        const int c = e / M, n = e % M;
+        const long row = r0 + c;
+        if (row < rows) st(row, n, lds[c * SC + n]);
+    }
+}
+
+// ------------------------------------------------------------------ plan
+// Twiddle tables for the lengths one grid needs, in one precision.
+template <typename R>
+struct Plan {
+    int nx, ny, nz, nzc;
+    long nr, nc;
+    C2<R> *tw_x = nullptr, *tw_y = nullptr, *tw_m = nullptr, *tw_n = nullptr;
+    C2<R> *scratch = nullptr;  // nc complex, for C2R (input is preserved)
+    void *owner_pool = nullptr;
+};
+
+template <typename R>
+void plan_init(Plan<R> &p, int nx, int ny, int nz, hipStream_t s);
+template <typename R>
+void plan_free(Plan<R> &p);
+
+// Supported lengths: x, y in {2,3,4,6,8,12,16,24,32,48,64,96,128,192,256};
+// Nz even with Nz/2 in that set.
+bool length_supported(int n);
+bool grid_supported(int nx, int ny, int nz);
+
+template <typename R>
+void r2c(const Plan<R> &p, const R *in, C2<R> *out, hipStream_t s);
+template <typename R>
+void c2r(const Plan<R> &p, const C2<R> *in, R *out, hipStream_t s);
+
+// ------------------------------------------------------------------ launchers
+// Columns per workgroup for strided passes and rows per workgroup for z
+// passes: aim at 256 (FP64) / 512 (FP32) threads and 32 KiB of LDS.
+template <typename R>
+__host__ __device__ constexpr int cb_for(int n)
+{
+    int v = (sizeof(R) == 4 ? 512 : 256) / tpt_for(n);
+    return v < 8 ? 8 : (v > 64 ? 64 : v);
+}
+
+#define KF_FFT_LENGTHS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
+
+template <typename R, int DIR, class LD, class ST_>
+void launch_strided(int n, long Q, const C2<R> *tw, LD ld, ST_ st, hipStream_t s)
+{
+    switch (n) {
+#define KF_CASE(NN)                                                                       \
+    case NN: {                                                                            \
+        constexpr int CB = cb_for<R>(NN);                                                 \
+        dim3 grid((unsigned)((Q + CB - 1) / CB));                                         \
+        hipLaunchKernelGGL((k_strided<R, NN, CB, DIR, LD, ST_>), grid, dim3(tpt_for(NN) * CB), \
+                           0, s, tw, Q, ld, st);                                          \
+        break;                                                                            \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported transform length " + std::to_string(n));
+    }
+}
+
+template <typename R, class LD, class ST_>
+void launch_zr2c(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST_ st,
+                 hipStream_t s)
+{
+    switch (m) {
+#define KF_CASE(NN)                                                                       \
+    case NN: {                                                                            \
+        constexpr int RB = cb_for<R>(NN);                                                 \
+        dim3 grid((unsigned)((rows + RB - 1) / RB));                                      \
+        hipLaunchKernelGGL((k_zr2c<R, NN, RB, LD, ST_>), grid, dim3(tpt_for(NN) * RB), 0, s, \
+                           twm, twn, rows, ld, st);                                       \
+        break;                                                                            \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported z length " + std::to_string(2 * m));
+    }
+}
+
+// rows per workgroup for the C2R z pass, which stages two row buffers
+template <typename R>
+__host__ __device__ constexpr int rb_c2r_for(int m)
+{
+    int lim = 24576 / ((m + 1) * (int)sizeof(C2<R>));
+    int v = cb_for<R>(m);
+    while (v > lim && v > 1) v /= 2;
+    return v;
+}
+
+template <typename R, class LD, class ST_>
+void launch_zc2r(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST_ st,
+                 hipStream_t s)
+{
+    switch (m) {
+#define KF_CASE(NN)                                                                       \
+    case NN: {                                                                            \
+        constexpr int RB = rb_c2r_for<R>(NN);                                                 \
+        dim3 grid((unsigned)((rows + RB - 1) / RB));                                      \
+        hipLaunchKernelGGL((k_zc2r<R, NN, RB, LD, ST_>), grid, dim3(tpt_for(NN) * RB), 0, s, \
+                           twm, twn, rows, ld, st);                                       \
+        break;                                                                            \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported z length " + std::to_string(2 * m));
+    }
+}
+
+// address helpers for the y and x passes on the [Nx][Ny][Nzc] layout
+inline StridedAddr addr_y(int nx, int ny, int nzc)
+{
+    return StridedAddr{nzc, (long)ny * nzc, nzc};
+}
+inline StridedAddr addr_x(int nx, int ny, int nzc)
+{
+    return StridedAddr{(long)ny * nzc, 0, (long)ny * nzc};
+}
+
+}  // namespace fft
+}  // namespace kf

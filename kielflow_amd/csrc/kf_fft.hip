@@ -1,0 +1,113 @@
+// kf_fft.hip -- plans and plain 3D R2C / C2R drivers (OP_FFT / OP_iFFT
+// semantics: OP_FFT.cpp:49-63 unnormalised forward, OP_iFFT.cpp:49-64 inverse
+// scaled by 1/N; the caller's input is preserved in both directions).
+#include "kf_fft.h"
+
+#include <cmath>
+
+namespace kf {
+namespace fft {
+
+static bool in_set(int n)
+{
+    switch (n) {
+    case 2: case 3: case 4: case 6: case 8: case 12: case 16: case 24: case 32:
+    case 48: case 64: case 96: case 128: case 192: case 256:
+        return true;
+    default:
+        return false;
+    }
+}
+
+bool length_supported(int n) { return in_set(n); }
+
+bool grid_supported(int nx, int ny, int nz)
+{
+    return in_set(nx) && in_set(ny) && nz % 2 == 0 && in_set(nz / 2);
+}
+
+// w[m] = exp(-2 pi i m / n), m = 0..count-1, rounded from extended precision
+template <typename R>
+static C2<R> *twiddles(int n, int count, hipStream_t s)
+{
+    std::vector<C2<R>> h(count);
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int m = 0; m < count; ++m) {
+        long double a = -2.0L * pi * (long double)m / (long double)n;
+        h[m].x = (R)cosl(a);
+        h[m].y = (R)sinl(a);
+    }
+    C2<R> *d = nullptr;
+    KF_HIP(hipMalloc(&d, sizeof(C2<R>) * count));
+    KF_HIP(hipMemcpyAsync(d, h.data(), sizeof(C2<R>) * count, hipMemcpyHostToDevice, s));
+    KF_HIP(hipStreamSynchronize(s));
+    return d;
+}
+
+template <typename R>
+void plan_init(Plan<R> &p, int nx, int ny, int nz, hipStream_t s)
+{
+    if (!grid_supported(nx, ny, nz))
+        throw Error("grid " + std::to_string(nx) + "x" + std::to_string(ny) + "x" +
+                    std::to_string(nz) + " not supported by the FFT (lengths 2^a 3^b <= 256, Nz even)");
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    p.nzc = nz / 2 + 1;
+    p.nr = (long)nx * ny * nz;
+    p.nc = (long)nx * ny * p.nzc;
+    p.tw_x = twiddles<R>(nx, nx, s);
+    p.tw_y = twiddles<R>(ny, ny, s);
+    p.tw_m = twiddles<R>(nz / 2, nz / 2, s);
+    p.tw_n = twiddles<R>(nz, nz / 2 + 1, s);
+    KF_HIP(hipMalloc(&p.scratch, sizeof(C2<R>) * p.nc));
+}
+
+template <typename R>
+void plan_free(Plan<R> &p)
+{
+    (void)hipFree(p.tw_x);
+    (void)hipFree(p.tw_y);
+    (void)hipFree(p.tw_m);
+    (void)hipFree(p.tw_n);
+    (void)hipFree(p.scratch);
+    p.tw_x = p.tw_y = p.tw_m = p.tw_n = p.scratch = nullptr;
+}
+
+template <typename R>
+void r2c(const Plan<R> &p, const R *in, C2<R> *out, hipStream_t s)
+{
+    const long rows = (long)p.nx * p.ny;
+    launch_zr2c<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadReal<R>{in, p.nz},
+                   RowStoreC<R>{out, p.nzc}, s);
+    StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
+    launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{out, ay},
+                          StorePlain<R>{out, ay}, s);
+    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{out, ax},
+                          StorePlain<R>{out, ax}, s);
+}
+
+template <typename R>
+void c2r(const Plan<R> &p, const C2<R> *in, R *out, hipStream_t s)
+{
+    StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
+    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{in, ax},
+                          StorePlain<R>{p.scratch, ax}, s);
+    launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
+                          StorePlain<R>{p.scratch, ay}, s);
+    const long rows = (long)p.nx * p.ny;
+    launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc},
+                   RowStoreReal<R>{out, p.nz, (R)(1.0 / (double)p.nr)}, s);
+}
+
+template void plan_init<float>(Plan<float> &, int, int, int, hipStream_t);
+template void plan_init<double>(Plan<double> &, int, int, int, hipStream_t);
+template void plan_free<float>(Plan<float> &);
+template void plan_free<double>(Plan<double> &);
+template void r2c<float>(const Plan<float> &, const float *, float2 *, hipStream_t);
+template void r2c<double>(const Plan<double> &, const double *, double2 *, hipStream_t);
+template void c2r<float>(const Plan<float> &, const float2 *, float *, hipStream_t);
+template void c2r<double>(const Plan<double> &, const double2 *, double *, hipStream_t);
+
+}  // namespace fft
+}  // namespace kf

@@ -1,0 +1,73 @@
+// kf_grid.h -- a kielflow grid (three axes, grid_base.hpp:9-39) with the
+// per-axis device tables the spectral kernels read.
+#pragma once
+
+#include "kf_common.h"
+#include "kf_poisson.h"
+
+namespace kf {
+
+struct Grid {
+    AxisDesc ax[3];
+    int nx, ny, nz, nzc;
+    long nr, nc;
+    double d23_thr;
+    double *k64[3] = {nullptr, nullptr, nullptr};
+    float *k32[3] = {nullptr, nullptr, nullptr};
+    double *q[3] = {nullptr, nullptr, nullptr};
+    double *qzf = nullptr;
+    // dealiased 1/S'(xi) per stretched axis (OP_partial_derivative.cpp:311-331);
+    // nullptr on equidistant axes
+    double *prof64[3] = {nullptr, nullptr, nullptr};
+    float *prof32[3] = {nullptr, nullptr, nullptr};
+    // penalization barrier cos^2(3 pi x/L) for x < -L/6, a function of the
+    // x index only (rhs_standard.cpp:39-49)
+    double *bar64 = nullptr;
+    float *bar32 = nullptr;
+
+    Grid(const AxisDesc axes[3], hipStream_t s);
+    ~Grid();
+    Grid(const Grid &) = delete;
+    Grid &operator=(const Grid &) = delete;
+
+    bool stretched(int d) const { return ax[d].type != 1; }
+    bool any_stretched() const { return stretched(0) || stretched(1) || stretched(2); }
+
+    template <typename R>
+    GridView<R> view() const
+    {
+        GridView<R> v;
+        v.nx = nx;
+        v.ny = ny;
+        v.nz = nz;
+        v.nzc = nzc;
+        v.nr = nr;
+        v.nc = nc;
+        for (int d = 0; d < 3; ++d) {
+            if constexpr (std::is_same<R, double>::value) v.k[d] = k64[d];
+            else v.k[d] = k32[d];
+            v.q[d] = q[d];
+        }
+        v.qzf = qzf;
+        v.d23_thr = d23_thr;
+        return v;
+    }
+    template <typename R>
+    const R *prof(int d) const
+    {
+        if constexpr (std::is_same<R, double>::value) return prof64[d];
+        else return prof32[d];
+    }
+    template <typename R>
+    const R *bar() const
+    {
+        if constexpr (std::is_same<R, double>::value) return bar64;
+        else return bar32;
+    }
+};
+
+// smallest double t with sqrt(t) >= 2/3, so that for any s,
+// sqrt(s) < 2./3.  <=>  s < t   (sqrt is correctly rounded)
+double d23_threshold();
+
+}  // namespace kf

@@ -1,0 +1,125 @@
+// kf_grid.hip -- host construction of the per-axis device tables.
+#include "kf_grid.h"
+
+#include <cmath>
+#include <vector>
+
+namespace kf {
+
+double d23_threshold()
+{
+    const double c = 2. / 3.;
+    double t = c * c;
+    while (sqrt(t) >= c) t = nextafter(t, 0.);
+    while (!(sqrt(t) >= c)) t = nextafter(t, 1.);
+    return t;
+}
+
+template <typename T>
+static T *upload(const std::vector<T> &h, hipStream_t s)
+{
+    T *d = nullptr;
+    KF_HIP(hipMalloc(&d, sizeof(T) * (h.empty() ? 1 : h.size())));
+    // synchronous: callers pass temporaries
+    if (!h.empty()) KF_HIP(hipMemcpy(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice));
+    (void)s;
+    return d;
+}
+
+Grid::Grid(const AxisDesc axes[3], hipStream_t s)
+{
+    for (int d = 0; d < 3; ++d) {
+        ax[d] = axes[d];
+        if (ax[d].type != 1 && ax[d].type != 2 && ax[d].type != 3) throw Error("unknown axis type");
+        if (ax[d].n < 2) throw Error("axis too short");
+    }
+    nx = ax[0].n;
+    ny = ax[1].n;
+    nz = ax[2].n;
+    nzc = nz / 2 + 1;
+    nr = (long)nx * ny * nz;
+    nc = (long)nx * ny * nzc;
+    d23_thr = d23_threshold();
+    for (int d = 0; d < 3; ++d) {
+        const int len = (d == 2) ? nzc : ax[d].n;
+        std::vector<double> kk(len), qq(len);
+        // k_max = k_val_at(N/2) (OP_dealiasing.hpp:17-19, rhs_standard.cpp:52-54)
+        const double kmax = axis_k_val_at(ax[d], ax[d].n / 2);
+        for (int i = 0; i < len; ++i) {
+            kk[i] = axis_k_val_at(ax[d], i);
+            qq[i] = pow(kk[i] / kmax, 2.);
+        }
+        k64[d] = upload(kk, s);
+        k32[d] = upload(std::vector<float>(kk.begin(), kk.end()), s);
+        q[d] = upload(qq, s);
+        if (d == 2) {
+            // operations.hpp:23: kz_max = k_val_at(Nz_complex/2)
+            const double kzf = axis_k_val_at(ax[2], nzc / 2);
+            std::vector<double> qf(len);
+            for (int i = 0; i < len; ++i) qf[i] = pow(kk[i] / kzf, 2.);
+            qzf = upload(qf, s);
+        }
+        if (ax[d].type != 1) {
+            // dealiased 1/S': DFT of the 1D profile, keep the modes whose
+            // 3D 2/3-rule test passes on the k_d axis line, inverse DFT
+            const int n = ax[d].n;
+            std::vector<long double> f(n), re(n), im(n);
+            for (int i = 0; i < n; ++i) f[i] = 1.0L / (long double)axis_dS(ax[d], i);
+            const long double pi = 3.141592653589793238462643383279502884L;
+            for (int m = 0; m < n; ++m) {
+                long double a = 0, b = 0;
+                for (int i = 0; i < n; ++i) {
+                    long double th = -2.0L * pi * (long double)(((long)m * i) % n) / n;
+                    a += f[i] * cosl(th);
+                    b += f[i] * sinl(th);
+                }
+                re[m] = a;
+                im[m] = b;
+            }
+            std::vector<double> pr(n);
+            for (int i = 0; i < n; ++i) {
+                long double acc = 0;
+                for (int m = 0; m < n; ++m) {
+                    // mode m <-> k index m (negative half included); the
+                    // squared ratio of the other two axes is 0 on this line
+                    double qm = pow(axis_k_val_at(ax[d], m) / kmax, 2.);
+                    double ssum = (d == 0) ? ((qm + 0.) + 0.) : (d == 1 ? ((0. + qm) + 0.) : ((0. + 0.) + qm));
+                    if (!(ssum < d23_thr)) continue;
+                    long double th = 2.0L * pi * (long double)(((long)m * i) % n) / n;
+                    acc += re[m] * cosl(th) - im[m] * sinl(th);
+                }
+                pr[i] = (double)(acc / n);
+            }
+            prof64[d] = upload(pr, s);
+            prof32[d] = upload(std::vector<float>(pr.begin(), pr.end()), s);
+        }
+    }
+    {
+        std::vector<double> b(nx);
+        const double L = ax[0].L;
+        const double pi = acos(-1);
+        for (int i = 0; i < nx; ++i) {
+            double x = axis_val_at(ax[0], i);
+            b[i] = (x < -0.5 * L / 3.) ? pow(cos(2 * (1.5) * pi / L * x), 2.) : 0.;
+        }
+        bar64 = upload(b, s);
+        bar32 = upload(std::vector<float>(b.begin(), b.end()), s);
+    }
+    KF_HIP(hipStreamSynchronize(s));
+}
+
+Grid::~Grid()
+{
+    for (int d = 0; d < 3; ++d) {
+        (void)hipFree(k64[d]);
+        (void)hipFree(k32[d]);
+        (void)hipFree(q[d]);
+        (void)hipFree(prof64[d]);
+        (void)hipFree(prof32[d]);
+    }
+    (void)hipFree(qzf);
+    (void)hipFree(bar64);
+    (void)hipFree(bar32);
+}
+
+}  // namespace kf

@@ -1,0 +1,143 @@
+// kf_poisson.h -- nonlinear Poisson solver for Boltzmann electrons,
+//   L_h phi - exp(phi) + rho = 0,
+// on kielflow's non-uniform 7-point stencil (solver_poisson_jacobi_nlin.cpp:
+// 177-199): L_h phi = 2 sum_nb phi_nb / (h_nb (h_+ + h_-)) - D phi with
+// D = sum_axes 2/(h_+ h_-), h from axis val_at differences (cpp:112-126).
+//
+// Solvers:
+//  * Newton-multigrid (default): outer Newton iterations on F(phi) with the
+//    residual, convergence test  max|F| <= tol * max|rho|  and update in
+//    FP64; the Newton correction J d = -F, J = L_h - diag(exp(phi)), is
+//    solved by linear V-cycles with red-black Gauss-Seidel smoothing in the
+//    sweep precision (FP64, or FP32 in mixed mode: FP64 defect correction).
+//    Kernels check a device-side done flag, so a solve can be enqueued
+//    without host round trips; an exhausted budget is reported as an error.
+//  * the reference iteration: point-Jacobi with damped Newton
+//    (newton_method.hpp:17-63), SOR blend, Carre omega and the update-based
+//    stop test of solver_poisson_jacobi_nlin.cpp:140-265.
+//  * the reference coarse-to-fine cascade (solver_poisson_multigrid.cpp:
+//    246-418) built on that iteration.
+#pragma once
+
+#include <vector>
+
+#include "kf_common.h"
+
+namespace kf {
+
+struct AxisDesc {
+    int type;  // 1 EqSt, 2 HySt, 3 SiSt (IO.cpp:153-164)
+    int n;
+    double l0, L, m;
+};
+
+// axis functions restated on the host (axis_EqSt.cpp:251-283,
+// axis_SiSt.cpp:64-134, axis_HySt.cpp:49-80, axis_FoHySt.cpp:107-117)
+double axis_val_at(const AxisDesc &a, int index);
+double axis_k_val_at(const AxisDesc &a, int index);
+double axis_dS(const AxisDesc &a, int index);
+
+// stencil coefficients of one axis at one level, index i = 0..n-1
+template <typename R>
+struct AxisCoef {
+    const R *cp, *cm, *dd;  // 1/(h+ (h+ + h-)), 1/(h- (h+ + h-)), 2/(h+ h-)
+};
+
+template <typename R>
+struct LevelView {
+    int n[3];
+    long N;
+    AxisCoef<R> c[3];
+};
+
+struct PoissonStats {
+    int cycles;      // Newton iterations, or sweeps of the reference modes
+    double resid;    // max|F| / max|rho|
+    int converged;
+};
+
+enum PoissonMode { POISSON_NEWTON_MG = 0, POISSON_NLJ = 1, POISSON_CASCADE = 2 };
+
+// One solver instance per grid and precision of the sweeps.  FP64 mode
+// smooths in FP64.  Mixed mode smooths in FP32 inside a defect-correction
+// loop whose residual and correction are FP64.
+class Poisson {
+  public:
+    Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s);
+    ~Poisson();
+    Poisson(const Poisson &) = delete;
+    Poisson &operator=(const Poisson &) = delete;
+
+    void set_mode(int mode) { mode_ = mode; }
+    void set_tolerance(double t) { tol_ = t; }
+    void set_max_iterations(int m) { max_it_ = m; }
+    void set_smoothing(int pre, int post, int coarse) { nu1_ = pre; nu2_ = post; nuc_ = coarse; }
+    int mode() const { return mode_; }
+    double tolerance() const { return tol_; }
+    int max_iterations() const { return max_it_; }
+
+    // Linear solve J x = b (J at the phi of the last solve) for the adjoint:
+    // x, b FP64 device fields; cycles V-cycles at most, relative tol.
+    int linear_solve(double *x, const double *b, double rtol, int max_cycles, hipStream_t s);
+
+    // phi: FP64 in/out (warm start), rho: FP64 or FP32 real field.
+    // Enqueues the solve on stream s.  When sync is true the call waits and
+    // fills stats (and returns nonzero if the solve did not converge);
+    // otherwise stats are available from last_stats() after the stream ends.
+    int solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync);
+    PoissonStats last_stats();          // synchronises the stream
+    int *device_fail_counter() { return d_fail_; }
+
+    // reference NLJ state (solver_poisson_jacobi_nlin.hpp:140-171)
+    struct NljState {
+        double norm_max = 0, norm_sum = 0, supremum = 0, infinum = 0;
+        double norm_max_old = -1, norm_sum_old = -1;
+        double limit_max = 1.45e-06, omega_SOR = 1.;
+        int iteration = 0, invocation = 0, max_iterations = 50, converged = 0;
+    };
+    NljState &nlj() { return nlj_; }
+    int nlj_solve_level(int level, double *phi, const double *rho, hipStream_t s);
+
+    int levels() const { return (int)lv_.size(); }
+
+  private:
+    struct Level {
+        int n[3];
+        long N;
+        std::vector<double> h[3];  // H[d][i], i=0..n  (host, for the reference mode)
+        double *c64 = nullptr;     // device: [3 axes][cp, cm, dd][n]
+        float *c32 = nullptr;
+        // linear MG fields: x (correction), b (right side), e (exp(phi) coefficient), r
+        double *x64 = nullptr, *b64 = nullptr, *e64 = nullptr, *r64 = nullptr;
+        float *x32 = nullptr, *b32 = nullptr, *e32 = nullptr, *r32 = nullptr;
+    };
+    template <typename R>
+    LevelView<R> view(int l) const;
+    template <typename R>
+    void vcycle(int l, hipStream_t s);
+    template <typename R>
+    void smooth(int l, int sweeps, hipStream_t s);
+    template <typename R>
+    void newton(double *phi, const double *rho64, const float *rho32, hipStream_t s);
+    int solve_cascade(double *phi, const double *rho, hipStream_t s);
+
+    AxisDesc ax_[3];
+    bool mixed_;
+    int mode_ = POISSON_NEWTON_MG;
+    double tol_ = 1e-10;
+    int max_it_ = 100;
+    int nu1_ = 2, nu2_ = 2, nuc_ = 24, inner_ = 1;
+    std::vector<Level> lv_;
+    // device control block: [0] done flag, [1] cycles, [2] fail counter
+    int *d_ctl_ = nullptr;
+    int *d_fail_ = nullptr;
+    double *d_red_ = nullptr;     // per-block partial reductions
+    double *d_stat_ = nullptr;    // [0] resid, [1] rhomax, ...
+    double *h_stat_ = nullptr;    // pinned mirror
+    int *h_ctl_ = nullptr;
+    double *corr64_ = nullptr;    // FP64 scratch (mixed defect correction)
+    NljState nlj_;
+    hipStream_t s0_;
+};
+
+}  // namespace kf

@@ -1,0 +1,855 @@
+// kf_poisson.hip -- nonlinear Poisson solver (Boltzmann electrons) on the GPU.
+#include "kf_poisson.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace kf {
+
+// ======================================================================== axes
+double axis_val_at(const AxisDesc &a, int index)
+{
+    if (a.type == 1) return a.l0 + index * (a.L / a.n);  // axis_EqSt.cpp:251-254
+    if (a.type == 3) {                                  // axis_SiSt.cpp:64-89
+        int N = a.n, shift = 0;
+        if (index < 0) {
+            do { shift += 1; } while (index + shift * N < 0);
+        }
+        if (index >= N) {
+            do { shift -= 1; } while (index + shift * N >= N - 1);
+        }
+        const double pi = acos(-1.);
+        double x = a.l0 + a.L * (double)(index + shift * N) / (N);
+        return (x - a.m * sin(2. * pi * x / a.L)) - shift * a.L;
+    }
+    // axis_HySt.cpp:49-80
+    double L = a.L;
+    double x_val = a.l0 + L * (double)(index) / (a.n);
+    int shift = 0;
+    if (x_val < -L / 2.) {
+        do { x_val += L; shift -= 1; } while (x_val < -L / 2.);
+    }
+    if (x_val >= L / 2.) {
+        do { x_val -= L; shift += 1; } while (x_val >= a.l0 + L);
+    }
+    return shift * L + (L / 2.) * sinh(2 * (x_val) * a.m / L) / sinh(a.m);
+}
+
+double axis_k_val_at(const AxisDesc &a, int index)
+{
+    // axis_EqSt.cpp:256-260 (also SiSt, and the computational grid of HySt)
+    double pi = acos(-1.);
+    return ((index <= a.n / 2) ? (2. * pi * index / a.L) : (2. * pi * (index - a.n) / a.L));
+}
+
+double axis_dS(const AxisDesc &a, int index)
+{
+    if (a.type == 1) return 1.;
+    if (a.type == 3) {  // axis_SiSt.cpp:129-134
+        const double pi = acos(-1.);
+        double x = a.l0 + (a.L / a.n) * index;
+        return 1. - (2. * pi * a.m / a.L) * cos(2. * pi * x / a.L);
+    }
+    // out_sourced/axis_FoHySt.cpp:107-117
+    double L = a.L, m = a.m;
+    double x_val = a.l0 + L * (double)(index) / (double)(a.n);
+    if (x_val >= L / 2.) return (m / sinh(m)) * cosh(2. * (x_val - L) * m / L);
+    if (x_val < -L / 2.) return (m / sinh(m)) * cosh(2. * (x_val + L) * m / L);
+    return (m / sinh(m)) * cosh(2. * (x_val)*m / L);
+}
+
+// ======================================================================== device helpers
+constexpr int RED_BLOCK = 256;
+
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+template <typename R>
+__device__ __forceinline__ void stencil(const LevelView<R> &L, const R *__restrict__ x, int i, int j,
+                                        int k, R &A, R &D)
+{
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const long sx = (long)ny * nz;
+    const int ip = wrapi(i + 1, nx), im = wrapi(i - 1, nx);
+    const int jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
+    const int kp = wrapi(k + 1, nz), km = wrapi(k - 1, nz);
+    const long row = (long)i * sx + (long)j * nz;
+    R a = L.c[0].cp[i] * x[(long)ip * sx + (long)j * nz + k];
+    a += L.c[0].cm[i] * x[(long)im * sx + (long)j * nz + k];
+    a += L.c[1].cp[j] * x[(long)i * sx + (long)jp * nz + k];
+    a += L.c[1].cm[j] * x[(long)i * sx + (long)jm * nz + k];
+    a += L.c[2].cp[k] * x[row + kp];
+    a += L.c[2].cm[k] * x[row + km];
+    A = (R)2 * a;
+    D = L.c[0].dd[i] + L.c[1].dd[j] + L.c[2].dd[k];
+}
+
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T *sh)
+{
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = sh[tid] > sh[tid + s] ? sh[tid] : sh[tid + s];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_min(T v, T *sh)
+{
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] = sh[tid] < sh[tid + s] ? sh[tid] : sh[tid + s];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh)
+{
+    const int tid = threadIdx.x;
+    sh[tid] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) sh[tid] += sh[tid + s];
+        __syncthreads();
+    }
+    T r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ======================================================================== Newton-MG kernels
+// F = L_h phi - exp(phi) + rho (FP64); b = -F and e = exp(phi) in the sweep
+// precision; per-block max|F| and max|rho|.
+template <typename RIN, typename R>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *__restrict__ rho,
+               R *__restrict__ b, R *__restrict__ e, double *__restrict__ part, const int *ctl)
+{
+    __shared__ double sh[RED_BLOCK];
+    if (ctl[0]) return;
+    const long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x;
+    double fmaxv = 0., rmax = 0.;
+    if (p < L.N) {
+        const int nz = L.n[2], ny = L.n[1];
+        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+        double A, D;
+        stencil<double>(L, phi, i, j, k, A, D);
+        const double x = phi[p];
+        const double ex = exp(x);
+        const double rh = (double)rho[p];
+        const double F = A - D * x - ex + rh;
+        b[p] = (R)(-F);
+        e[p] = (R)ex;
+        fmaxv = fabs(F);
+        rmax = fabs(rh);
+    }
+    fmaxv = block_max(fmaxv, sh);
+    rmax = block_max(rmax, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = fmaxv;
+        part[2 * blockIdx.x + 1] = rmax;
+    }
+}
+
+// ctl: [0] done, [1] iterations, [2] converged; stat: [0] rel resid
+__global__ void __launch_bounds__(RED_BLOCK)
+k_newton_check(const double *__restrict__ part, int nblk, int it, int max_it, double tol, int *ctl,
+               int *fail, double *stat)
+{
+    __shared__ double sh[RED_BLOCK];
+    if (ctl[0]) return;
+    double fm = 0., rm = 0.;
+    for (int b = threadIdx.x; b < nblk; b += RED_BLOCK) {
+        fm = fmax(fm, part[2 * b]);
+        rm = fmax(rm, part[2 * b + 1]);
+    }
+    fm = block_max(fm, sh);
+    rm = block_max(rm, sh);
+    if (threadIdx.x == 0) {
+        double rel = fm / (rm > 0. ? rm : 1.);
+        stat[0] = rel;
+        ctl[1] = it;
+        if (rel <= tol || !(rel == rel)) {
+            ctl[0] = 1;
+            ctl[2] = (rel <= tol) ? 1 : 0;
+            if (!(rel <= tol)) fail[0] += 1;
+        } else if (it >= max_it) {
+            ctl[0] = 1;
+            ctl[2] = 0;
+            fail[0] += 1;
+        }
+    }
+}
+
+template <typename R>
+__global__ void k_fill(R *__restrict__ x, long n, R v, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) x[p] = v;
+}
+
+template <typename R>
+__global__ void k_update(double *__restrict__ phi, const R *__restrict__ x, long n, const int *ctl)
+{
+    if (ctl[0]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) phi[p] += (double)x[p];
+}
+
+// red-black Gauss-Seidel half sweep of  L x - e x = b  (all dims even)
+template <typename R>
+__global__ void k_rb(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
+                     const R *__restrict__ e, int color, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= L.N / 2) return;
+    const int nz2 = L.n[2] / 2, ny = L.n[1];
+    const int kk = (int)(q % nz2), j = (int)((q / nz2) % ny), i = (int)(q / ((long)nz2 * ny));
+    const int k = 2 * kk + ((i + j + color) & 1);
+    R A, D;
+    stencil<R>(L, x, i, j, k, A, D);
+    const long p = ((long)i * ny + j) * L.n[2] + k;
+    x[p] = (A - b[p]) / (D + e[p]);
+}
+
+// weighted Jacobi sweep (levels with an odd extent): out = x + w((A-b)/(D+e) - x)
+template <typename R>
+__global__ void k_jacobi(LevelView<R> L, const R *__restrict__ x, R *__restrict__ out,
+                         const R *__restrict__ b, const R *__restrict__ e, R w, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L.N) return;
+    const int nz = L.n[2], ny = L.n[1];
+    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    R A, D;
+    stencil<R>(L, x, i, j, k, A, D);
+    const R xn = (A - b[p]) / (D + e[p]);
+    out[p] = x[p] + w * (xn - x[p]);
+}
+
+template <typename R>
+__global__ void k_copy(R *__restrict__ dst, const R *__restrict__ src, long n, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) dst[p] = src[p];
+}
+
+// r = b - (L x - e x)
+template <typename R>
+__global__ void k_lin_resid(LevelView<R> L, const R *__restrict__ x, const R *__restrict__ b,
+                            const R *__restrict__ e, R *__restrict__ r, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L.N) return;
+    const int nz = L.n[2], ny = L.n[1];
+    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    R A, D;
+    stencil<R>(L, x, i, j, k, A, D);
+    r[p] = b[p] - (A - D * x[p] - e[p] * x[p]);
+}
+
+// full weighting (1/4, 1/2, 1/4)^3, periodic; fine dims nf, coarse nf/2
+template <typename R>
+__global__ void k_restrict(const R *__restrict__ f, int fx, int fy, int fz, R *__restrict__ c,
+                           const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)cx * cy * cz) return;
+    const int k = (int)(p % cz), j = (int)((p / cz) % cy), i = (int)(p / ((long)cz * cy));
+    const R w[3] = {(R)0.25, (R)0.5, (R)0.25};
+    R acc = 0;
+    for (int a = -1; a <= 1; ++a) {
+        const int fi = wrapi(2 * i + a, fx);
+        for (int bb = -1; bb <= 1; ++bb) {
+            const int fj = wrapi(2 * j + bb, fy);
+            for (int cc = -1; cc <= 1; ++cc) {
+                const int fk = wrapi(2 * k + cc, fz);
+                acc += w[a + 1] * w[bb + 1] * w[cc + 1] * f[((long)fi * fy + fj) * fz + fk];
+            }
+        }
+    }
+    c[p] = acc;
+}
+
+// fine += trilinear(index space) interpolation of coarse
+template <typename R>
+__global__ void k_prolong_add(const R *__restrict__ c, int cx, int cy, int cz, R *__restrict__ f,
+                              const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)fx * fy * fz) return;
+    const int k = (int)(p % fz), j = (int)((p / fz) % fy), i = (int)(p / ((long)fz * fy));
+    const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    const int i1 = wrapi(i0 + 1, cx), j1 = wrapi(j0 + 1, cy), k1 = wrapi(k0 + 1, cz);
+    const R wi1 = (i & 1) ? (R)0.5 : (R)0, wj1 = (j & 1) ? (R)0.5 : (R)0, wk1 = (k & 1) ? (R)0.5 : (R)0;
+    const R wi0 = (R)1 - wi1, wj0 = (R)1 - wj1, wk0 = (R)1 - wk1;
+#define CV(a, b, d) c[((long)(a)*cy + (b)) * cz + (d)]
+    R v = wi0 * (wj0 * (wk0 * CV(i0, j0, k0) + wk1 * CV(i0, j0, k1)) +
+                 wj1 * (wk0 * CV(i0, j1, k0) + wk1 * CV(i0, j1, k1))) +
+          wi1 * (wj0 * (wk0 * CV(i1, j0, k0) + wk1 * CV(i1, j0, k1)) +
+                 wj1 * (wk0 * CV(i1, j1, k0) + wk1 * CV(i1, j1, k1)));
+#undef CV
+    f[p] += v;
+}
+
+// ======================================================================== reference NLJ kernels
+// newton_method::solve (newton_method.hpp:17-63), f = A - D x - e^x
+__device__ __forceinline__ double newton_ref(double x, double A, double D)
+{
+    const double eps = 0.05;
+    const int Nmax = 20;
+    double omega = 1.;
+    const double g = 0.8;
+    double dx = 0.;
+    double dx_alt = (A - D * x - exp(x)) / (-D - exp(x));
+    int i = 0;
+    do {
+        x -= omega * dx;
+        dx = (A - D * x - exp(x)) / (-D - exp(x));
+        omega = fabs(dx) > fabs(dx_alt) ? g * omega : (1. - g) * omega + g * 1.;
+        dx_alt = dx;
+        ++i;
+    } while (fabs(dx) > eps && i < Nmax);
+    return x - dx;
+}
+
+// one iteration_loop sweep (solver_poisson_jacobi_nlin.cpp:170-212); the
+// stencil uses the reference's divisions, in its order
+__global__ void __launch_bounds__(RED_BLOCK)
+k_nlj_ref(int nx, int ny, int nz, const double *__restrict__ HX, const double *__restrict__ HY,
+          const double *__restrict__ HZ, const double *__restrict__ in, double *__restrict__ out,
+          const double *__restrict__ rho, double omega, double *__restrict__ part)
+{
+    __shared__ double sh[RED_BLOCK];
+    const long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x;
+    const long N = (long)nx * ny * nz;
+    double sup = -INFINITY, inf = INFINITY, nmax = 0., nsum = 0.;
+    if (p < N) {
+        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+        const int ip = wrapi(i + 1, nx), im = wrapi(i - 1, nx), jp = wrapi(j + 1, ny),
+                  jm = wrapi(j - 1, ny), kp = wrapi(k + 1, nz), km = wrapi(k - 1, nz);
+#define IN(a, b, c) in[((long)(a)*ny + (b)) * nz + (c)]
+        double D = 2. / (HX[i + 1] * HX[i]);
+        D += 2. / (HY[j + 1] * HY[j]);
+        D += 2. / (HZ[k + 1] * HZ[k]);
+        double A = IN(ip, j, k) / (HX[i + 1] * (HX[i + 1] + HX[i]));
+        A += IN(im, j, k) / (HX[i] * (HX[i + 1] + HX[i]));
+        A += IN(i, jp, k) / (HY[j + 1] * (HY[j + 1] + HY[j]));
+        A += IN(i, jm, k) / (HY[j] * (HY[j + 1] + HY[j]));
+        A += IN(i, j, kp) / (HZ[k + 1] * (HZ[k + 1] + HZ[k]));
+        A += IN(i, j, km) / (HZ[k] * (HZ[k + 1] + HZ[k]));
+#undef IN
+        A = 2. * A;
+        A += rho[p];
+        const double P_old = in[p];
+        const double P_new = newton_ref(P_old, A, D);
+        const double delta = fabs(P_new - P_old);
+        out[p] = (1. - omega) * P_old + omega * P_new;
+        sup = P_new;
+        inf = P_new;
+        nmax = delta;
+        nsum = delta;
+    }
+    sup = block_max(sup, sh);
+    inf = block_min(inf, sh);
+    nmax = block_max(nmax, sh);
+    nsum = block_sum(nsum, sh);
+    if (threadIdx.x == 0) {
+        part[4 * blockIdx.x + 0] = sup;
+        part[4 * blockIdx.x + 1] = inf;
+        part[4 * blockIdx.x + 2] = nmax;
+        part[4 * blockIdx.x + 3] = nsum;
+    }
+}
+
+// OP_8htoh / OP_4htoh / OP_2htoh_lvl0 (field_interpolation.cpp:233-315)
+__global__ void k_inject(const double *__restrict__ in, int fx, int fy, int fz, int f,
+                         double *__restrict__ out)
+{
+    const int cx = fx / f, cy = fy / f, cz = fz / f;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)cx * cy * cz) return;
+    const int k = (int)(p % cz), j = (int)((p / cz) % cy), i = (int)(p / ((long)cz * cy));
+    out[p] = in[((long)(f * i) * fy + f * j) * fz + f * k];
+}
+
+// OP_hto2h (field_interpolation.cpp:56-147), one of its four stages.
+// g[] holds gamma per fine index of the stage's axis (val_at based).
+__global__ void k_hto2h(int stage, const double *__restrict__ in, int cx, int cy, int cz,
+                        double *__restrict__ out, const double *__restrict__ g)
+{
+    const int nx = 2 * cx, ny = 2 * cy, nz = 2 * cz;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)nx * ny * nz) return;
+    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+#define O(a, b, c) out[((long)(a)*ny + (b)) * nz + (c)]
+    if (stage == 0) {
+        if (!(i & 1) && !(j & 1) && !(k & 1)) O(i, j, k) = in[((long)(i / 2) * cy + j / 2) * cz + k / 2];
+    } else if (stage == 1) {
+        if (!(i & 1) && !(j & 1) && (k & 1))
+            O(i, j, k) = (1. - g[k]) * O(i, j, wrapi(k - 1, nz)) + g[k] * O(i, j, wrapi(k + 1, nz));
+    } else if (stage == 2) {
+        if (!(i & 1) && (j & 1))
+            O(i, j, k) = (1. - g[j]) * O(i, wrapi(j - 1, ny), k) + g[j] * O(i, wrapi(j + 1, ny), k);
+    } else {
+        if (i & 1)
+            O(i, j, k) = (1. - g[i]) * O(wrapi(i - 1, nx), j, k) + g[i] * O(wrapi(i + 1, nx), j, k);
+    }
+#undef O
+}
+
+// ======================================================================== host
+static inline unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
+
+Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed), s0_(s)
+{
+    for (int d = 0; d < 3; ++d) ax_[d] = ax[d];
+    int n[3] = {ax[0].n, ax[1].n, ax[2].n};
+    for (int l = 0; l < 16; ++l) {
+        Level L;
+        for (int d = 0; d < 3; ++d) L.n[d] = n[d];
+        L.N = (long)n[0] * n[1] * n[2];
+        std::vector<double> c(9 * (size_t)std::max(n[0], std::max(n[1], n[2])));
+        std::vector<double> h64;
+        std::vector<double> coef;
+        coef.reserve(3 * 3 * 256);
+        for (int d = 0; d < 3; ++d) {
+            AxisDesc a = ax[d];
+            a.n = n[d];
+            L.h[d].resize(n[d] + 1);
+            for (int i = 0; i < n[d] + 1; ++i) L.h[d][i] = axis_val_at(a, i) - axis_val_at(a, i - 1);
+        }
+        size_t off = 0;
+        std::vector<double> hc(3 * (size_t)(n[0] + n[1] + n[2]));
+        for (int d = 0; d < 3; ++d) {
+            const std::vector<double> &H = L.h[d];
+            for (int i = 0; i < n[d]; ++i) {
+                hc[off + i] = 1. / (H[i + 1] * (H[i + 1] + H[i]));
+                hc[off + n[d] + i] = 1. / (H[i] * (H[i + 1] + H[i]));
+                hc[off + 2 * n[d] + i] = 2. / (H[i + 1] * H[i]);
+            }
+            off += 3 * (size_t)n[d];
+        }
+        KF_HIP(hipMalloc(&L.c64, sizeof(double) * hc.size()));
+        KF_HIP(hipMemcpy(L.c64, hc.data(), sizeof(double) * hc.size(), hipMemcpyHostToDevice));
+        std::vector<float> hf(hc.begin(), hc.end());
+        KF_HIP(hipMalloc(&L.c32, sizeof(float) * hf.size()));
+        KF_HIP(hipMemcpy(L.c32, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice));
+        if (mixed_) {
+            KF_HIP(hipMalloc(&L.x32, sizeof(float) * L.N));
+            KF_HIP(hipMalloc(&L.b32, sizeof(float) * L.N));
+            KF_HIP(hipMalloc(&L.e32, sizeof(float) * L.N));
+            KF_HIP(hipMalloc(&L.r32, sizeof(float) * L.N));
+        }
+        KF_HIP(hipMalloc(&L.x64, sizeof(double) * L.N));
+        KF_HIP(hipMalloc(&L.b64, sizeof(double) * L.N));
+        KF_HIP(hipMalloc(&L.e64, sizeof(double) * L.N));
+        KF_HIP(hipMalloc(&L.r64, sizeof(double) * L.N));
+        lv_.push_back(L);
+        bool can = (n[0] % 2 == 0) && (n[1] % 2 == 0) && (n[2] % 2 == 0) && n[0] / 2 >= 3 &&
+                   n[1] / 2 >= 3 && n[2] / 2 >= 3;
+        if (!can) break;
+        n[0] /= 2;
+        n[1] /= 2;
+        n[2] /= 2;
+    }
+    long nb = nblocks(lv_[0].N, RED_BLOCK);
+    KF_HIP(hipMalloc(&d_red_, sizeof(double) * 4 * (nb + 1)));
+    KF_HIP(hipMalloc(&d_ctl_, sizeof(int) * 8));
+    KF_HIP(hipMalloc(&d_fail_, sizeof(int) * 2));
+    KF_HIP(hipMemset(d_fail_, 0, sizeof(int) * 2));
+    KF_HIP(hipMalloc(&d_stat_, sizeof(double) * 4));
+    KF_HIP(hipHostMalloc(&h_stat_, sizeof(double) * 4, hipHostMallocDefault));
+    KF_HIP(hipHostMalloc(&h_ctl_, sizeof(int) * 8, hipHostMallocDefault));
+    KF_HIP(hipDeviceSynchronize());
+}
+
+Poisson::~Poisson()
+{
+    for (Level &L : lv_) {
+        (void)hipFree(L.c64);
+        (void)hipFree(L.c32);
+        (void)hipFree(L.x64);
+        (void)hipFree(L.b64);
+        (void)hipFree(L.e64);
+        (void)hipFree(L.r64);
+        (void)hipFree(L.x32);
+        (void)hipFree(L.b32);
+        (void)hipFree(L.e32);
+        (void)hipFree(L.r32);
+    }
+    (void)hipFree(d_red_);
+    (void)hipFree(d_ctl_);
+    (void)hipFree(d_fail_);
+    (void)hipFree(d_stat_);
+    (void)hipHostFree(h_stat_);
+    (void)hipHostFree(h_ctl_);
+    (void)hipFree(corr64_);
+}
+
+template <typename R>
+LevelView<R> Poisson::view(int l) const
+{
+    const Level &L = lv_[l];
+    LevelView<R> v;
+    for (int d = 0; d < 3; ++d) v.n[d] = L.n[d];
+    v.N = L.N;
+    const R *base = nullptr;
+    if constexpr (std::is_same<R, double>::value) base = L.c64;
+    else base = L.c32;
+    size_t off = 0;
+    for (int d = 0; d < 3; ++d) {
+        v.c[d].cp = base + off;
+        v.c[d].cm = base + off + L.n[d];
+        v.c[d].dd = base + off + 2 * L.n[d];
+        off += 3 * (size_t)L.n[d];
+    }
+    return v;
+}
+
+template <typename R>
+void Poisson::smooth(int l, int sweeps, hipStream_t s)
+{
+    Level &L = lv_[l];
+    LevelView<R> v = view<R>(l);
+    R *x, *b, *e, *r;
+    if constexpr (std::is_same<R, double>::value) { x = L.x64; b = L.b64; e = L.e64; r = L.r64; }
+    else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; }
+    const bool rb = (L.n[0] % 2 == 0) && (L.n[1] % 2 == 0) && (L.n[2] % 2 == 0);
+    for (int it = 0; it < sweeps; ++it) {
+        if (rb) {
+            for (int c = 0; c < 2; ++c)
+                hipLaunchKernelGGL(k_rb<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x, b, e, c,
+                                   (const int *)d_ctl_);
+        } else {
+            hipLaunchKernelGGL(k_jacobi<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x, r,
+                               (const R *)b, (const R *)e, (R)0.8, (const int *)d_ctl_);
+            hipLaunchKernelGGL(k_copy<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, x, (const R *)r, L.N,
+                               (const int *)d_ctl_);
+        }
+    }
+}
+
+template <typename R>
+void Poisson::vcycle(int l, hipStream_t s)
+{
+    const int nl = (int)lv_.size();
+    if (l == nl - 1) {
+        smooth<R>(l, nuc_, s);
+        return;
+    }
+    Level &L = lv_[l], &C = lv_[l + 1];
+    R *x, *b, *e, *r, *cx, *cb;
+    if constexpr (std::is_same<R, double>::value) { x = L.x64; b = L.b64; e = L.e64; r = L.r64; cx = C.x64; cb = C.b64; }
+    else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; cx = C.x32; cb = C.b32; }
+    smooth<R>(l, nu1_, s);
+    LevelView<R> v = view<R>(l);
+    hipLaunchKernelGGL(k_lin_resid<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x,
+                       (const R *)b, (const R *)e, r, (const int *)d_ctl_);
+    hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(C.N, 256)), dim3(256), 0, s, (const R *)r, L.n[0],
+                       L.n[1], L.n[2], cb, (const int *)d_ctl_);
+    hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(C.N, 256)), dim3(256), 0, s, cx, C.N, (R)0,
+                       (const int *)d_ctl_);
+    vcycle<R>(l + 1, s);
+    hipLaunchKernelGGL(k_prolong_add<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, (const R *)cx,
+                       C.n[0], C.n[1], C.n[2], x, (const int *)d_ctl_);
+    smooth<R>(l, nu2_, s);
+}
+
+// Newton iterations with FP64 residual/update and R-precision linear MG
+template <typename R>
+void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipStream_t s)
+{
+    Level &L0 = lv_[0];
+    const long N = L0.N;
+    const unsigned nb = nblocks(N, RED_BLOCK);
+    LevelView<double> v64 = view<double>(0);
+    R *x0, *b0, *e0;
+    if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
+    else { x0 = L0.x32; b0 = L0.b32; e0 = L0.e32; }
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
+    for (int it = 0; it <= max_it_; ++it) {
+        if (rho64)
+            hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nb), dim3(RED_BLOCK), 0, s, v64,
+                               (const double *)phi, rho64, b0, e0, d_red_, (const int *)d_ctl_);
+        else
+            hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nb), dim3(RED_BLOCK), 0, s, v64,
+                               (const double *)phi, rho32, b0, e0, d_red_, (const int *)d_ctl_);
+        hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nb,
+                           it, max_it_, tol_, d_ctl_, d_fail_, d_stat_);
+        if (it == max_it_) break;
+        // host-synchronous early exit: one pinned read per Newton iteration
+        KF_HIP(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+        KF_HIP(hipStreamSynchronize(s));
+        if (h_ctl_[0]) break;
+        // coefficient e on coarse levels
+        for (size_t l = 0; l + 1 < lv_.size(); ++l) {
+            Level &A = lv_[l], &B = lv_[l + 1];
+            const R *ea;
+            R *eb;
+            if constexpr (std::is_same<R, double>::value) { ea = A.e64; eb = B.e64; }
+            else { ea = A.e32; eb = B.e32; }
+            hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1],
+                               A.n[2], eb, (const int *)d_ctl_);
+        }
+        hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0,
+                           (const int *)d_ctl_);
+        for (int c = 0; c < inner_; ++c) vcycle<R>(0, s);
+        hipLaunchKernelGGL(k_update<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const R *)x0, N,
+                           (const int *)d_ctl_);
+    }
+}
+
+int Poisson::solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync)
+{
+    if (mode_ == POISSON_NEWTON_MG) {
+        if (mixed_) newton<float>(phi, rho64, rho32, s);
+        else newton<double>(phi, rho64, rho32, s);
+        if (!sync) return 0;
+        PoissonStats st = last_stats();
+        return st.converged ? 0 : 1;
+    }
+    if (!rho64) throw Error("reference Poisson modes need an FP64 rho");
+    if (mode_ == POISSON_NLJ) {
+        nlj_.max_iterations = max_it_;
+        nlj_.limit_max = tol_;
+        int r = nlj_solve_level(0, phi, rho64, s);
+        return r < 0 ? r : 0;
+    }
+    int r = solve_cascade(phi, rho64, s);
+    return r < 0 ? r : 0;
+}
+
+PoissonStats Poisson::last_stats()
+{
+    PoissonStats st{};
+    if (mode_ != POISSON_NEWTON_MG) {
+        st.cycles = nlj_.iteration;
+        st.converged = nlj_.converged;
+        st.resid = h_stat_[0];
+        return st;
+    }
+    KF_HIP(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s0_));
+    KF_HIP(hipMemcpyAsync(h_stat_, d_stat_, sizeof(double) * 2, hipMemcpyDeviceToHost, s0_));
+    KF_HIP(hipStreamSynchronize(s0_));
+    st.cycles = h_ctl_[1];
+    st.converged = h_ctl_[2];
+    st.resid = h_stat_[0];
+    return st;
+}
+
+// solver_poisson_jacobi_nlin::solve (cpp:50-108) on level l's grid
+int Poisson::nlj_solve_level(int level, double *phi, const double *rho, hipStream_t s)
+{
+    NljState &S = nlj_;
+    int n[3];
+    std::vector<double> H[3];
+    if (level >= 0) {
+        for (int d = 0; d < 3; ++d) { n[d] = lv_[level].n[d]; H[d] = lv_[level].h[d]; }
+    } else {
+        // negative level: -shift below the finest grid (cascade levels)
+        for (int d = 0; d < 3; ++d) {
+            AxisDesc a = ax_[d];
+            a.n = ax_[d].n >> (-level);
+            n[d] = a.n;
+            H[d].resize(a.n + 1);
+            for (int i = 0; i < a.n + 1; ++i) H[d][i] = axis_val_at(a, i) - axis_val_at(a, i - 1);
+        }
+    }
+    const long N = (long)n[0] * n[1] * n[2];
+    S.supremum = 0.;
+    S.infinum = 0.;
+    S.iteration = 0;
+    S.omega_SOR = 1.;
+    S.invocation++;
+    S.converged = 0;
+    S.norm_sum_old = -1.;
+    if (S.max_iterations < 0) S.max_iterations = 1000;
+    double *dH = nullptr, *tmp = nullptr;
+    KF_HIP(hipMalloc(&dH, sizeof(double) * (n[0] + n[1] + n[2] + 3)));
+    KF_HIP(hipMalloc(&tmp, sizeof(double) * N));
+    KF_HIP(hipMemcpyAsync(dH, H[0].data(), sizeof(double) * (n[0] + 1), hipMemcpyHostToDevice, s));
+    KF_HIP(hipMemcpyAsync(dH + n[0] + 1, H[1].data(), sizeof(double) * (n[1] + 1), hipMemcpyHostToDevice, s));
+    KF_HIP(hipMemcpyAsync(dH + n[0] + n[1] + 2, H[2].data(), sizeof(double) * (n[2] + 1),
+                          hipMemcpyHostToDevice, s));
+    const unsigned nb = nblocks(N, RED_BLOCK);
+    std::vector<double> part(4 * (size_t)nb);
+    int rc = 0;
+    while (!S.converged && (S.iteration < S.max_iterations)) {
+        S.iteration++;
+        double in0;
+        KF_HIP(hipMemcpyAsync(&in0, phi, sizeof(double), hipMemcpyDeviceToHost, s));
+        hipLaunchKernelGGL(k_nlj_ref, dim3(nb), dim3(RED_BLOCK), 0, s, n[0], n[1], n[2], (const double *)dH,
+                           (const double *)(dH + n[0] + 1), (const double *)(dH + n[0] + n[1] + 2),
+                           (const double *)phi, tmp, rho, S.omega_SOR, d_red_);
+        KF_HIP(hipMemcpyAsync(part.data(), d_red_, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, s));
+        KF_HIP(hipMemcpyAsync(phi, tmp, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+        KF_HIP(hipStreamSynchronize(s));
+        // iteration_loop bookkeeping (cpp:154-159, 208-259)
+        S.norm_sum_old = S.norm_sum;
+        S.norm_max_old = S.norm_max;
+        double sup = in0, inf = in0, nmax = 0., nsum = 0.;
+        for (unsigned b = 0; b < nb; ++b) {
+            sup = std::max(sup, part[4 * b]);
+            inf = std::min(inf, part[4 * b + 1]);
+            nmax = std::max(nmax, part[4 * b + 2]);
+            nsum += part[4 * b + 3];
+        }
+        S.supremum = sup;
+        S.infinum = inf;
+        S.norm_max = nmax;
+        S.norm_sum = nsum / (double)N;
+        if ((S.norm_max < S.norm_max_old) && (S.iteration > 1)) {
+            double lambda = S.norm_sum / S.norm_sum_old;
+            double w = S.omega_SOR;
+            S.omega_SOR = 2. / (1. + sqrt(1. - pow(lambda + w - 1., 2.) / (lambda * pow(w, 2.))));
+        } else {
+            S.omega_SOR = .5;
+        }
+        if (S.omega_SOR != S.omega_SOR) S.omega_SOR = .5;
+        if (S.norm_max < S.limit_max * (S.supremum - S.infinum)) S.converged = 1;
+        if (!(S.norm_max == S.norm_max)) { rc = -2; break; }  // NaN (cpp:95-99)
+    }
+    (void)hipFree(dH);
+    (void)hipFree(tmp);
+    h_stat_[0] = S.norm_max;
+    return rc < 0 ? rc : S.iteration;
+}
+
+// solver_poisson_multigrid::solve (cpp:246-418) with kielflow.cpp:207-217's
+// cycle {lvl_down3, lvl_up x3}, steps {0,-1,-1,-1}, tol {1e-2,5e-4,1e-4,1e-4}
+int Poisson::solve_cascade(double *phi, const double *rho, hipStream_t s)
+{
+    const int nx = ax_[0].n, ny = ax_[1].n, nz = ax_[2].n;
+    if (nx < 16 || ny < 16 || nz < 16 || nx % 8 || ny % 8 || nz % 8)
+        throw Error("reference MG cascade needs extents that are multiples of 8 and >= 16");
+    const long N = (long)nx * ny * nz;
+    double *a = nullptr, *b = nullptr, *rl = nullptr, *g = nullptr;
+    KF_HIP(hipMalloc(&a, sizeof(double) * N));
+    KF_HIP(hipMalloc(&b, sizeof(double) * N));
+    KF_HIP(hipMalloc(&rl, sizeof(double) * N));
+    KF_HIP(hipMalloc(&g, sizeof(double) * (nx + ny + nz)));
+    const int steps[4] = {0, -1, -1, -1};
+    const double tol[4] = {0.01, 0.0005, 0.0001, 0.0001};
+    int lvl = 3;
+    {
+        long Nc = N / 512;
+        hipLaunchKernelGGL(k_inject, dim3(nblocks(Nc, 256)), dim3(256), 0, s, (const double *)phi, nx, ny, nz,
+                           8, a);
+    }
+    int total = 0, rc = 0;
+    for (int c = 0; c < 4; ++c) {
+        if (c > 0) {
+            // OP_hto2h from level lvl to lvl-1
+            int cx = nx >> lvl, cy = ny >> lvl, cz = nz >> lvl;
+            lvl -= 1;
+            int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
+            std::vector<double> gam(fx + fy + fz, 0.);
+            int ff[3] = {fx, fy, fz};
+            size_t off = 0;
+            for (int d = 0; d < 3; ++d) {
+                AxisDesc A = ax_[d];
+                A.n = ff[d];
+                for (int i = 1; i < ff[d]; i += 2) {
+                    double x1 = axis_val_at(A, i - 1), x2 = axis_val_at(A, i + 1), x = axis_val_at(A, i);
+                    gam[off + i] = (x - x1) / (x2 - x1);
+                }
+                off += ff[d];
+            }
+            KF_HIP(hipMemcpyAsync(g, gam.data(), sizeof(double) * gam.size(), hipMemcpyHostToDevice, s));
+            long Nf = (long)fx * fy * fz;
+            const double *gz = g + fx + fy, *gy = g + fx, *gx = g;
+            hipLaunchKernelGGL(k_hto2h, dim3(nblocks(Nf, 256)), dim3(256), 0, s, 0, (const double *)a, cx, cy,
+                               cz, b, (const double *)nullptr);
+            hipLaunchKernelGGL(k_hto2h, dim3(nblocks(Nf, 256)), dim3(256), 0, s, 1, (const double *)a, cx, cy,
+                               cz, b, gz);
+            hipLaunchKernelGGL(k_hto2h, dim3(nblocks(Nf, 256)), dim3(256), 0, s, 2, (const double *)a, cx, cy,
+                               cz, b, gy);
+            hipLaunchKernelGGL(k_hto2h, dim3(nblocks(Nf, 256)), dim3(256), 0, s, 3, (const double *)a, cx, cy,
+                               cz, b, gx);
+            std::swap(a, b);
+        }
+        // rho on this level: injection from the finest grid
+        const int f = 1 << lvl;
+        long Nl = N >> (3 * lvl);
+        hipLaunchKernelGGL(k_inject, dim3(nblocks(Nl, 256)), dim3(256), 0, s, rho, nx, ny, nz, f, rl);
+        nlj_.max_iterations = steps[c];
+        nlj_.limit_max = tol[c];
+        int it = nlj_solve_level(-lvl, a, rl, s);
+        if (it < 0) { rc = it; break; }
+        total += it;
+    }
+    if (rc == 0) KF_HIP(hipMemcpyAsync(phi, a, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    KF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(a);
+    (void)hipFree(b);
+    (void)hipFree(rl);
+    (void)hipFree(g);
+    return rc < 0 ? rc : total;
+}
+
+int Poisson::linear_solve(double *x, const double *b, double rtol, int max_cycles, hipStream_t s)
+{
+    // J x = b with J = L_h - diag(e), e = exp(phi) left by the last Newton
+    // residual on every level (FP64 coefficients).
+    Level &L0 = lv_[0];
+    const long N = L0.N;
+    const unsigned nb = nblocks(N, RED_BLOCK);
+    (void)nb;
+    KF_HIP(hipMemcpyAsync(L0.b64, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    KF_HIP(hipMemcpyAsync(L0.x64, x, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
+    for (size_t l = 0; l + 1 < lv_.size(); ++l)
+        hipLaunchKernelGGL(k_restrict<double>, dim3(nblocks(lv_[l + 1].N, 256)), dim3(256), 0, s,
+                           (const double *)lv_[l].e64, lv_[l].n[0], lv_[l].n[1], lv_[l].n[2], lv_[l + 1].e64,
+                           (const int *)d_ctl_);
+    std::vector<double> hr(N), hb(N);
+    KF_HIP(hipMemcpyAsync(hb.data(), b, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+    double bmax = 0.;
+    for (long p = 0; p < N; ++p) bmax = std::max(bmax, fabs(hb[p]));
+    if (bmax == 0.) bmax = 1.;
+    int c = 0;
+    for (; c < max_cycles; ++c) {
+        LevelView<double> v = view<double>(0);
+        hipLaunchKernelGGL(k_lin_resid<double>, dim3(nblocks(N, 256)), dim3(256), 0, s, v,
+                           (const double *)L0.x64, (const double *)L0.b64, (const double *)L0.e64, L0.r64,
+                           (const int *)nullptr);
+        KF_HIP(hipMemcpyAsync(hr.data(), L0.r64, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+        KF_HIP(hipStreamSynchronize(s));
+        double rmax = 0.;
+        for (long p = 0; p < N; ++p) rmax = std::max(rmax, fabs(hr[p]));
+        h_stat_[0] = rmax / bmax;
+        if (rmax <= rtol * bmax) break;
+        vcycle<double>(0, s);
+    }
+    KF_HIP(hipMemcpyAsync(x, L0.x64, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    KF_HIP(hipStreamSynchronize(s));
+    return c < max_cycles ? c : -1;
+}
+
+template void Poisson::newton<float>(double *, const double *, const float *, hipStream_t);
+template void Poisson::newton<double>(double *, const double *, const float *, hipStream_t);
+
+}  // namespace kf

@@ -1,0 +1,115 @@
+// kf_rhs.h -- kielflow's right-hand side (rhs_standard::solve,
+// rhs_standard.cpp:89-566) and classic RK4 (Runge_kutta_O4::solve,
+// RKO4.cpp:50-248) on the GPU.
+//
+// State: FUx, FUy, FUz (velocity perturbation) and Fn (spectral ln n), each
+// complex [Nx][Ny][Nz/2+1].  The RHS overwrites its input with d/dt of the
+// state (interface_rhs.hpp:11-12).
+#pragma once
+
+#include "kf_common.h"
+#include "kf_fft.h"
+#include "kf_grid.h"
+#include "kf_poisson.h"
+
+namespace kf {
+
+struct Params {
+    double M, tau, theta, mu, beta;  // parameters.hpp:13-18
+};
+
+struct Switches {
+    int sv_on;   // _RHS_SVISCOSITY  (rhs_standard.hpp:30)
+    int pen_on;  // _RHS_PENALIZATION (rhs_standard.hpp:29)
+};
+
+// RHS in compute precision R (double: FP64 mode; float: mixed mode).
+template <typename R>
+class Rhs {
+  public:
+    Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream_t s);
+    ~Rhs();
+    Rhs(const Rhs &) = delete;
+    Rhs &operator=(const Rhs &) = delete;
+
+    // phi: caller-owned persistent FP64 potential (warm start, in/out);
+    // nd, mask: real FP64 fields copied (and converted) into the RHS.
+    void set_fields(double *phi, const double *nd, const double *mask);
+    // F[0..3]: stage input, overwritten with the RHS (same precision R)
+    void eval(double t, C2<R> *F[4]);
+    void set_force_general(bool v) { force_general_ = v; }
+    void set_params(const Params &p) { pa_ = p; }
+    const Params &params() const { return pa_; }
+    Poisson &poisson() { return ps_; }
+    fft::Plan<R> &plan() { return plan_; }
+    Grid &grid() { return g_; }
+    hipStream_t stream() const { return s_; }
+    int poisson_failures();   // synchronises
+    // real-space fields of the last evaluation (device)
+    const R *U(int b) const { return U_[b]; }
+    const R *lnn() const { return L_; }
+    double *phi() { return phi_; }
+
+  private:
+    void eval_fused(C2<R> *F[4]);
+    void eval_general(C2<R> *F[4]);
+    void poisson_step(C2<R> *Fn);
+    void output(C2<R> *F[4]);
+    void deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accumulate);
+
+    Grid &g_;
+    Poisson &ps_;
+    Params pa_;
+    Switches sw_;
+    hipStream_t s_;
+    fft::Plan<R> plan_;
+    bool force_general_ = false;
+    double *phi_ = nullptr;     // caller-owned
+    R *phiR_ = nullptr;         // FP32 copy (mixed) for the forward transform
+    R *nd_ = nullptr, *mask_ = nullptr;
+    R *U_[3] = {nullptr, nullptr, nullptr};
+    R *L_ = nullptr, *rho_ = nullptr;
+    R *adv_[4] = {nullptr, nullptr, nullptr, nullptr};
+    R *rtmp_ = nullptr;
+    C2<R> *lin_[4] = {nullptr, nullptr, nullptr, nullptr};
+    C2<R> *F1_ = nullptr;
+    C2<R> *dF_[12] = {};
+};
+
+// RK4 with an FP64 state and accumulator; stages evaluated in precision R.
+template <typename R>
+class Rk4 {
+  public:
+    Rk4(Rhs<R> &rhs, double dt);
+    ~Rk4();
+    Rk4(const Rk4 &) = delete;
+    Rk4 &operator=(const Rk4 &) = delete;
+    void step(double2 *y[4]);  // in place
+    double dt() const { return dt_; }
+    void set_dt(double dt) { dt_ = dt; }
+    double t() const { return t_; }
+    void set_t(double t) { t_ = t; }
+    Rhs<R> &rhs() { return rhs_; }
+
+  private:
+    Rhs<R> &rhs_;
+    double dt_, t_ = 0.;
+    C2<R> *S_[4] = {};
+    double2 *acc_[4] = {};
+};
+
+// conversions and small field kernels shared with the C ABI
+void to_single(const double2 *in, float2 *out, long n, hipStream_t s);
+void to_double(const float2 *in, double2 *out, long n, hipStream_t s);
+void real_to_single(const double *in, float *out, long n, hipStream_t s);
+
+// spectral operators on FP64 fields (OP_partial_derivative, OP_dealiasing,
+// dealiasing_undesignated with the Phi filter)
+void op_partial_derivative(Grid &g, fft::Plan<double> &p, int dir, const double2 *in, double2 *out,
+                           hipStream_t s);
+void op_dealias23(Grid &g, double2 *inout, hipStream_t s);
+void op_filter_phi(Grid &g, double2 *inout, hipStream_t s);
+// CFL.cpp:29-82 (perturbation velocity, 0.01 cap)
+double cfl_all(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], hipStream_t s);
+
+}  // namespace kf

@@ -1,0 +1,808 @@
+// kf_rhs.hip -- RHS (rhs_standard.cpp:89-566) and RK4 (RKO4.cpp:50-248).
+//
+// The equidistant-grid RHS fuses every spectral multiplier into the first
+// pass of an inverse transform and every real-space product, exp, and
+// penalization blend into the last pass of the transform that produces it;
+// 25 transforms per evaluation (the reference's 33 minus 8 that repeat an
+// inverse already computed or undo a forward transform).  Stretched axes
+// take the general path, which materialises the stretched derivatives
+// d_b X = FFT(dS_b * iFFT(D23 i k_b X)) as the reference does
+// (OP_partial_derivative.cpp:104-136).
+#include "kf_rhs.h"
+
+#include <cmath>
+#include <vector>
+
+namespace kf {
+
+using namespace fft;
+
+enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN };
+enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN };
+
+// ------------------------------------------------------------------ spectral factors
+template <typename R>
+__device__ __forceinline__ bool keep23(const GridView<R> &g, int i, int j, int k)
+{
+    return d23_keep(g.q[0][i], g.q[1][j], g.q[2][k], g.d23_thr);
+}
+
+// spectral viscosity 0.5 (1 - exp(-8 sqrt(knorm)^8)) (rhs_standard.cpp:68-71)
+template <typename R>
+__device__ __forceinline__ R sv_factor(const GridView<R> &g, int i, int j, int k)
+{
+    const R s = (R)((g.q[0][i] + g.q[1][j]) + g.q[2][k]);
+    return (R)0.5 * ((R)1 - exp(-(R)8 * (s * s)));
+}
+
+// exp(-2000 kabs^15) with the z norm of operations.hpp:23 (rhs_standard.cpp:220)
+template <typename R>
+__device__ __forceinline__ R phi_filter(const GridView<R> &g, int i, int j, int k)
+{
+    const R kab = sqrt((R)((g.q[0][i] + g.q[1][j]) + g.qzf[k]));
+    return exp((R)-2000 * pow(kab, (R)15));
+}
+
+template <typename R>
+__device__ __forceinline__ C2<R> ik(R kv, C2<R> v)
+{
+    return mk<C2<R>>(-kv * v.y, kv * v.x);
+}
+
+template <typename R>
+__device__ __forceinline__ void decode_x(const GridView<R> &g, long q, int t, int &i, int &j, int &k,
+                                         long &idx)
+{
+    i = t;
+    j = (int)(q / g.nzc);
+    k = (int)(q % g.nzc);
+    idx = (long)t * g.ny * g.nzc + q;
+}
+
+// ------------------------------------------------------------------ functors
+// first pass of an inverse transform: v = [D23] [i k_dir] src
+template <typename R>
+struct XLoadSpec {
+    const C2<R> *src;
+    GridView<R> g;
+    int d23, dir;
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const
+    {
+        int i, j, k;
+        long idx;
+        decode_x(g, q, t, i, j, k, idx);
+        C2<R> v = src[idx];
+        if (d23 && !keep23(g, i, j, k)) return mk<C2<R>>((R)0, (R)0);
+        if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+        return v;
+    }
+};
+
+// last pass of an inverse transform (real rows, pairs 2n, 2n+1)
+template <typename R>
+struct ZStore {
+    int kind, nz, ny;
+    R scale;
+    R *o0, *o1;
+    const R *a0, *a1, *a2, *a3;
+    R c0, c1;
+    int flag;
+    const R *prof;
+    int pdir;
+    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const
+    {
+        using C = C2<R>;
+        const R x0 = v.x * scale, x1 = v.y * scale;
+        const long base = row * nz + 2 * n;
+        switch (kind) {
+        case ZK_WRITE:
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
+            break;
+        case ZK_PROF: {
+            R p0, p1;
+            if (pdir == 2) {
+                p0 = prof[2 * n];
+                p1 = prof[2 * n + 1];
+            } else {
+                const int id = (pdir == 1) ? (int)(row % ny) : (int)(row / ny);
+                p0 = p1 = prof[id];
+            }
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0 * p0, x1 * p1);
+            break;
+        }
+        case ZK_RHO: {
+            const C d = *reinterpret_cast<const C *>(a0 + base);
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
+            *reinterpret_cast<C *>(o1 + base) = mk<C>(exp(x0) + d.x, exp(x1) + d.y);
+            break;
+        }
+        case ZK_PROD: {
+            const C u = *reinterpret_cast<const C *>(a0 + base);
+            C prev = flag ? *reinterpret_cast<const C *>(o0 + base) : mk<C>((R)0, (R)0);
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(prev.x + u.x * x0, prev.y + u.y * x1);
+            break;
+        }
+        case ZK_PEN: {
+            // rhs_standard.cpp:483-490: eps1 H (f + c H) + (1-H) B + b f
+            const C f = *reinterpret_cast<const C *>(a0 + base);
+            const C ad = *reinterpret_cast<const C *>(a1 + base);
+            const C H = *reinterpret_cast<const C *>(a2 + base);
+            const R b = a3[row / ny];
+            const R t0 = flag ? c1 * H.x : c1, t1 = flag ? c1 * H.y : c1;
+            R p0 = c0 * H.x * (f.x + t0);
+            R p1 = c0 * H.y * (f.y + t1);
+            p0 += ((R)1 - H.x) * (x0 + ad.x);
+            p1 += ((R)1 - H.y) * (x1 + ad.y);
+            p0 += b * f.x;
+            p1 += b * f.y;
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(p0, p1);
+            break;
+        }
+        }
+    }
+};
+
+// last pass of a forward transform
+template <typename R>
+struct XStore {
+    int kind;
+    GridView<R> g;
+    C2<R> *o0, *o1, *o2, *o3;
+    const C2<R> *a0, *a1, *a2, *a3;
+    R tau, M, theta;
+    int sv_on;
+    __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const
+    {
+        using C = C2<R>;
+        int i, j, k;
+        long idx;
+        decode_x(g, q, t, i, j, k, idx);
+        switch (kind) {
+        case XK_WRITE:
+            o0[idx] = X;
+            break;
+        case XK_ACC: {
+            C o = o0[idx];
+            o0[idx] = mk<C>(o.x + X.x, o.y + X.y);
+            break;
+        }
+        case XK_NEG_D23:
+            o0[idx] = keep23(g, i, j, k) ? mk<C>(-X.x, -X.y) : mk<C>((R)0, (R)0);
+            break;
+        case XK_NEG_ACC: {
+            C l = a0[idx];
+            o0[idx] = mk<C>(-(l.x + X.x), -(l.y + X.y));
+            break;
+        }
+        case XK_F1: {
+            const R f = phi_filter(g, i, j, k);
+            C n = a0[idx];
+            o0[idx] = mk<C>(f * X.x + n.x / theta, f * X.y + n.y / theta);
+            break;
+        }
+        case XK_LIN: {
+            // rhs_standard.cpp:119-140, 219-247, 279-319 on an equidistant grid
+            const R f = phi_filter(g, i, j, k);
+            const C fn = a3[idx];
+            const C F1 = mk<C>(f * X.x + fn.x / theta, f * X.y + fn.y / theta);
+            const bool kp = keep23(g, i, j, k);
+            const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
+            const R kx = g.k[0][i], ky = g.k[1][j], kz = g.k[2][k];
+            const C *FU[3] = {a0, a1, a2};
+            C *LO[3] = {o0, o1, o2};
+            const R ka[3] = {kx, ky, kz};
+            C divu = mk<C>((R)0, (R)0);
+            C dnx;
+            {
+                const C fd = kp ? fn : mk<C>((R)0, (R)0);
+                dnx = ik(kx, fd);
+            }
+            C ln = sv_on ? mk<C>(sv * fn.x, sv * fn.y) : mk<C>((R)0, (R)0);
+            ln = mk<C>(ln.x + M * dnx.x, ln.y + M * dnx.y);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const C u = FU[a][idx];
+                const R fac = sv_on ? (tau + sv) : tau;
+                C l = mk<C>(fac * u.x, fac * u.y);
+                const C dF1 = ik(ka[a], F1);
+                l = mk<C>(l.x + dF1.x, l.y + dF1.y);
+                const C ud = kp ? u : mk<C>((R)0, (R)0);
+                const C dx = ik(kx, ud);
+                l = mk<C>(l.x + M * dx.x, l.y + M * dx.y);
+                LO[a][idx] = l;
+                const C da = ik(ka[a], ud);
+                divu = (a == 0) ? da : mk<C>(divu.x + da.x, divu.y + da.y);
+                if (a == 0) ln = mk<C>(ln.x + da.x, ln.y + da.y);
+                else ln = mk<C>(ln.x + da.x, ln.y + da.y);
+            }
+            (void)divu;
+            o3[idx] = ln;
+            break;
+        }
+        }
+    }
+};
+
+template <typename R, typename RS>
+struct RowLoadConv {
+    const RS *p;
+    int nz;
+    __device__ __forceinline__ C2<R> operator()(long row, int n) const
+    {
+        const C2<RS> v = reinterpret_cast<const C2<RS> *>(p + row * nz)[n];
+        return mk<C2<R>>((R)v.x, (R)v.y);
+    }
+};
+
+// ------------------------------------------------------------------ transforms with functors
+template <typename R>
+static void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, const ZStore<R> &st,
+                     hipStream_t s)
+{
+    GridView<R> gv = g.view<R>();
+    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
+                          StorePlain<R>{p.scratch, ax}, s);
+    launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
+                          StorePlain<R>{p.scratch, ay}, s);
+    launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc}, st, s);
+}
+
+template <typename R, typename RS>
+static void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, hipStream_t s)
+{
+    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    launch_zr2c<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<R, RS>{src, p.nz},
+                   RowStoreC<R>{p.scratch, p.nzc}, s);
+    launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
+                          StorePlain<R>{p.scratch, ay}, s);
+    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s);
+}
+
+template <typename R>
+static ZStore<R> zs(int kind, const Plan<R> &p)
+{
+    ZStore<R> z{};
+    z.kind = kind;
+    z.nz = p.nz;
+    z.ny = p.ny;
+    z.scale = (R)(1.0 / (double)p.nr);
+    return z;
+}
+
+template <typename R>
+static XStore<R> xs(int kind, Grid &g)
+{
+    XStore<R> x{};
+    x.kind = kind;
+    x.g = g.view<R>();
+    return x;
+}
+
+// ------------------------------------------------------------------ pointwise spectral kernels
+static inline unsigned nbk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
+
+// out (+)= [i k_dir] [D23] in [* filter] ; mode: 0 write, 1 accumulate
+template <typename R>
+__global__ void k_spec(GridView<R> g, const C2<R> *__restrict__ in, C2<R> *out, int d23, int dir,
+                       int filt, int acc)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc) return;
+    const int k = (int)(idx % g.nzc), j = (int)((idx / g.nzc) % g.ny), i = (int)(idx / ((long)g.nzc * g.ny));
+    C2<R> v = in[idx];
+    if (d23 && !keep23(g, i, j, k)) v = mk<C2<R>>((R)0, (R)0);
+    if (filt) {
+        const R f = phi_filter(g, i, j, k);
+        v = mk<C2<R>>(v.x * f, v.y * f);
+    }
+    if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+    if (acc) {
+        C2<R> o = out[idx];
+        v = mk<C2<R>>(o.x + v.x, o.y + v.y);
+    }
+    out[idx] = v;
+}
+
+// general path: linear terms of rhs_standard.cpp:119-163
+template <typename R>
+__global__ void k_lin_init(GridView<R> g, const C2<R> *F0, const C2<R> *F1, const C2<R> *F2,
+                           const C2<R> *F3, C2<R> *L0, C2<R> *L1, C2<R> *L2, C2<R> *L3, R tau, int sv_on)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.nc) return;
+    const int k = (int)(idx % g.nzc), j = (int)((idx / g.nzc) % g.ny), i = (int)(idx / ((long)g.nzc * g.ny));
+    const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
+    const R fac = sv_on ? tau + sv : tau;
+    const C2<R> *F[3] = {F0, F1, F2};
+    C2<R> *L[3] = {L0, L1, L2};
+    for (int a = 0; a < 3; ++a) {
+        C2<R> u = F[a][idx];
+        L[a][idx] = mk<C2<R>>(fac * u.x, fac * u.y);
+    }
+    C2<R> n = F3[idx];
+    L3[idx] = sv_on ? mk<C2<R>>(sv * n.x, sv * n.y) : mk<C2<R>>((R)0, (R)0);
+}
+
+// general path: lin_a += M dF_a,x ; lin_n += M dF_n,x + dF_x,x + dF_y,y + dF_z,z
+template <typename R>
+__global__ void k_lin_combine(long nc, C2<R> *L0, C2<R> *L1, C2<R> *L2, C2<R> *L3, const C2<R> *dx0,
+                              const C2<R> *dx1, const C2<R> *dx2, const C2<R> *dnx, const C2<R> *d00,
+                              const C2<R> *d11, const C2<R> *d22, R M)
+{
+    const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nc) return;
+    C2<R> *L[3] = {L0, L1, L2};
+    const C2<R> *D[3] = {dx0, dx1, dx2};
+    for (int a = 0; a < 3; ++a) {
+        C2<R> l = L[a][idx], d = D[a][idx];
+        L[a][idx] = mk<C2<R>>(l.x + M * d.x, l.y + M * d.y);
+    }
+    C2<R> l = L3[idx], d = dnx[idx];
+    l = mk<C2<R>>(l.x + M * d.x, l.y + M * d.y);
+    C2<R> a = d00[idx], b = d11[idx], c = d22[idx];
+    l = mk<C2<R>>(l.x + a.x, l.y + a.y);
+    l = mk<C2<R>>(l.x + b.x, l.y + b.y);
+    l = mk<C2<R>>(l.x + c.x, l.y + c.y);
+    L3[idx] = l;
+}
+
+template <typename A, typename B>
+__global__ void k_convert(const A *__restrict__ in, B *__restrict__ out, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (B)in[i];
+}
+
+template <typename CA, typename CB>
+__global__ void k_convert2(const CA *__restrict__ in, CB *__restrict__ out, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        CA v = in[i];
+        CB w;
+        w.x = v.x;
+        w.y = v.y;
+        out[i] = w;
+    }
+}
+
+void to_single(const double2 *in, float2 *out, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_convert2<double2, float2>), dim3(nbk(n)), dim3(256), 0, s, in, out, n);
+}
+void to_double(const float2 *in, double2 *out, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_convert2<float2, double2>), dim3(nbk(n)), dim3(256), 0, s, in, out, n);
+}
+void real_to_single(const double *in, float *out, long n, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(nbk(n)), dim3(256), 0, s, in, out, n);
+}
+
+// ------------------------------------------------------------------ Rhs
+template <typename R>
+static R *ralloc(long n)
+{
+    R *p = nullptr;
+    KF_HIP(hipMalloc(&p, sizeof(R) * n));
+    return p;
+}
+
+template <typename R>
+Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream_t s)
+    : g_(g), ps_(p), pa_(pa), sw_(sw), s_(s)
+{
+    plan_init<R>(plan_, g.nx, g.ny, g.nz, s);
+    const long nr = g.nr, nc = g.nc;
+    nd_ = ralloc<R>(nr);
+    mask_ = ralloc<R>(nr);
+    for (int b = 0; b < 3; ++b) U_[b] = ralloc<R>(nr);
+    L_ = ralloc<R>(nr);
+    rho_ = ralloc<R>(nr);
+    rtmp_ = ralloc<R>(nr);
+    for (int a = 0; a < 4; ++a) adv_[a] = ralloc<R>(nr);
+    for (int a = 0; a < 4; ++a) lin_[a] = ralloc<C2<R>>(nc);
+    if (!std::is_same<R, double>::value) phiR_ = ralloc<R>(nr);
+    KF_HIP(hipMemsetAsync(nd_, 0, sizeof(R) * nr, s));
+    KF_HIP(hipMemsetAsync(mask_, 0, sizeof(R) * nr, s));
+    KF_HIP(hipStreamSynchronize(s));
+}
+
+template <typename R>
+Rhs<R>::~Rhs()
+{
+    plan_free<R>(plan_);
+    (void)hipFree(nd_);
+    (void)hipFree(mask_);
+    for (int b = 0; b < 3; ++b) (void)hipFree(U_[b]);
+    (void)hipFree(L_);
+    (void)hipFree(rho_);
+    (void)hipFree(rtmp_);
+    for (int a = 0; a < 4; ++a) {
+        (void)hipFree(adv_[a]);
+        (void)hipFree(lin_[a]);
+    }
+    (void)hipFree(F1_);
+    for (auto &d : dF_) (void)hipFree(d);
+    (void)hipFree(phiR_);
+}
+
+template <typename R>
+void Rhs<R>::set_fields(double *phi, const double *nd, const double *mask)
+{
+    if (phi) phi_ = phi;
+    const long nr = g_.nr;
+    if (nd) hipLaunchKernelGGL((k_convert<double, R>), dim3(nbk(nr)), dim3(256), 0, s_, nd, nd_, nr);
+    if (mask) hipLaunchKernelGGL((k_convert<double, R>), dim3(nbk(nr)), dim3(256), 0, s_, mask, mask_, nr);
+    KF_HIP(hipGetLastError());
+}
+
+template <typename R>
+int Rhs<R>::poisson_failures()
+{
+    int f = 0;
+    KF_HIP(hipMemcpyAsync(&f, ps_.device_fail_counter(), sizeof(int), hipMemcpyDeviceToHost, s_));
+    KF_HIP(hipStreamSynchronize(s_));
+    return f;
+}
+
+// rhs_standard.cpp:100-113: rho = exp(iFFT(D23 Fn)) + n_d, then Poisson
+template <typename R>
+void Rhs<R>::poisson_step(C2<R> *Fn)
+{
+    ZStore<R> z = zs<R>(ZK_RHO, plan_);
+    z.o0 = L_;
+    z.o1 = rho_;
+    z.a0 = nd_;
+    c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
+    if (!phi_) throw Error("RHS potential field not set");
+    if constexpr (std::is_same<R, double>::value) ps_.solve(phi_, rho_, nullptr, s_, false);
+    else ps_.solve(phi_, nullptr, rho_, s_, false);
+}
+
+template <typename R>
+void Rhs<R>::output(C2<R> *F[4])
+{
+    if (sw_.pen_on) {
+        // rhs_standard.cpp:459-533
+        for (int a = 0; a < 4; ++a) {
+            ZStore<R> z = zs<R>(ZK_PEN, plan_);
+            z.o0 = adv_[a];
+            z.a0 = (a < 3) ? U_[a] : L_;
+            z.a1 = adv_[a];
+            z.a2 = mask_;
+            z.a3 = g_.bar<R>();
+            z.c0 = (R)2;
+            z.c1 = (a == 0) ? (R)pa_.M : (a == 3 ? (R)10 : (R)0);
+            z.flag = (a == 0);
+            c2r_spec<R>(g_, plan_, lin_[a], 0, -1, z, s_);
+            XStore<R> x = xs<R>(XK_NEG_D23, g_);
+            x.o0 = F[a];
+            r2c_store<R, R>(g_, plan_, adv_[a], x, s_);
+        }
+    } else {
+        for (int a = 0; a < 4; ++a) {
+            XStore<R> x = xs<R>(XK_NEG_ACC, g_);
+            x.o0 = F[a];
+            x.a0 = lin_[a];
+            r2c_store<R, R>(g_, plan_, adv_[a], x, s_);
+        }
+    }
+}
+
+template <typename R>
+void Rhs<R>::eval_fused(C2<R> *F[4])
+{
+    poisson_step(F[3]);
+    // FFT(Phi) -> filter -> + Fn/theta -> all linear terms (4 outputs)
+    XStore<R> x = xs<R>(XK_LIN, g_);
+    x.o0 = lin_[0];
+    x.o1 = lin_[1];
+    x.o2 = lin_[2];
+    x.o3 = lin_[3];
+    x.a0 = F[0];
+    x.a1 = F[1];
+    x.a2 = F[2];
+    x.a3 = F[3];
+    x.tau = (R)pa_.tau;
+    x.M = (R)pa_.M;
+    x.theta = (R)pa_.theta;
+    x.sv_on = sw_.sv_on;
+    r2c_store<R, double>(g_, plan_, phi_, x, s_);
+    // velocities (rhs_standard.cpp:258-260)
+    for (int b = 0; b < 3; ++b) {
+        ZStore<R> z = zs<R>(ZK_WRITE, plan_);
+        z.o0 = U_[b];
+        c2r_spec<R>(g_, plan_, F[b], 1, -1, z, s_);
+    }
+    // advection / continuity products (rhs_standard.cpp:323-343, 403-437)
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 3; ++b) {
+            ZStore<R> z = zs<R>(ZK_PROD, plan_);
+            z.o0 = adv_[a];
+            z.a0 = U_[b];
+            z.flag = (b > 0);
+            c2r_spec<R>(g_, plan_, F[a], 1, b, z, s_);
+        }
+    output(F);
+}
+
+// d_b X of OP_partial_derivative.cpp:104-136 into out (write or accumulate)
+template <typename R>
+void Rhs<R>::deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accumulate)
+{
+    if (!g_.stretched(b)) {
+        hipLaunchKernelGGL(k_spec<R>, dim3(nbk(g_.nc)), dim3(256), 0, s_, g_.view<R>(), X, out, d23_in, b, 0,
+                           accumulate);
+        return;
+    }
+    ZStore<R> z = zs<R>(ZK_PROF, plan_);
+    z.o0 = rtmp_;
+    z.prof = g_.prof<R>(b);
+    z.pdir = b;
+    c2r_spec<R>(g_, plan_, X, 1, b, z, s_);
+    XStore<R> x = xs<R>(accumulate ? XK_ACC : XK_WRITE, g_);
+    x.o0 = out;
+    r2c_store<R, R>(g_, plan_, rtmp_, x, s_);
+}
+
+template <typename R>
+void Rhs<R>::eval_general(C2<R> *F[4])
+{
+    const long nc = g_.nc;
+    if (!F1_) {
+        F1_ = ralloc<C2<R>>(nc);
+        for (auto &d : dF_) d = ralloc<C2<R>>(nc);
+    }
+    poisson_step(F[3]);
+    XStore<R> x = xs<R>(XK_F1, g_);
+    x.o0 = F1_;
+    x.a0 = F[3];
+    x.theta = (R)pa_.theta;
+    r2c_store<R, double>(g_, plan_, phi_, x, s_);
+    hipLaunchKernelGGL(k_lin_init<R>, dim3(nbk(nc)), dim3(256), 0, s_, g_.view<R>(), (const C2<R> *)F[0],
+                       (const C2<R> *)F[1], (const C2<R> *)F[2], (const C2<R> *)F[3], lin_[0], lin_[1],
+                       lin_[2], lin_[3], (R)pa_.tau, sw_.sv_on);
+    for (int a = 0; a < 3; ++a) deriv_into(F1_, 0, a, lin_[a], 1);
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 3; ++b) deriv_into(F[a], 1, b, dF_[3 * a + b], 0);
+    hipLaunchKernelGGL(k_lin_combine<R>, dim3(nbk(nc)), dim3(256), 0, s_, nc, lin_[0], lin_[1], lin_[2],
+                       lin_[3], (const C2<R> *)dF_[0], (const C2<R> *)dF_[3], (const C2<R> *)dF_[6],
+                       (const C2<R> *)dF_[9], (const C2<R> *)dF_[0], (const C2<R> *)dF_[4],
+                       (const C2<R> *)dF_[8], (R)pa_.M);
+    for (int b = 0; b < 3; ++b) {
+        ZStore<R> z = zs<R>(ZK_WRITE, plan_);
+        z.o0 = U_[b];
+        c2r_spec<R>(g_, plan_, F[b], 1, -1, z, s_);
+    }
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 3; ++b) {
+            ZStore<R> z = zs<R>(ZK_PROD, plan_);
+            z.o0 = adv_[a];
+            z.a0 = U_[b];
+            z.flag = (b > 0);
+            c2r_spec<R>(g_, plan_, dF_[3 * a + b], 1, -1, z, s_);
+        }
+    output(F);
+}
+
+template <typename R>
+void Rhs<R>::eval(double t, C2<R> *F[4])
+{
+    (void)t;
+    if (g_.any_stretched() || force_general_) eval_general(F);
+    else eval_fused(F);
+    KF_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ RK4
+template <typename R>
+struct Quad {
+    const double2 *y[4];
+    C2<R> *S[4];
+    double2 *acc[4];
+};
+
+template <typename R>
+__global__ void k_rk_init(Quad<R> q, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    double2 v = q.y[f][i];
+    q.S[f][i] = mk<C2<R>>((R)v.x, (R)v.y);
+}
+
+// after stage s: acc = (s==1 ? 1*k : acc + 2*k); S = y0 + c*k
+template <typename R>
+__global__ void k_rk_stage(Quad<R> q, long n, double c, int first)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    const C2<R> kk = q.S[f][i];
+    const double kx = (double)kk.x, ky = (double)kk.y;
+    double2 a;
+    if (first) {
+        a.x = 1. * kx;
+        a.y = 1. * ky;
+    } else {
+        a = q.acc[f][i];
+        a.x = a.x + 2. * kx;
+        a.y = a.y + 2. * ky;
+    }
+    q.acc[f][i] = a;
+    const double2 y0 = q.y[f][i];
+    q.S[f][i] = mk<C2<R>>((R)(y0.x + c * kx), (R)(y0.y + c * ky));
+}
+
+// y = y0 + (dt/6) (acc + 1*k4)   (RKO4.cpp:225-238)
+template <typename R>
+__global__ void k_rk_final(Quad<R> q, double2 *y0_, double2 *y1_, double2 *y2_, double2 *y3_, long n,
+                           double dt6)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    double2 *Y[4] = {y0_, y1_, y2_, y3_};
+    const C2<R> kk = q.S[f][i];
+    const double2 a = q.acc[f][i];
+    const double2 y0 = Y[f][i];
+    double2 o;
+    o.x = y0.x + dt6 * (a.x + 1. * (double)kk.x);
+    o.y = y0.y + dt6 * (a.y + 1. * (double)kk.y);
+    Y[f][i] = o;
+}
+
+template <typename R>
+Rk4<R>::Rk4(Rhs<R> &rhs, double dt) : rhs_(rhs), dt_(dt)
+{
+    const long nc = rhs.grid().nc;
+    for (int f = 0; f < 4; ++f) {
+        S_[f] = ralloc<C2<R>>(nc);
+        acc_[f] = ralloc<double2>(nc);
+    }
+}
+
+template <typename R>
+Rk4<R>::~Rk4()
+{
+    for (int f = 0; f < 4; ++f) {
+        (void)hipFree(S_[f]);
+        (void)hipFree(acc_[f]);
+    }
+}
+
+// RKO4.cpp:50-248 with an FP64 running sum k1 + 2 k2 + 2 k3
+template <typename R>
+void Rk4<R>::step(double2 *y[4])
+{
+    hipStream_t s = rhs_.stream();
+    const long nc = rhs_.grid().nc;
+    Quad<R> q;
+    for (int f = 0; f < 4; ++f) {
+        q.y[f] = y[f];
+        q.S[f] = S_[f];
+        q.acc[f] = acc_[f];
+    }
+    dim3 grid(nbk(nc), 4);
+    hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
+    const double c[3] = {0.5 * dt_, 0.5 * dt_, 1.0 * dt_};
+    const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
+    for (int st = 0; st < 4; ++st) {
+        rhs_.eval(tt[st], S_);
+        if (st < 3)
+            hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, y[0], y[1], y[2], y[3], nc, dt_ / 6.);
+    KF_HIP(hipGetLastError());
+    t_ += dt_;
+}
+
+// ------------------------------------------------------------------ OP_* on FP64 fields
+void op_partial_derivative(Grid &g, Plan<double> &p, int dir, const double2 *in, double2 *out,
+                           hipStream_t s)
+{
+    if (!g.stretched(dir)) {
+        hipLaunchKernelGGL(k_spec<double>, dim3(nbk(g.nc)), dim3(256), 0, s, g.view<double>(), in, out, 0, dir,
+                           0, 0);
+        return;
+    }
+    double *tmp = nullptr;
+    KF_HIP(hipMalloc(&tmp, sizeof(double) * g.nr));
+    ZStore<double> z = zs<double>(ZK_PROF, p);
+    z.o0 = tmp;
+    z.prof = g.prof<double>(dir);
+    z.pdir = dir;
+    c2r_spec<double>(g, p, in, 1, dir, z, s);
+    XStore<double> x = xs<double>(XK_WRITE, g);
+    x.o0 = out;
+    r2c_store<double, double>(g, p, tmp, x, s);
+    KF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(tmp);
+}
+
+void op_dealias23(Grid &g, double2 *inout, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spec<double>, dim3(nbk(g.nc)), dim3(256), 0, s, g.view<double>(),
+                       (const double2 *)inout, inout, 1, -1, 0, 0);
+}
+
+void op_filter_phi(Grid &g, double2 *inout, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_spec<double>, dim3(nbk(g.nc)), dim3(256), 0, s, g.view<double>(),
+                       (const double2 *)inout, inout, 0, -1, 1, 0);
+}
+
+// CFL.cpp:43-82
+__global__ void k_cfl(const double *__restrict__ Ux, const double *__restrict__ Uy,
+                      const double *__restrict__ Uz, const double *__restrict__ dx, const double *__restrict__ dy,
+                      const double *__restrict__ dz, int nx, int ny, int nz, double dt, double *part)
+{
+    __shared__ double sh[3][256];
+    const long N = (long)nx * ny * nz;
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    double cx = 0., cy = 0., cz = 0.;
+    if (p < N) {
+        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+        cx = dt * fabs(Ux[p]) / dx[i];
+        cy = dt * fabs(Uy[p]) / dy[j];
+        cz = dt * fabs(Uz[p]) / dz[k];
+    }
+    sh[0][threadIdx.x] = cx;
+    sh[1][threadIdx.x] = cy;
+    sh[2][threadIdx.x] = cz;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            for (int c = 0; c < 3; ++c) sh[c][threadIdx.x] = fmax(sh[c][threadIdx.x], sh[c][threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int c = 0; c < 3; ++c) part[3 * blockIdx.x + c] = sh[c][0];
+}
+
+double cfl_all(Grid &g, Plan<double> &p, double dt, const double2 *F[3], hipStream_t s)
+{
+    const long nr = g.nr;
+    double *U = nullptr, *dd = nullptr, *part = nullptr;
+    const unsigned nb = nbk(nr);
+    KF_HIP(hipMalloc(&U, sizeof(double) * 3 * nr));
+    KF_HIP(hipMalloc(&dd, sizeof(double) * (g.nx + g.ny + g.nz)));
+    KF_HIP(hipMalloc(&part, sizeof(double) * 3 * nb));
+    std::vector<double> h(g.nx + g.ny + g.nz);
+    int n3[3] = {g.nx, g.ny, g.nz};
+    size_t off = 0;
+    for (int d = 0; d < 3; ++d) {
+        for (int i = 0; i < n3[d]; ++i) {
+            // get_delta (CFL.cpp:6-12) at i and i+1, smaller of the two
+            double d0 = axis_val_at(g.ax[d], i + 1) - axis_val_at(g.ax[d], i);
+            double d1 = axis_val_at(g.ax[d], i + 2) - axis_val_at(g.ax[d], i + 1);
+            h[off + i] = d0 < d1 ? d0 : d1;
+        }
+        off += n3[d];
+    }
+    KF_HIP(hipMemcpyAsync(dd, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
+    for (int b = 0; b < 3; ++b) c2r<double>(p, F[b], U + b * nr, s);
+    hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, s, (const double *)U, (const double *)(U + nr),
+                       (const double *)(U + 2 * nr), (const double *)dd, (const double *)(dd + g.nx),
+                       (const double *)(dd + g.nx + g.ny), g.nx, g.ny, g.nz, dt, part);
+    std::vector<double> hp(3 * (size_t)nb);
+    KF_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+    (void)hipFree(U);
+    (void)hipFree(dd);
+    (void)hipFree(part);
+    double cmax = 0.;
+    for (double v : hp) cmax = v > cmax ? v : cmax;
+    if (.4 * dt / cmax >= 0.01) return 0.01;
+    return .4 * dt / cmax;
+}
+
+template class Rhs<float>;
+template class Rhs<double>;
+template class Rk4<float>;
+template class Rk4<double>;
+
+}  // namespace kf

@@ -1,0 +1,108 @@
+"""GPU nonlinear Poisson solve against the oracle (same discrete equation)."""
+import numpy as np
+import pytest
+
+from helpers import axes_for, grain_radius, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(kind, n, Q=-0.2):
+    import kielflow_amd as kf
+    axes, oaxes = axes_for(kind, n)
+    a = grain_radius(kind, n)
+    ln_n, nd, mask = kf.setup_host(axes, seed=3, amp=1e-3, Q=Q, a=a)
+    rho = np.exp(ln_n) + nd
+    return axes, oaxes, rho
+
+
+@pytest.mark.parametrize("kind,n", [("eqst", 32), ("eqst", 48), ("sist", 32), ("hyst", 24)])
+def test_newton_mg_fp64_matches_oracle(ctx, oracle, kind, n):
+    import kielflow_amd as kf
+    print("poisson fp64", kind, n, flush=True)
+    axes, oaxes, rho = _problem(kind, n)
+    og = oracle.Grid(*oaxes)
+    ref, it, res = og.poisson_fas(np.zeros(og.rshape), rho, 1e-13, 400)
+    assert it >= 0
+    g = kf.Grid(ctx, axes)
+    p = kf.Poisson(g, kf.KF_FP64, tol=1e-13, max_it=60)
+    dphi = g.rfield()
+    drho = ctx.to_device(rho)
+    iters, resid, code = p.solve(dphi, drho)
+    assert resid <= 1e-13
+    phi = dphi.get()
+    assert og.poisson_residual(phi, rho)[0] < 1e-12
+    assert np.abs(phi - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+    # warm start converges in fewer iterations
+    it2, res2, _ = p.solve(dphi, drho)
+    assert it2 <= 1
+    p.close()
+    g.close()
+
+
+def test_newton_mg_mixed_precision(ctx, oracle):
+    import kielflow_amd as kf
+    print("poisson mixed", flush=True)
+    axes, oaxes, rho = _problem("eqst", 64, Q=-0.3)
+    g = kf.Grid(ctx, axes)
+    p64 = kf.Poisson(g, kf.KF_FP64, tol=1e-12, max_it=60)
+    pm = kf.Poisson(g, kf.KF_MIXED, tol=1e-10, max_it=60)
+    drho = ctx.to_device(rho)
+    a, b = g.rfield(), g.rfield()
+    p64.solve(a, drho)
+    it, res, code = pm.solve(b, drho)
+    assert res <= 1e-10
+    assert np.abs(a.get() - b.get()).max() < 1e-8 * np.abs(a.get()).max()
+
+
+def test_not_converged_is_an_error(ctx):
+    import kielflow_amd as kf
+    axes, oaxes, rho = _problem("eqst", 16)
+    g = kf.Grid(ctx, axes)
+    p = kf.Poisson(g, kf.KF_FP64, tol=1e-15, max_it=1)
+    it, res, code = p.solve(g.rfield(), ctx.to_device(rho), raise_on_error=False)
+    assert code == kf.api.KF_ERR_NOT_CONVERGED
+
+
+@pytest.mark.parametrize("iters", [1, 3, 25])
+def test_reference_nlj_iterates_match_oracle(ctx, oracle, iters):
+    """The reference iteration (Jacobi + damped Newton + Carre SOR), iterate by
+    iterate, at 16^3."""
+    import kielflow_amd as kf
+    print("nlj", iters, flush=True)
+    axes, oaxes, rho = _problem("sist", 16)
+    og = oracle.Grid(*oaxes)
+    ref, it_o, st = og.nlj_solve(np.zeros(og.rshape), rho, max_iter=iters, tol=1e-30)
+    g = kf.Grid(ctx, axes)
+    p = kf.Poisson(g, kf.KF_FP64, mode=kf.POISSON_NLJ, tol=1e-30, max_it=iters)
+    dphi = g.rfield()
+    it, res, code = p.solve(dphi, ctx.to_device(rho))
+    assert it == it_o == iters
+    assert np.abs(dphi.get() - ref).max() < 1e-13
+
+
+def test_reference_nlj_converged_count(ctx, oracle):
+    import kielflow_amd as kf
+    axes, oaxes, rho = _problem("eqst", 16)
+    og = oracle.Grid(*oaxes)
+    ref, it_o, st = og.nlj_solve(np.zeros(og.rshape), rho, max_iter=500, tol=1e-6)
+    g = kf.Grid(ctx, axes)
+    p = kf.Poisson(g, kf.KF_FP64, mode=kf.POISSON_NLJ, tol=1e-6, max_it=500)
+    dphi = g.rfield()
+    it, res, code = p.solve(dphi, ctx.to_device(rho))
+    assert st.converged == 1 and it == it_o
+    assert np.abs(dphi.get() - ref).max() < 1e-12
+
+
+def test_reference_cascade_matches_oracle(ctx, oracle):
+    import kielflow_amd as kf
+    print("cascade", flush=True)
+    axes, oaxes, rho = _problem("sist", 32)
+    og = oracle.Grid(*oaxes)
+    phi0 = np.zeros(og.rshape)
+    ref, it_o, st = og.mg_cascade(phi0, rho)
+    g = kf.Grid(ctx, axes)
+    p = kf.Poisson(g, kf.KF_FP64, mode=kf.POISSON_CASCADE)
+    dphi = g.rfield()
+    it, res, code = p.solve(dphi, ctx.to_device(rho))
+    assert np.abs(dphi.get() - ref).max() < 1e-11

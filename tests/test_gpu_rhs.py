@@ -1,0 +1,146 @@
+"""GPU RHS (rhs_standard::solve) and RK4 (Runge_kutta_O4::solve) against the
+oracle on the same seeded inputs.  FP64: per-field rel-L2 <= 1e-11;
+mixed precision: <= 1e-5."""
+import numpy as np
+import pytest
+
+from helpers import initial_state, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-11
+MIXED_TOL = 1e-5
+
+
+def _perturb(og, state, seed=5, amp=1e-2):
+    """Add a seeded band-limited velocity so every term of the RHS acts."""
+    rng = np.random.default_rng(seed)
+    out = [s.copy() for s in state]
+    for a in range(3):
+        u = amp * rng.standard_normal(og.rshape)
+        out[a] = og.dealias23(og.r2c(u))
+    return out
+
+
+def _build(ctx, oracle, kind, n, sv, pen, M, Q, prec, general=False, seed=0):
+    import kielflow_amd as kf
+    state, nd, mask, og = initial_state(kind, n, seed=seed, Q=Q)
+    state = _perturb(og, state)
+    g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+    ps = kf.Poisson(g, kf.KF_FP64 if prec == "fp64" else kf.KF_MIXED, tol=1e-13 if prec == "fp64" else 1e-10,
+                    max_it=80)
+    rhs = kf.Rhs(g, ps, M=M, tau=0.1, theta=100.0, sv_on=sv, pen_on=pen,
+                 precision=kf.KF_FP64 if prec == "fp64" else kf.KF_MIXED)
+    if general:
+        rhs.force_general(True)
+    phi = g.rfield()
+    rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+    orhs = oracle.Rhs(og, M=M, tau=0.1, theta=100.0, sv_on=sv, pen_on=pen,
+                      poisson_mode=oracle.POISSON_FAS, poisson_tol=1e-14, poisson_max=400)
+    orhs.set_fields(np.zeros(og.rshape), nd, mask)
+    return g, ps, rhs, phi, orhs, og, state
+
+
+CASES = [
+    ("eqst", 32, False, True, 1.0, -0.1, False),
+    ("eqst", 32, False, True, 1.0, -0.1, True),
+    ("eqst", 24, True, False, 1.2, -0.2, False),
+    ("eqst", 48, True, True, 1.5, -0.3, False),
+    ("sist", 24, True, True, 1.2, -0.2, False),
+    ("hyst", 16, False, True, 1.0, -0.1, False),
+]
+
+
+@pytest.mark.parametrize("kind,n,sv,pen,M,Q,general", CASES)
+def test_rhs_fp64_matches_oracle(ctx, oracle, kind, n, sv, pen, M, Q, general):
+    print("rhs fp64", kind, n, sv, pen, general, flush=True)
+    g, ps, rhs, phi, orhs, og, state = _build(ctx, oracle, kind, n, sv, pen, M, Q, "fp64", general)
+    dF = [ctx.to_device(s) for s in state]
+    rhs.solve(0.0, dF)
+    ref, st = orhs.eval(0.0, state)
+    assert st == 0
+    for a in range(4):
+        e = rel_l2(dF[a].get(), ref[a])
+        assert e < FP64_TOL, (a, e)
+    assert rel_l2(phi.get(), orhs.phi()) < FP64_TOL
+    assert rhs.poisson_failures() == 0
+
+
+def test_rhs_mixed_matches_oracle(ctx, oracle):
+    print("rhs mixed", flush=True)
+    g, ps, rhs, phi, orhs, og, state = _build(ctx, oracle, "eqst", 32, True, True, 1.5, -0.3, "mixed")
+    dF = [ctx.to_device(s) for s in state]
+    rhs.solve(0.0, dF)
+    ref, st = orhs.eval(0.0, state)
+    for a in range(4):
+        e = rel_l2(dF[a].get(), ref[a])
+        assert e < MIXED_TOL, (a, e)
+
+
+@pytest.mark.parametrize("kind,n,steps,sv", [("eqst", 16, 20, False), ("eqst", 32, 4, False),
+                                             ("sist", 16, 4, True)])
+def test_rk4_fp64_matches_oracle(ctx, oracle, kind, n, steps, sv):
+    import kielflow_amd as kf
+    print("rk4 fp64", kind, n, steps, flush=True)
+    g, ps, rhs, phi, orhs, og, state = _build(ctx, oracle, kind, n, sv, True, 1.0, -0.1, "fp64")
+    dF = [ctx.to_device(s) for s in state]
+    rk = kf.Rk4(rhs, 1e-3)
+    ost = [s.copy() for s in state]
+    t = 0.0
+    rk.step(dF, steps)
+    for _ in range(steps):
+        t, s = orhs.rk4(1e-3, t, ost)
+        assert s == 0
+    for a in range(4):
+        e = rel_l2(dF[a].get(), ost[a])
+        assert e < FP64_TOL, (a, e)
+    assert rel_l2(phi.get(), orhs.phi()) < FP64_TOL
+    assert rk.t == pytest.approx(steps * 1e-3)
+    assert rhs.poisson_failures() == 0
+
+
+def test_rk4_mixed_matches_oracle(ctx, oracle):
+    import kielflow_amd as kf
+    print("rk4 mixed", flush=True)
+    g, ps, rhs, phi, orhs, og, state = _build(ctx, oracle, "eqst", 32, False, True, 1.5, -0.3, "mixed")
+    dF = [ctx.to_device(s) for s in state]
+    rk = kf.Rk4(rhs, 1e-3)
+    ost = [s.copy() for s in state]
+    t = 0.0
+    rk.step(dF, 4)
+    for _ in range(4):
+        t, s = orhs.rk4(1e-3, t, ost)
+    for a in range(4):
+        e = rel_l2(dF[a].get(), ost[a])
+        assert e < MIXED_TOL, (a, e)
+
+
+def test_spectral_operators_match_oracle(ctx, oracle):
+    import kielflow_amd as kf
+    for kind, n in (("eqst", 24), ("sist", 24), ("hyst", 16)):
+        print("ops", kind, n, flush=True)
+        state, nd, mask, og = initial_state(kind, n)
+        g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+        rng = np.random.default_rng(1)
+        X = og.r2c(rng.standard_normal(og.rshape))
+        for d in range(3):
+            dX = ctx.to_device(X)
+            out = g.cfield()
+            g.deriv(d, dX, out)
+            assert rel_l2(out.get(), og.deriv(d, X)) < 1e-13
+        dX = ctx.to_device(X)
+        g.dealias23(dX)
+        assert np.array_equal(dX.get() == 0, og.dealias23(X) == 0)
+        assert rel_l2(dX.get(), og.dealias23(X)) == 0.0
+        dX = ctx.to_device(X)
+        g.filter_phi(dX)
+        assert rel_l2(dX.get(), og.filter_phi(X)) < 1e-14
+
+
+def test_cfl_matches_oracle(ctx, oracle):
+    import kielflow_amd as kf
+    state, nd, mask, og = initial_state("sist", 16)
+    state = _perturb(og, state, amp=0.5)
+    g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+    d = [ctx.to_device(s) for s in state[:3]]
+    assert g.cfl(0.01, *d) == pytest.approx(og.cfl(0.01, *state[:3]), rel=1e-13)
