@@ -72,6 +72,7 @@ class Poisson {
     void set_tolerance(double t) { tol_ = t; }
     void set_max_iterations(int m) { max_it_ = m; }
     void set_smoothing(int pre, int post, int coarse) { nu1_ = pre; nu2_ = post; nuc_ = coarse; }
+    void set_krylov(bool on, int iters) { krylov_ = on; kiters_ = iters; }
     int mode() const { return mode_; }
     double tolerance() const { return tol_; }
     int max_iterations() const { return max_it_; }
@@ -119,6 +120,12 @@ class Poisson {
     void smooth(int l, int sweeps, hipStream_t s);
     template <typename R>
     void newton(double *phi, const double *rho64, const float *rho32, hipStream_t s);
+    // BiCGStab on J x = b (level-0 b holds the right side on entry), right
+    // preconditioned by one V-cycle; result in level-0 x
+    template <typename R>
+    void bicgstab(int iters, hipStream_t s);
+    template <typename R>
+    void dot(const R *a, const R *b, int slot, hipStream_t s);
     int solve_cascade(double *phi, const double *rho, hipStream_t s);
 
     AxisDesc ax_[3];
@@ -136,6 +143,10 @@ class Poisson {
     double *h_stat_ = nullptr;    // pinned mirror
     int *h_ctl_ = nullptr;
     double *corr64_ = nullptr;    // FP64 scratch (mixed defect correction)
+    bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
+    int kiters_ = 2;
+    void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)
+    double *d_scal_ = nullptr;    // device scalars of BiCGStab
     NljState nlj_;
     hipStream_t s0_;
 };

@@ -313,6 +313,106 @@ __global__ void k_prolong_add(const R *__restrict__ c, int cx, int cy, int cz, R
     f[p] += v;
 }
 
+// ======================================================================== BiCGStab kernels
+// scalar slots: 0 rho, 1 alpha, 2 omega, 3 beta, 4 dot result, 5 (rh,v), 6 (t,s), 7 (t,t), 8 rho_new
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_dot_part(const R *__restrict__ a, const R *__restrict__ b, long n, double *__restrict__ part,
+           const int *ctl)
+{
+    __shared__ double sh[RED_BLOCK];
+    if (ctl[0]) return;
+    double acc = 0.;
+    for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < n; p += (long)gridDim.x * RED_BLOCK)
+        acc += (double)a[p] * (double)b[p];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ void __launch_bounds__(RED_BLOCK)
+k_dot_final(const double *__restrict__ part, int nb, double *scal, int slot, const int *ctl)
+{
+    __shared__ double sh[RED_BLOCK];
+    if (ctl[0]) return;
+    double acc = 0.;
+    for (int b = threadIdx.x; b < nb; b += RED_BLOCK) acc += part[b];
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) scal[slot] = acc;
+}
+
+__device__ __forceinline__ double safe_div(double a, double b) { return b != 0. ? a / b : 0.; }
+
+// op 0: init (rho=alpha=omega=1); op 1: beta = (rho_new/rho)(alpha/omega), rho = rho_new;
+// op 2: alpha = rho/(rh,v); op 3: omega = (t,s)/(t,t)
+__global__ void k_scal(double *scal, int op, const int *ctl)
+{
+    if (ctl[0] || threadIdx.x != 0) return;
+    if (op == 0) {
+        scal[0] = 1.;
+        scal[1] = 1.;
+        scal[2] = 1.;
+    } else if (op == 1) {
+        scal[3] = safe_div(scal[8], scal[0]) * safe_div(scal[1], scal[2]);
+        scal[0] = scal[8];
+    } else if (op == 2) {
+        scal[1] = safe_div(scal[0], scal[5]);
+    } else {
+        scal[2] = safe_div(scal[6], scal[7]);
+    }
+}
+
+// p = r + beta (p - omega v)
+template <typename R>
+__global__ void k_bicg_p(R *__restrict__ p, const R *__restrict__ r, const R *__restrict__ v, long n,
+                         const double *scal, const int *ctl)
+{
+    if (ctl[0]) return;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R beta = (R)scal[3], om = (R)scal[2];
+    p[i] = r[i] + beta * (p[i] - om * v[i]);
+}
+
+// s = r - alpha v
+template <typename R>
+__global__ void k_bicg_s(R *__restrict__ s, const R *__restrict__ r, const R *__restrict__ v, long n,
+                         const double *scal, const int *ctl)
+{
+    if (ctl[0]) return;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    s[i] = r[i] - (R)scal[1] * v[i];
+}
+
+// x += alpha ph + omega sh ; r = s - omega t
+template <typename R>
+__global__ void k_bicg_xr(R *__restrict__ x, R *__restrict__ r, const R *__restrict__ ph,
+                          const R *__restrict__ sh, const R *__restrict__ s, const R *__restrict__ t, long n,
+                          const double *scal, const int *ctl)
+{
+    if (ctl[0]) return;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R al = (R)scal[1], om = (R)scal[2];
+    x[i] = x[i] + al * ph[i] + om * sh[i];
+    r[i] = s[i] - om * t[i];
+}
+
+// y = J x = L x - e x
+template <typename R>
+__global__ void k_apply_J(LevelView<R> L, const R *__restrict__ x, const R *__restrict__ e, R *__restrict__ y,
+                          const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= L.N) return;
+    const int nz = L.n[2], ny = L.n[1];
+    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    R A, D;
+    stencil<R>(L, x, i, j, k, A, D);
+    y[p] = A - D * x[p] - e[p] * x[p];
+}
+
 // ======================================================================== reference NLJ kernels
 // newton_method::solve (newton_method.hpp:17-63), f = A - D x - e^x
 __device__ __forceinline__ double newton_ref(double x, double A, double D)
@@ -481,6 +581,12 @@ Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed
     KF_HIP(hipMemset(d_fail_, 0, sizeof(int) * 2));
     KF_HIP(hipMalloc(&d_stat_, sizeof(double) * 4));
     KF_HIP(hipHostMalloc(&h_stat_, sizeof(double) * 4, hipHostMallocDefault));
+    KF_HIP(hipMalloc(&d_scal_, sizeof(double) * 16));
+    krylov_ = (ax[0].type != 1) || (ax[1].type != 1) || (ax[2].type != 1);
+    {
+        const size_t es = mixed_ ? sizeof(float) : sizeof(double);
+        for (auto &v : kv_) KF_HIP(hipMalloc(&v, es * lv_[0].N));
+    }
     KF_HIP(hipHostMalloc(&h_ctl_, sizeof(int) * 8, hipHostMallocDefault));
     KF_HIP(hipDeviceSynchronize());
 }
@@ -506,6 +612,8 @@ Poisson::~Poisson()
     (void)hipHostFree(h_stat_);
     (void)hipHostFree(h_ctl_);
     (void)hipFree(corr64_);
+    (void)hipFree(d_scal_);
+    for (auto &v : kv_) (void)hipFree(v);
 }
 
 template <typename R>
@@ -613,12 +721,79 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
             hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1],
                                A.n[2], eb, (const int *)d_ctl_);
         }
-        hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0,
-                           (const int *)d_ctl_);
-        for (int c = 0; c < inner_; ++c) vcycle<R>(0, s);
+        if (krylov_) {
+            bicgstab<R>(kiters_, s);
+        } else {
+            hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0,
+                               (const int *)d_ctl_);
+            for (int c = 0; c < inner_; ++c) vcycle<R>(0, s);
+        }
         hipLaunchKernelGGL(k_update<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const R *)x0, N,
                            (const int *)d_ctl_);
     }
+}
+
+template <typename R>
+void Poisson::dot(const R *a, const R *b, int slot, hipStream_t s)
+{
+    const long N = lv_[0].N;
+    unsigned nb = nblocks(N, RED_BLOCK);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_dot_part<R>, dim3(nb), dim3(RED_BLOCK), 0, s, a, b, N, d_red_, (const int *)d_ctl_);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nb, d_scal_,
+                       slot, (const int *)d_ctl_);
+}
+
+// right-preconditioned BiCGStab (van der Vorst 1992) for J x = b, x0 = 0,
+// M = one V-cycle from zero; scalars stay on the device
+template <typename R>
+void Poisson::bicgstab(int iters, hipStream_t s)
+{
+    Level &L0 = lv_[0];
+    const long N = L0.N;
+    const unsigned nb = nblocks(N, 256);
+    const int *ctl = d_ctl_;
+    R *x0, *b0, *e0;
+    if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
+    else { x0 = L0.x32; b0 = L0.b32; e0 = L0.e32; }
+    R *X = (R *)kv_[0], *r = (R *)kv_[1], *rh = (R *)kv_[2], *p = (R *)kv_[3], *v = (R *)kv_[4];
+    R *sv = (R *)kv_[5], *t = (R *)kv_[6], *ph = (R *)kv_[7], *shv = (R *)kv_[8];
+    LevelView<R> v0 = view<R>(0);
+    const size_t bytes = sizeof(R) * N;
+    // r = b, rh = b, x = 0, p = v = 0
+    hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, r, (const R *)b0, N, ctl);
+    hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, rh, (const R *)b0, N, ctl);
+    hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, X, N, (R)0, ctl);
+    hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, p, N, (R)0, ctl);
+    hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, v, N, (R)0, ctl);
+    hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 0, ctl);
+    (void)bytes;
+    auto precond = [&](const R *in, R *out) {
+        hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, b0, in, N, ctl);
+        hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, x0, N, (R)0, ctl);
+        vcycle<R>(0, s);
+        hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, out, (const R *)x0, N, ctl);
+    };
+    for (int k = 0; k < iters; ++k) {
+        dot<R>(rh, r, 8, s);
+        hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 1, ctl);
+        hipLaunchKernelGGL(k_bicg_p<R>, dim3(nb), dim3(256), 0, s, p, (const R *)r, (const R *)v, N,
+                           (const double *)d_scal_, ctl);
+        precond(p, ph);
+        hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)ph, (const R *)e0, v, ctl);
+        dot<R>(rh, v, 5, s);
+        hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 2, ctl);
+        hipLaunchKernelGGL(k_bicg_s<R>, dim3(nb), dim3(256), 0, s, sv, (const R *)r, (const R *)v, N,
+                           (const double *)d_scal_, ctl);
+        precond(sv, shv);
+        hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)shv, (const R *)e0, t, ctl);
+        dot<R>(t, sv, 6, s);
+        dot<R>(t, t, 7, s);
+        hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 3, ctl);
+        hipLaunchKernelGGL(k_bicg_xr<R>, dim3(nb), dim3(256), 0, s, X, r, (const R *)ph, (const R *)shv,
+                           (const R *)sv, (const R *)t, N, (const double *)d_scal_, ctl);
+    }
+    hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, x0, (const R *)X, N, ctl);
 }
 
 int Poisson::solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync)
@@ -850,6 +1025,8 @@ int Poisson::linear_solve(double *x, const double *b, double rtol, int max_cycle
 }
 
 template void Poisson::newton<float>(double *, const double *, const float *, hipStream_t);
+template void Poisson::bicgstab<float>(int, hipStream_t);
+template void Poisson::bicgstab<double>(int, hipStream_t);
 template void Poisson::newton<double>(double *, const double *, const float *, hipStream_t);
 
 }  // namespace kf
