@@ -1,0 +1,191 @@
+"""Benchmark: RK4 steps/s of kielflow's pseudo-spectral step on one MI355X.
+
+Workload: the 256^3 equidistant box (L = 16 per side) with M = 1.5,
+Q = -0.3, 1e-3 seeded density noise, Boltzmann electrons solved to a
+relative residual of 1e-10 every stage, 2/3 dealiasing, penalization on,
+dt = 1e-3.  `value` is the mixed-precision rate (FFT, RHS and Poisson sweeps
+in FP32; RK4 state, accumulation, Poisson residual and Newton correction in
+FP64); the FP64 rate of the same workload, their ratio and the per-field
+relative L2 distance between the two runs are reported alongside.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+L_BOX = 16.0
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--n", type=int, default=256)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    p.add_argument("--cpu-n", type=int, default=128, help="box size of the CPU sample")
+    return p.parse_args()
+
+
+def run_precision(kf, ctx, g, axes, prec, nd_d, mask_d, Fn0, steps, warmup, prof_family=None):
+    P = kf.KF_MIXED if prec == "mixed" else kf.KF_FP64
+    ps = kf.Poisson(g, P, tol=1e-10, max_it=60)
+    rhs = kf.Rhs(g, ps, M=1.5, tau=0.1, theta=100.0, sv_on=False, pen_on=True, precision=P)
+    phi = g.rfield()
+    rhs.set_fields(phi, nd_d, mask_d)
+    F = [g.cfield(), g.cfield(), g.cfield(), g.cfield()]
+    F[3].copy_from(Fn0)
+    rk = kf.Rk4(rhs, 1e-3)
+    rk.step(F, warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    rk.step(F, steps)
+    ctx.sync()
+    t1 = time.perf_counter()
+    out = {"steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
+           "poisson_failures": rhs.poisson_failures()}
+    # state after warmup + timed steps (compared across precisions)
+    out["state"] = [f.get() for f in F]
+    out["phi"] = phi.get()
+    if prof_family is not None:
+        # the same steps again with the dominant kernel family bracketed by
+        # HIP events on the library's stream
+        kf.api.profile_enable(prof_family)
+        rk.step(F, steps)
+        ctx.sync()
+        n, ms = kf.api.profile_read()
+        kf.api.profile_enable(-1)
+        out["prof_launches"] = n
+        out["prof_ms"] = ms
+    for o in (rk, rhs, ps):
+        o.close()
+    return out
+
+
+def cpu_baseline(n_sample, n_target):
+    """One RHS evaluation of the same configuration on a smaller box, timed
+    with the FP64 oracle on one host thread, scaled to n_target by the
+    n^3 log2 n work ratio of the transforms and per-point kernels."""
+    import kielflow_amd as kf
+    from oracle import oracle as O
+    O.build()
+    og = O.Grid.cube(n_sample, L_BOX)
+    axes = kf.cube(n_sample, L_BOX)
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=-0.3, a=2.0 * L_BOX / n_sample)
+    r = O.Rhs(og, M=1.5, tau=0.1, theta=100.0, sv_on=False, pen_on=True,
+              poisson_mode=O.POISSON_FAS, poisson_tol=1e-10, poisson_max=200)
+    r.set_fields(np.zeros(og.rshape), nd, mask)
+    st = [np.zeros(og.cshape, complex) for _ in range(3)] + [og.r2c(ln_n)]
+    t0 = time.perf_counter()
+    r.eval(0.0, st)
+    t_rhs = time.perf_counter() - t0
+    scale = (n_target / n_sample) ** 3 * (np.log2(n_target) / np.log2(n_sample))
+    t_step = 4 * t_rhs * scale
+    cpu = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": 1, "kind": "port",
+            "sample": (f"one RHS evaluation (Poisson to 1e-10) of the same configuration at "
+                       f"{n_sample}^3 took {t_rhs:.2f} s on 1 thread of {cpu}; a {n_target}^3 RK4 step "
+                       f"(4 RHS) estimated as 4 x that x {scale:.2f} (n^3 log2 n ratio)"),
+            "host_threads_available": os.cpu_count()}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one MI355X: --gpus {args.gpus} is not supported (independent boxes "
+              "would shard one per GPU with no exchange; a single box is not split across GPUs).",
+              file=sys.stderr)
+        sys.exit(2)
+    import kielflow_amd as kf
+    n = args.n
+    axes = kf.cube(n, L_BOX)
+    a = 2.0 * L_BOX / n
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=-0.3, a=a)
+    ctx = kf.Context(0)
+    g = kf.Grid(ctx, axes)
+    fft = kf.FFT(g, kf.KF_FP64)
+    d_ln = ctx.to_device(ln_n)
+    Fn0 = g.cfield()
+    fft.r2c(d_ln, Fn0)
+    nd_d, mask_d = ctx.to_device(nd), ctx.to_device(mask)
+    del d_ln
+
+    fam = kf.api.PROF_FFT_Y
+    mixed = run_precision(kf, ctx, g, axes, "mixed", nd_d, mask_d, Fn0, args.steps, args.warmup, fam)
+    fp64 = None
+    if not args.no_fp64:
+        fp64 = run_precision(kf, ctx, g, axes, "fp64", nd_d, mask_d, Fn0, args.steps, args.warmup)
+
+    # roofline of the dominant kernel family: strided y pass, FP32, reads and
+    # writes the complex field once: 2 * Nx*Ny*(Nz/2+1) * 8 bytes per launch
+    nc = n * n * (n // 2 + 1)
+    alg_bytes = 2 * nc * 8
+    avg_ms = mixed["prof_ms"] / max(mixed["prof_launches"], 1)
+    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_strided (FFT y pass, FP32, length 256)",
+            "avg_launch_us": round(avg_ms * 1e3, 2), "launches": mixed["prof_launches"],
+            "alg_bytes_per_launch": alg_bytes}
+
+    extra = {"poisson_failures_mixed": mixed["poisson_failures"]}
+    if fp64 is not None:
+        errs = {f: rel_l2(mixed["state"][i], fp64["state"][i]) for i, f in
+                enumerate(("FUx", "FUy", "FUz", "ln_n"))}
+        errs["phi"] = rel_l2(mixed["phi"], fp64["phi"])
+        extra.update({"fp64_steps_per_s": round(fp64["steps_per_s"], 3),
+                      "fp64_ms_per_step": round(fp64["ms_per_step"], 3),
+                      "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
+                      "rel_l2_mixed_vs_fp64_after_steps": errs,
+                      "poisson_failures_fp64": fp64["poisson_failures"]})
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, n)
+    line = {
+        "metric": "RK4 steps/s at 256^3 (mixed precision; FP64 rate and ratio alongside)",
+        "value": round(mixed["steps_per_s"], 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(mixed["ms_per_step"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded 1e-3 density noise, Philox4x32-10)",
+        "config": {"workload": f"{n}^3 equidistant box RK4 step, mixed precision (FP32 FFT/RHS/sweeps, "
+                               "FP64 state, accumulation, Poisson residual)",
+                   "n": n, "L": L_BOX, "dt": 1e-3, "M": 1.5, "Q": -0.3, "tau": 0.1, "theta": 100.0,
+                   "penalization": True, "spectral_viscosity": False, "poisson_tol": 1e-10},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

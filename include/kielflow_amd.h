@@ -124,6 +124,9 @@ int kf_poisson_create(kf_grid g, int precision, int mode, kf_poisson *out);
 int kf_poisson_destroy(kf_poisson p);
 int kf_poisson_set_tolerance(kf_poisson p, double tol);
 int kf_poisson_set_max_iterations(kf_poisson p, int iters);
+/* multigrid smoothing of the Newton-MG solver: pre/post red-black sweeps per
+ * level and sweeps on the coarsest level (defaults 2, 2, 24) */
+int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse);
 /* solve(Phi_IO, rho): phi FP64 in/out (warm start), rho FP64.  Returns
  * KF_ERR_NOT_CONVERGED when the budget is spent; iters/resid may be NULL. */
 int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid);
@@ -164,6 +167,16 @@ int kf_rk4_time(kf_rk4 k, double *t);
 /* cfl_all(dt, FUx, FUy, FUz) (CFL.cpp:29-82) */
 int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
                double *dt_new);
+
+/* ---------------------------------------------------------------- launch timing
+ * Diagnostic for roofline figures (no counterpart in the reference, which
+ * has no profiler hooks, logger.hpp:13-77 aside).  family: 0 FFT x passes,
+ * 1 FFT y passes, 2 FFT z R2C passes, 3 FFT z C2R passes, 4 whole Poisson
+ * solves; -1 disables.  Each launch of the enabled family is bracketed by
+ * HIP events on its stream; read returns the count and summed milliseconds
+ * since the last enable. */
+int kf_profile_enable(int family);
+int kf_profile_read(int *launches, double *total_ms);
 
 /* ---------------------------------------------------------------- set-up
  * Initial state of the kielflow driver (kielflow.cpp:172-193, 227-233) on

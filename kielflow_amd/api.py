@@ -70,6 +70,7 @@ SIGNATURES = {
     "kf_poisson_destroy": (C.c_int, [_vp]),
     "kf_poisson_set_tolerance": (C.c_int, [_vp, C.c_double]),
     "kf_poisson_set_max_iterations": (C.c_int, [_vp, C.c_int]),
+    "kf_poisson_set_smoothing": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "kf_poisson_solve": (C.c_int, [_vp, _vp, _vp, _ip, _dp]),
     "kf_poisson_converged": (C.c_int, [_vp]),
     "kf_poisson_residual": (C.c_int, [_vp, _vp, _vp, _dp]),
@@ -88,6 +89,8 @@ SIGNATURES = {
     "kf_rk4_set_dt": (C.c_int, [_vp, C.c_double]),
     "kf_rk4_time": (C.c_int, [_vp, _dp]),
     "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
+    "kf_profile_enable": (C.c_int, [C.c_int]),
+    "kf_profile_read": (C.c_int, [_ip, _dp]),
     "kf_setup_host": (C.c_int, [_AX3, C.c_ulonglong, C.c_double, C.c_double, C.c_double, _vp, _vp,
                                 _vp]),
 }
@@ -278,6 +281,9 @@ class Poisson:
     def set_max_iterations(self, m):
         check(self.grid.ctx.lib.kf_poisson_set_max_iterations(self.h, m))
 
+    def set_smoothing(self, pre, post, coarse=24):
+        check(self.grid.ctx.lib.kf_poisson_set_smoothing(self.h, pre, post, coarse))
+
     def solve(self, phi: DeviceArray, rho: DeviceArray, raise_on_error=True):
         it = C.c_int(0)
         res = C.c_double(0)
@@ -358,6 +364,20 @@ class Rk4:
         v = C.c_double(0)
         check(self.rhs.grid.ctx.lib.kf_rk4_time(self.h, C.byref(v)))
         return v.value
+
+
+PROF_FFT_X, PROF_FFT_Y, PROF_FFT_ZR2C, PROF_FFT_ZC2R, PROF_POISSON = 0, 1, 2, 3, 4
+
+
+def profile_enable(family: int):
+    check(load().kf_profile_enable(family))
+
+
+def profile_read():
+    n = C.c_int(0)
+    ms = C.c_double(0)
+    check(load().kf_profile_read(C.byref(n), C.byref(ms)))
+    return n.value, ms.value
 
 
 def setup_host(axes, seed=0, amp=1e-3, Q=-0.1, a=1.0):

@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkielflow_amd.so")
 SOURCES = ["kf_fft.hip", "kf_grid.hip", "kf_poisson.hip", "kf_rhs.hip", "kf_adjoint.hip",
-           "kf_batch.hip", "kf_capi.hip"]
+           "kf_batch.hip", "kf_prof.hip", "kf_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KF_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",

@@ -10,6 +10,7 @@
 #include "kf_fft.h"
 #include "kf_grid.h"
 #include "kf_poisson.h"
+#include "kf_prof.h"
 #include "kf_rhs.h"
 
 using namespace kf;
@@ -371,6 +372,16 @@ int kf_poisson_set_max_iterations(kf_poisson p, int iters)
     });
 }
 
+int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse)
+{
+    return guard([&] {
+        KF_REQUIRE(p, "null solver");
+        KF_REQUIRE(pre >= 0 && post >= 0 && coarse >= 1 && pre + post >= 1, "bad smoothing counts");
+        p->p->set_smoothing(pre, post, coarse);
+        return KF_OK;
+    });
+}
+
 int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid)
 {
     return guard([&] {
@@ -590,6 +601,23 @@ int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const
         KF_REQUIRE(g && FUx && FUy && FUz && dt_new, "null argument");
         const double2 *F[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
         *dt_new = cfl_all(*g->g, *g->p64, dt, F, g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+int kf_profile_enable(int family)
+{
+    return guard([&] {
+        KF_REQUIRE(family >= -1 && family < prof::NFAM, "unknown family");
+        prof::enable(family);
+        return KF_OK;
+    });
+}
+
+int kf_profile_read(int *launches, double *total_ms)
+{
+    return guard([&] {
+        prof::read(launches, total_ms);
         return KF_OK;
     });
 }

@@ -18,6 +18,7 @@
 
 #include "kf_common.h"
 #include "kf_fft_engine.h"
+#include "kf_prof.h"
 
 namespace kf {
 namespace fft {
@@ -33,10 +34,13 @@ __host__ __device__ constexpr int tpt_for(int n)
 //   (q / qdiv) * qplane + (q % qdiv) + t * tstride
 struct StridedAddr {
     long qdiv, qplane, tstride;
-    __device__ __forceinline__ long operator()(long q, int t) const
+    __device__ __forceinline__ long base(long q) const
     {
-        return (q / qdiv) * qplane + (q % qdiv) + (long)t * tstride;
+        const unsigned u = (unsigned)q, d = (unsigned)qdiv;
+        const unsigned r = u / d;
+        return (long)r * qplane + (long)(u - r * d);
     }
+    __device__ __forceinline__ long operator()(long q, int t) const { return base(q) + (long)t * tstride; }
 };
 
 template <typename R>
@@ -68,17 +72,18 @@ k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
     const int tid = threadIdx.x;
     const long q0 = (long)blockIdx.x * CB;
     for (int m = tid; m < N; m += WG) tw[m] = tw_g[m];
-    for (int e = tid; e < N * CB; e += WG) {
-        const int c = e % CB, t = e / CB;
-        const long q = q0 + c;
-        lds[t * CB + c] = (q < Q) ? ld(q, t) : mk<C>((R)0, (R)0);
-    }
+    // each thread owns one column c and rows t0, t0+TPT, ... (the column's
+    // address arithmetic is loop invariant)
+    const int c = tid % CB, t0 = tid / CB;
+    const long q = q0 + c;
+    const bool live = q < Q;
+#pragma unroll
+    for (int t = t0; t < N; t += TPT) lds[t * CB + c] = live ? ld(q, t) : mk<C>((R)0, (R)0);
     __syncthreads();
-    fft_lds<C, N, TPT, 1, CB, DIR>(lds, tw, tid % CB, tid / CB);
-    for (int e = tid; e < N * CB; e += WG) {
-        const int c = e % CB, t = e / CB;
-        const long q = q0 + c;
-        if (q < Q) st(q, t, lds[t * CB + c]);
+    fft_lds<C, N, TPT, 1, CB, DIR>(lds, tw, c, t0);
+    if (live) {
+#pragma unroll
+        for (int t = t0; t < N; t += TPT) st(q, t, lds[t * CB + c]);
     }
 }
 
@@ -251,8 +256,9 @@ __host__ __device__ constexpr int cb_for(int n)
 #define KF_FFT_LENGTHS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
 template <typename R, int DIR, class LD, class ST_>
-void launch_strided(int n, long Q, const C2<R> *tw, LD ld, ST_ st, hipStream_t s)
+void launch_strided(int n, long Q, const C2<R> *tw, LD ld, ST_ st, hipStream_t s, int fam)
 {
+    prof::Scope ps_(fam, s);
     switch (n) {
 #define KF_CASE(NN)                                                                       \
     case NN: {                                                                            \
@@ -273,6 +279,7 @@ template <typename R, class LD, class ST_>
 void launch_zr2c(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST_ st,
                  hipStream_t s)
 {
+    prof::Scope ps_(prof::FFT_ZR2C, s);
     switch (m) {
 #define KF_CASE(NN)                                                                       \
     case NN: {                                                                            \
@@ -303,6 +310,7 @@ template <typename R, class LD, class ST_>
 void launch_zc2r(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST_ st,
                  hipStream_t s)
 {
+    prof::Scope ps_(prof::FFT_ZC2R, s);
     switch (m) {
 #define KF_CASE(NN)                                                                       \
     case NN: {                                                                            \

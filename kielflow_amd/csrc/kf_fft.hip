@@ -82,9 +82,9 @@ void r2c(const Plan<R> &p, const R *in, C2<R> *out, hipStream_t s)
                    RowStoreC<R>{out, p.nzc}, s);
     StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
     launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{out, ay},
-                          StorePlain<R>{out, ay}, s);
+                          StorePlain<R>{out, ay}, s, prof::FFT_Y);
     launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{out, ax},
-                          StorePlain<R>{out, ax}, s);
+                          StorePlain<R>{out, ax}, s, prof::FFT_X);
 }
 
 template <typename R>
@@ -92,9 +92,9 @@ void c2r(const Plan<R> &p, const C2<R> *in, R *out, hipStream_t s)
 {
     StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
     launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{in, ax},
-                          StorePlain<R>{p.scratch, ax}, s);
+                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X);
     launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s);
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
     const long rows = (long)p.nx * p.ny;
     launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc},
                    RowStoreReal<R>{out, p.nz, (R)(1.0 / (double)p.nr)}, s);

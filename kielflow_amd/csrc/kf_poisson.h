@@ -50,6 +50,9 @@ struct LevelView {
     AxisCoef<R> c[3];
 };
 
+template <typename R>
+struct CoarseSet;
+
 struct PoissonStats {
     int cycles;      // Newton iterations, or sweeps of the reference modes
     double resid;    // max|F| / max|rho|
@@ -126,6 +129,11 @@ class Poisson {
     void bicgstab(int iters, hipStream_t s);
     template <typename R>
     void dot(const R *a, const R *b, int slot, hipStream_t s);
+    template <typename R>
+    CoarseSet<R> coarse_set(int l0) const;
+    int first_coarse() const;
+    template <typename R>
+    void restrict_e(hipStream_t s, const int *ctl);
     int solve_cascade(double *phi, const double *rho, hipStream_t s);
 
     AxisDesc ax_[3];
@@ -147,6 +155,7 @@ class Poisson {
     int kiters_ = 2;
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)
     double *d_scal_ = nullptr;    // device scalars of BiCGStab
+    hipEvent_t ev_[2] = {};       // convergence-flag copies of the look-ahead loop
     NljState nlj_;
     hipStream_t s0_;
 };

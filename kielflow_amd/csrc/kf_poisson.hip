@@ -1,5 +1,6 @@
 // kf_poisson.hip -- nonlinear Poisson solver (Boltzmann electrons) on the GPU.
 #include "kf_poisson.h"
+#include "kf_prof.h"
 
 #include <algorithm>
 #include <cmath>
@@ -64,6 +65,18 @@ constexpr int RED_BLOCK = 256;
 
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// linear index -> (i, j, k) of a [nx][ny][nz] array, 32-bit unsigned
+// arithmetic (every grid here has fewer than 2^31 points)
+__device__ __forceinline__ void dec3(long p, int ny, int nz, int &i, int &j, int &k)
+{
+    const unsigned u = (unsigned)p, z = (unsigned)nz, y = (unsigned)ny;
+    const unsigned r = u / z;
+    k = (int)(u - r * z);
+    const unsigned q = r / y;
+    j = (int)(r - q * y);
+    i = (int)q;
+}
+
 template <typename R>
 __device__ __forceinline__ void stencil(const LevelView<R> &L, const R *__restrict__ x, int i, int j,
                                         int k, R &A, R &D)
@@ -84,50 +97,41 @@ __device__ __forceinline__ void stencil(const LevelView<R> &L, const R *__restri
     D = L.c[0].dd[i] + L.c[1].dd[j] + L.c[2].dd[k];
 }
 
-template <typename T>
-__device__ __forceinline__ T block_max(T v, T *sh)
+// block reductions: wavefront (64 lanes) shuffles, then one LDS slot per wave
+template <typename T, class Op>
+__device__ __forceinline__ T block_reduce(T v, T *sh, Op op)
 {
-    const int tid = threadIdx.x;
-    sh[tid] = v;
+    for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) sh[tid] = sh[tid] > sh[tid + s] ? sh[tid] : sh[tid + s];
-        __syncthreads();
-    }
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
     T r = sh[0];
+    for (int w = 1; w < nw; ++w) r = op(r, sh[w]);
     __syncthreads();
     return r;
 }
+struct OpMax {
+    template <typename T>
+    __device__ T operator()(T a, T b) const { return a > b ? a : b; }
+};
+struct OpMin {
+    template <typename T>
+    __device__ T operator()(T a, T b) const { return a < b ? a : b; }
+};
+struct OpSum {
+    template <typename T>
+    __device__ T operator()(T a, T b) const { return a + b; }
+};
+template <typename T>
+__device__ __forceinline__ T block_max(T v, T *sh) { return block_reduce(v, sh, OpMax()); }
+template <typename T>
+__device__ __forceinline__ T block_min(T v, T *sh) { return block_reduce(v, sh, OpMin()); }
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T *sh) { return block_reduce(v, sh, OpSum()); }
 
-template <typename T>
-__device__ __forceinline__ T block_min(T v, T *sh)
-{
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) sh[tid] = sh[tid] < sh[tid + s] ? sh[tid] : sh[tid + s];
-        __syncthreads();
-    }
-    T r = sh[0];
-    __syncthreads();
-    return r;
-}
-
-template <typename T>
-__device__ __forceinline__ T block_sum(T v, T *sh)
-{
-    const int tid = threadIdx.x;
-    sh[tid] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (tid < s) sh[tid] += sh[tid + s];
-        __syncthreads();
-    }
-    T r = sh[0];
-    __syncthreads();
-    return r;
-}
+// grid used by reduction kernels: grid-stride loops over at most this many blocks
+constexpr unsigned RED_MAX_BLOCKS = 2048;
 
 // ======================================================================== Newton-MG kernels
 // F = L_h phi - exp(phi) + rho (FP64); b = -F and e = exp(phi) in the sweep
@@ -137,13 +141,13 @@ __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *__restrict__ rho,
                R *__restrict__ b, R *__restrict__ e, double *__restrict__ part, const int *ctl)
 {
-    __shared__ double sh[RED_BLOCK];
+    __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
-    const long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x;
     double fmaxv = 0., rmax = 0.;
-    if (p < L.N) {
-        const int nz = L.n[2], ny = L.n[1];
-        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    const int nz = L.n[2], ny = L.n[1];
+    for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < L.N; p += (long)gridDim.x * RED_BLOCK) {
+        int i, j, k;
+        dec3(p, ny, nz, i, j, k);
         double A, D;
         stencil<double>(L, phi, i, j, k, A, D);
         const double x = phi[p];
@@ -152,8 +156,8 @@ k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *_
         const double F = A - D * x - ex + rh;
         b[p] = (R)(-F);
         e[p] = (R)ex;
-        fmaxv = fabs(F);
-        rmax = fabs(rh);
+        fmaxv = fmax(fmaxv, fabs(F));
+        rmax = fmax(rmax, fabs(rh));
     }
     fmaxv = block_max(fmaxv, sh);
     rmax = block_max(rmax, sh);
@@ -168,7 +172,7 @@ __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_check(const double *__restrict__ part, int nblk, int it, int max_it, double tol, int *ctl,
                int *fail, double *stat)
 {
-    __shared__ double sh[RED_BLOCK];
+    __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
     double fm = 0., rm = 0.;
     for (int b = threadIdx.x; b < nblk; b += RED_BLOCK) {
@@ -184,11 +188,11 @@ k_newton_check(const double *__restrict__ part, int nblk, int it, int max_it, do
         if (rel <= tol || !(rel == rel)) {
             ctl[0] = 1;
             ctl[2] = (rel <= tol) ? 1 : 0;
-            if (!(rel <= tol)) fail[0] += 1;
+            if (!(rel <= tol) && max_it > 0) fail[0] += 1;
         } else if (it >= max_it) {
             ctl[0] = 1;
             ctl[2] = 0;
-            fail[0] += 1;
+            if (max_it > 0) fail[0] += 1;
         }
     }
 }
@@ -218,7 +222,8 @@ __global__ void k_rb(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
     const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= L.N / 2) return;
     const int nz2 = L.n[2] / 2, ny = L.n[1];
-    const int kk = (int)(q % nz2), j = (int)((q / nz2) % ny), i = (int)(q / ((long)nz2 * ny));
+    int i, j, kk;
+    dec3(q, ny, nz2, i, j, kk);
     const int k = 2 * kk + ((i + j + color) & 1);
     R A, D;
     stencil<R>(L, x, i, j, k, A, D);
@@ -235,7 +240,8 @@ __global__ void k_jacobi(LevelView<R> L, const R *__restrict__ x, R *__restrict_
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= L.N) return;
     const int nz = L.n[2], ny = L.n[1];
-    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    int i, j, k;
+        dec3(p, ny, nz, i, j, k);
     R A, D;
     stencil<R>(L, x, i, j, k, A, D);
     const R xn = (A - b[p]) / (D + e[p]);
@@ -259,7 +265,8 @@ __global__ void k_lin_resid(LevelView<R> L, const R *__restrict__ x, const R *__
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= L.N) return;
     const int nz = L.n[2], ny = L.n[1];
-    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    int i, j, k;
+        dec3(p, ny, nz, i, j, k);
     R A, D;
     stencil<R>(L, x, i, j, k, A, D);
     r[p] = b[p] - (A - D * x[p] - e[p] * x[p]);
@@ -274,7 +281,8 @@ __global__ void k_restrict(const R *__restrict__ f, int fx, int fy, int fz, R *_
     const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long)cx * cy * cz) return;
-    const int k = (int)(p % cz), j = (int)((p / cz) % cy), i = (int)(p / ((long)cz * cy));
+    int i, j, k;
+        dec3(p, cy, cz, i, j, k);
     const R w[3] = {(R)0.25, (R)0.5, (R)0.25};
     R acc = 0;
     for (int a = -1; a <= 1; ++a) {
@@ -299,7 +307,8 @@ __global__ void k_prolong_add(const R *__restrict__ c, int cx, int cy, int cz, R
     const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long)fx * fy * fz) return;
-    const int k = (int)(p % fz), j = (int)((p / fz) % fy), i = (int)(p / ((long)fz * fy));
+    int i, j, k;
+        dec3(p, fy, fz, i, j, k);
     const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
     const int i1 = wrapi(i0 + 1, cx), j1 = wrapi(j0 + 1, cy), k1 = wrapi(k0 + 1, cz);
     const R wi1 = (i & 1) ? (R)0.5 : (R)0, wj1 = (j & 1) ? (R)0.5 : (R)0, wk1 = (k & 1) ? (R)0.5 : (R)0;
@@ -313,6 +322,146 @@ __global__ void k_prolong_add(const R *__restrict__ c, int cx, int cy, int cz, R
     f[p] += v;
 }
 
+// ======================================================================== coarse levels in one workgroup
+// The tail of a V-cycle (every level with at most COARSE_N points) runs in a
+// single workgroup: block-stride loops separated by barriers replace the
+// many tiny launches those levels would otherwise cost.
+constexpr long COARSE_N = 4096;
+constexpr int COARSE_MAXL = 8;
+constexpr int COARSE_BLOCK = 1024;
+
+template <typename R>
+struct CoarseSet {
+    int nl;
+    LevelView<R> L[COARSE_MAXL];
+    R *x[COARSE_MAXL], *b[COARSE_MAXL], *e[COARSE_MAXL], *r[COARSE_MAXL];
+    int nu1, nu2, nuc;
+};
+
+template <typename R>
+__device__ void blk_smooth(const LevelView<R> &L, R *x, const R *b, const R *e, R *r, int sweeps)
+{
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const bool rb = !(nx & 1) && !(ny & 1) && !(nz & 1);
+    const int tid = threadIdx.x, bs = blockDim.x;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        if (rb) {
+            const int nz2 = nz / 2;
+            for (int c = 0; c < 2; ++c) {
+                for (long q = tid; q < L.N / 2; q += bs) {
+                    int i, j, kk;
+                    dec3(q, ny, nz2, i, j, kk);
+                    const int k = 2 * kk + ((i + j + c) & 1);
+                    R A, D;
+                    stencil<R>(L, x, i, j, k, A, D);
+                    const long p = ((long)i * ny + j) * nz + k;
+                    x[p] = (A - b[p]) / (D + e[p]);
+                }
+                __syncthreads();
+            }
+        } else {
+            for (long p = tid; p < L.N; p += bs) {
+                int i, j, k;
+        dec3(p, ny, nz, i, j, k);
+                R A, D;
+                stencil<R>(L, x, i, j, k, A, D);
+                r[p] = x[p] + (R)0.8 * ((A - b[p]) / (D + e[p]) - x[p]);
+            }
+            __syncthreads();
+            for (long p = tid; p < L.N; p += bs) x[p] = r[p];
+            __syncthreads();
+        }
+    }
+}
+
+template <typename R>
+__global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs, const int *ctl)
+{
+    if (ctl[0]) return;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const int nl = cs.nl;
+    for (int l = 0; l < nl - 1; ++l) {
+        const LevelView<R> &L = cs.L[l];
+        blk_smooth<R>(L, cs.x[l], cs.b[l], cs.e[l], cs.r[l], cs.nu1);
+        const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+        for (long p = tid; p < L.N; p += bs) {
+            int i, j, k;
+        dec3(p, ny, nz, i, j, k);
+            R A, D;
+            stencil<R>(L, cs.x[l], i, j, k, A, D);
+            cs.r[l][p] = cs.b[l][p] - (A - D * cs.x[l][p] - cs.e[l][p] * cs.x[l][p]);
+        }
+        __syncthreads();
+        const LevelView<R> &C = cs.L[l + 1];
+        const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+        const R w[3] = {(R)0.25, (R)0.5, (R)0.25};
+        for (long p = tid; p < C.N; p += bs) {
+            int i, j, k;
+        dec3(p, cy, cz, i, j, k);
+            R acc = 0;
+            for (int a = -1; a <= 1; ++a)
+                for (int bb = -1; bb <= 1; ++bb)
+                    for (int cc = -1; cc <= 1; ++cc) {
+                        const int fi = wrapi(2 * i + a, nx), fj = wrapi(2 * j + bb, ny), fk = wrapi(2 * k + cc, nz);
+                        acc += w[a + 1] * w[bb + 1] * w[cc + 1] * cs.r[l][((long)fi * ny + fj) * nz + fk];
+                    }
+            cs.b[l + 1][p] = acc;
+            cs.x[l + 1][p] = (R)0;
+        }
+        __syncthreads();
+    }
+    blk_smooth<R>(cs.L[nl - 1], cs.x[nl - 1], cs.b[nl - 1], cs.e[nl - 1], cs.r[nl - 1], cs.nuc);
+    for (int l = nl - 2; l >= 0; --l) {
+        const LevelView<R> &L = cs.L[l], &C = cs.L[l + 1];
+        const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
+        const int fy = L.n[1], fz = L.n[2];
+        const R *c = cs.x[l + 1];
+        for (long p = tid; p < L.N; p += bs) {
+            int i, j, k;
+        dec3(p, fy, fz, i, j, k);
+            const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+            const int i1 = wrapi(i0 + 1, cx), j1 = wrapi(j0 + 1, cy), k1 = wrapi(k0 + 1, cz);
+            const R wi1 = (i & 1) ? (R)0.5 : (R)0, wj1 = (j & 1) ? (R)0.5 : (R)0, wk1 = (k & 1) ? (R)0.5 : (R)0;
+            const R wi0 = (R)1 - wi1, wj0 = (R)1 - wj1, wk0 = (R)1 - wk1;
+#define CV(a, b, d) c[((long)(a)*cy + (b)) * cz + (d)]
+            R v = wi0 * (wj0 * (wk0 * CV(i0, j0, k0) + wk1 * CV(i0, j0, k1)) +
+                         wj1 * (wk0 * CV(i0, j1, k0) + wk1 * CV(i0, j1, k1))) +
+                  wi1 * (wj0 * (wk0 * CV(i1, j0, k0) + wk1 * CV(i1, j0, k1)) +
+                         wj1 * (wk0 * CV(i1, j1, k0) + wk1 * CV(i1, j1, k1)));
+#undef CV
+            cs.x[l][p] += v;
+        }
+        __syncthreads();
+        blk_smooth<R>(L, cs.x[l], cs.b[l], cs.e[l], cs.r[l], cs.nu2);
+    }
+}
+
+// restriction of the coefficient e through the coarse levels, one workgroup
+template <typename R>
+__global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_restrict_e(CoarseSet<R> cs, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const int tid = threadIdx.x, bs = blockDim.x;
+    const R w[3] = {(R)0.25, (R)0.5, (R)0.25};
+    for (int l = 0; l < cs.nl - 1; ++l) {
+        const LevelView<R> &L = cs.L[l], &C = cs.L[l + 1];
+        const int nx = L.n[0], ny = L.n[1], nz = L.n[2], cx = C.n[0], cy = C.n[1], cz = C.n[2];
+        for (long p = tid; p < C.N; p += bs) {
+            int i, j, k;
+        dec3(p, cy, cz, i, j, k);
+            R acc = 0;
+            for (int a = -1; a <= 1; ++a)
+                for (int bb = -1; bb <= 1; ++bb)
+                    for (int cc = -1; cc <= 1; ++cc) {
+                        const int fi = wrapi(2 * i + a, nx), fj = wrapi(2 * j + bb, ny), fk = wrapi(2 * k + cc, nz);
+                        acc += w[a + 1] * w[bb + 1] * w[cc + 1] * cs.e[l][((long)fi * ny + fj) * nz + fk];
+                    }
+            cs.e[l + 1][p] = acc;
+        }
+        __syncthreads();
+    }
+}
+
 // ======================================================================== BiCGStab kernels
 // scalar slots: 0 rho, 1 alpha, 2 omega, 3 beta, 4 dot result, 5 (rh,v), 6 (t,s), 7 (t,t), 8 rho_new
 template <typename R>
@@ -320,7 +469,7 @@ __global__ void __launch_bounds__(RED_BLOCK)
 k_dot_part(const R *__restrict__ a, const R *__restrict__ b, long n, double *__restrict__ part,
            const int *ctl)
 {
-    __shared__ double sh[RED_BLOCK];
+    __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
     double acc = 0.;
     for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < n; p += (long)gridDim.x * RED_BLOCK)
@@ -332,7 +481,7 @@ k_dot_part(const R *__restrict__ a, const R *__restrict__ b, long n, double *__r
 __global__ void __launch_bounds__(RED_BLOCK)
 k_dot_final(const double *__restrict__ part, int nb, double *scal, int slot, const int *ctl)
 {
-    __shared__ double sh[RED_BLOCK];
+    __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
     double acc = 0.;
     for (int b = threadIdx.x; b < nb; b += RED_BLOCK) acc += part[b];
@@ -407,7 +556,8 @@ __global__ void k_apply_J(LevelView<R> L, const R *__restrict__ x, const R *__re
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= L.N) return;
     const int nz = L.n[2], ny = L.n[1];
-    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    int i, j, k;
+        dec3(p, ny, nz, i, j, k);
     R A, D;
     stencil<R>(L, x, i, j, k, A, D);
     y[p] = A - D * x[p] - e[p] * x[p];
@@ -441,12 +591,13 @@ k_nlj_ref(int nx, int ny, int nz, const double *__restrict__ HX, const double *_
           const double *__restrict__ HZ, const double *__restrict__ in, double *__restrict__ out,
           const double *__restrict__ rho, double omega, double *__restrict__ part)
 {
-    __shared__ double sh[RED_BLOCK];
+    __shared__ double sh[RED_BLOCK / 64];
     const long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x;
     const long N = (long)nx * ny * nz;
     double sup = -INFINITY, inf = INFINITY, nmax = 0., nsum = 0.;
     if (p < N) {
-        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+        int i, j, k;
+        dec3(p, ny, nz, i, j, k);
         const int ip = wrapi(i + 1, nx), im = wrapi(i - 1, nx), jp = wrapi(j + 1, ny),
                   jm = wrapi(j - 1, ny), kp = wrapi(k + 1, nz), km = wrapi(k - 1, nz);
 #define IN(a, b, c) in[((long)(a)*ny + (b)) * nz + (c)]
@@ -490,7 +641,8 @@ __global__ void k_inject(const double *__restrict__ in, int fx, int fy, int fz, 
     const int cx = fx / f, cy = fy / f, cz = fz / f;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long)cx * cy * cz) return;
-    const int k = (int)(p % cz), j = (int)((p / cz) % cy), i = (int)(p / ((long)cz * cy));
+    int i, j, k;
+        dec3(p, cy, cz, i, j, k);
     out[p] = in[((long)(f * i) * fy + f * j) * fz + f * k];
 }
 
@@ -502,7 +654,8 @@ __global__ void k_hto2h(int stage, const double *__restrict__ in, int cx, int cy
     const int nx = 2 * cx, ny = 2 * cy, nz = 2 * cz;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= (long)nx * ny * nz) return;
-    const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+    int i, j, k;
+        dec3(p, ny, nz, i, j, k);
 #define O(a, b, c) out[((long)(a)*ny + (b)) * nz + (c)]
     if (stage == 0) {
         if (!(i & 1) && !(j & 1) && !(k & 1)) O(i, j, k) = in[((long)(i / 2) * cy + j / 2) * cz + k / 2];
@@ -582,6 +735,8 @@ Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed
     KF_HIP(hipMalloc(&d_stat_, sizeof(double) * 4));
     KF_HIP(hipHostMalloc(&h_stat_, sizeof(double) * 4, hipHostMallocDefault));
     KF_HIP(hipMalloc(&d_scal_, sizeof(double) * 16));
+    KF_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
+    KF_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
     krylov_ = (ax[0].type != 1) || (ax[1].type != 1) || (ax[2].type != 1);
     {
         const size_t es = mixed_ ? sizeof(float) : sizeof(double);
@@ -613,6 +768,8 @@ Poisson::~Poisson()
     (void)hipHostFree(h_ctl_);
     (void)hipFree(corr64_);
     (void)hipFree(d_scal_);
+    (void)hipEventDestroy(ev_[0]);
+    (void)hipEventDestroy(ev_[1]);
     for (auto &v : kv_) (void)hipFree(v);
 }
 
@@ -660,9 +817,60 @@ void Poisson::smooth(int l, int sweeps, hipStream_t s)
 }
 
 template <typename R>
+CoarseSet<R> Poisson::coarse_set(int l0) const
+{
+    CoarseSet<R> cs{};
+    cs.nl = (int)lv_.size() - l0;
+    if (cs.nl > COARSE_MAXL) throw Error("too many coarse levels");
+    for (int i = 0; i < cs.nl; ++i) {
+        const Level &L = lv_[l0 + i];
+        cs.L[i] = view<R>(l0 + i);
+        if constexpr (std::is_same<R, double>::value) {
+            cs.x[i] = L.x64; cs.b[i] = L.b64; cs.e[i] = L.e64; cs.r[i] = L.r64;
+        } else {
+            cs.x[i] = L.x32; cs.b[i] = L.b32; cs.e[i] = L.e32; cs.r[i] = L.r32;
+        }
+    }
+    cs.nu1 = nu1_;
+    cs.nu2 = nu2_;
+    cs.nuc = nuc_;
+    return cs;
+}
+
+int Poisson::first_coarse() const
+{
+    for (int l = 0; l < (int)lv_.size(); ++l)
+        if (lv_[l].N <= COARSE_N && (int)lv_.size() - l <= COARSE_MAXL) return l;
+    return (int)lv_.size() - 1;
+}
+
+// exp(phi) on every level: full weighting down the hierarchy
+template <typename R>
+void Poisson::restrict_e(hipStream_t s, const int *ctl)
+{
+    const int lc = first_coarse();
+    for (int l = 0; l < lc; ++l) {
+        Level &A = lv_[l], &B = lv_[l + 1];
+        const R *ea;
+        R *eb;
+        if constexpr (std::is_same<R, double>::value) { ea = A.e64; eb = B.e64; }
+        else { ea = A.e32; eb = B.e32; }
+        hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1], A.n[2],
+                           eb, ctl);
+    }
+    if (lc < (int)lv_.size() - 1)
+        hipLaunchKernelGGL(k_coarse_restrict_e<R>, dim3(1), dim3(COARSE_BLOCK), 0, s, coarse_set<R>(lc), ctl);
+}
+
+template <typename R>
 void Poisson::vcycle(int l, hipStream_t s)
 {
     const int nl = (int)lv_.size();
+    if (l == first_coarse() && l < nl - 1) {
+        hipLaunchKernelGGL(k_coarse_vcycle<R>, dim3(1), dim3(COARSE_BLOCK), 0, s, coarse_set<R>(l),
+                           (const int *)d_ctl_);
+        return;
+    }
     if (l == nl - 1) {
         smooth<R>(l, nuc_, s);
         return;
@@ -685,51 +893,50 @@ void Poisson::vcycle(int l, hipStream_t s)
     smooth<R>(l, nu2_, s);
 }
 
-// Newton iterations with FP64 residual/update and R-precision linear MG
+// Newton iterations with FP64 residual/update and R-precision linear MG.
+// The host stays one iteration ahead: while the GPU corrects iteration k,
+// the host waits for the convergence flag of iteration k's check.
 template <typename R>
 void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipStream_t s)
 {
     Level &L0 = lv_[0];
     const long N = L0.N;
-    const unsigned nb = nblocks(N, RED_BLOCK);
+    unsigned nbr = nblocks(N, RED_BLOCK);
+    if (nbr > RED_MAX_BLOCKS) nbr = RED_MAX_BLOCKS;
     LevelView<double> v64 = view<double>(0);
     R *x0, *b0, *e0;
     if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
     else { x0 = L0.x32; b0 = L0.b32; e0 = L0.e32; }
+    const int *ctl = d_ctl_;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
-    for (int it = 0; it <= max_it_; ++it) {
+    auto check = [&](int it) {
         if (rho64)
-            hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nb), dim3(RED_BLOCK), 0, s, v64,
-                               (const double *)phi, rho64, b0, e0, d_red_, (const int *)d_ctl_);
+            hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
+                               (const double *)phi, rho64, b0, e0, d_red_, ctl);
         else
-            hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nb), dim3(RED_BLOCK), 0, s, v64,
-                               (const double *)phi, rho32, b0, e0, d_red_, (const int *)d_ctl_);
-        hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nb,
+            hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
+                               (const double *)phi, rho32, b0, e0, d_red_, ctl);
+        hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr,
                            it, max_it_, tol_, d_ctl_, d_fail_, d_stat_);
-        if (it == max_it_) break;
-        // host-synchronous early exit: one pinned read per Newton iteration
-        KF_HIP(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
-        KF_HIP(hipStreamSynchronize(s));
-        if (h_ctl_[0]) break;
-        // coefficient e on coarse levels
-        for (size_t l = 0; l + 1 < lv_.size(); ++l) {
-            Level &A = lv_[l], &B = lv_[l + 1];
-            const R *ea;
-            R *eb;
-            if constexpr (std::is_same<R, double>::value) { ea = A.e64; eb = B.e64; }
-            else { ea = A.e32; eb = B.e32; }
-            hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1],
-                               A.n[2], eb, (const int *)d_ctl_);
-        }
+        KF_HIP(hipMemcpyAsync(h_ctl_ + 4 * (it & 1), d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+        KF_HIP(hipEventRecord(ev_[it & 1], s));
+    };
+    auto correct = [&]() {
+        restrict_e<R>(s, ctl);
         if (krylov_) {
             bicgstab<R>(kiters_, s);
         } else {
-            hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0,
-                               (const int *)d_ctl_);
+            hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0, ctl);
             for (int c = 0; c < inner_; ++c) vcycle<R>(0, s);
         }
-        hipLaunchKernelGGL(k_update<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const R *)x0, N,
-                           (const int *)d_ctl_);
+        hipLaunchKernelGGL(k_update<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const R *)x0, N, ctl);
+    };
+    check(0);
+    for (int it = 0; it < max_it_; ++it) {
+        correct();
+        check(it + 1);
+        KF_HIP(hipEventSynchronize(ev_[it & 1]));
+        if (h_ctl_[4 * (it & 1)]) break;
     }
 }
 
@@ -798,6 +1005,7 @@ void Poisson::bicgstab(int iters, hipStream_t s)
 
 int Poisson::solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync)
 {
+    prof::Scope ps_(prof::POISSON, s);
     if (mode_ == POISSON_NEWTON_MG) {
         if (mixed_) newton<float>(phi, rho64, rho32, s);
         else newton<double>(phi, rho64, rho32, s);
@@ -995,10 +1203,7 @@ int Poisson::linear_solve(double *x, const double *b, double rtol, int max_cycle
     KF_HIP(hipMemcpyAsync(L0.b64, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     KF_HIP(hipMemcpyAsync(L0.x64, x, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
-    for (size_t l = 0; l + 1 < lv_.size(); ++l)
-        hipLaunchKernelGGL(k_restrict<double>, dim3(nblocks(lv_[l + 1].N, 256)), dim3(256), 0, s,
-                           (const double *)lv_[l].e64, lv_[l].n[0], lv_[l].n[1], lv_[l].n[2], lv_[l + 1].e64,
-                           (const int *)d_ctl_);
+    restrict_e<double>(s, d_ctl_);
     std::vector<double> hr(N), hb(N);
     KF_HIP(hipMemcpyAsync(hb.data(), b, sizeof(double) * N, hipMemcpyDeviceToHost, s));
     KF_HIP(hipStreamSynchronize(s));

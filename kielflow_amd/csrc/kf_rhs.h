@@ -38,6 +38,10 @@ class Rhs {
     // F[0..3]: stage input, overwritten with the RHS (same precision R)
     void eval(double t, C2<R> *F[4]);
     void set_force_general(bool v) { force_general_ = v; }
+    // RK stage (0..3) of the next evaluations, or -1.  Within an RK run the
+    // Poisson solve of stage s starts from the linear extrapolation in time
+    // of the last two converged potentials of stage s.
+    void set_stage(int s);
     void set_params(const Params &p) { pa_ = p; }
     const Params &params() const { return pa_; }
     Poisson &poisson() { return ps_; }
@@ -74,6 +78,9 @@ class Rhs {
     C2<R> *lin_[4] = {nullptr, nullptr, nullptr, nullptr};
     C2<R> *F1_ = nullptr;
     C2<R> *dF_[12] = {};
+    int stage_ = -1;
+    double *ph_[4][2] = {};
+    int hc_[4] = {0, 0, 0, 0};
 };
 
 // RK4 with an FP64 state and accumulator; stages evaluated in precision R.

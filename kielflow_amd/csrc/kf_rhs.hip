@@ -53,10 +53,22 @@ template <typename R>
 __device__ __forceinline__ void decode_x(const GridView<R> &g, long q, int t, int &i, int &j, int &k,
                                          long &idx)
 {
+    const unsigned u = (unsigned)q, z = (unsigned)g.nzc;
+    const unsigned r = u / z;
     i = t;
-    j = (int)(q / g.nzc);
-    k = (int)(q % g.nzc);
+    j = (int)r;
+    k = (int)(u - r * z);
     idx = (long)t * g.ny * g.nzc + q;
+}
+
+__device__ __forceinline__ void dec3c(long p, int ny, int nz, int &i, int &j, int &k)
+{
+    const unsigned u = (unsigned)p, z = (unsigned)nz, y = (unsigned)ny;
+    const unsigned r = u / z;
+    k = (int)(u - r * z);
+    const unsigned q = r / y;
+    j = (int)(r - q * y);
+    i = (int)q;
 }
 
 // ------------------------------------------------------------------ functors
@@ -104,7 +116,8 @@ struct ZStore {
                 p0 = prof[2 * n];
                 p1 = prof[2 * n + 1];
             } else {
-                const int id = (pdir == 1) ? (int)(row % ny) : (int)(row / ny);
+                const unsigned rr = (unsigned)row, yy = (unsigned)ny, qq = rr / yy;
+                const int id = (pdir == 1) ? (int)(rr - qq * yy) : (int)qq;
                 p0 = p1 = prof[id];
             }
             *reinterpret_cast<C *>(o0 + base) = mk<C>(x0 * p0, x1 * p1);
@@ -127,7 +140,7 @@ struct ZStore {
             const C f = *reinterpret_cast<const C *>(a0 + base);
             const C ad = *reinterpret_cast<const C *>(a1 + base);
             const C H = *reinterpret_cast<const C *>(a2 + base);
-            const R b = a3[row / ny];
+            const R b = a3[(unsigned)row / (unsigned)ny];
             const R t0 = flag ? c1 * H.x : c1, t1 = flag ? c1 * H.y : c1;
             R p0 = c0 * H.x * (f.x + t0);
             R p1 = c0 * H.y * (f.y + t1);
@@ -242,9 +255,9 @@ static void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
     GridView<R> gv = g.view<R>();
     StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
     launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
-                          StorePlain<R>{p.scratch, ax}, s);
+                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X);
     launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s);
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
     launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc}, st, s);
 }
 
@@ -255,8 +268,8 @@ static void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, h
     launch_zr2c<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<R, RS>{src, p.nz},
                    RowStoreC<R>{p.scratch, p.nzc}, s);
     launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s);
-    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s);
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
+    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X);
 }
 
 template <typename R>
@@ -289,7 +302,8 @@ __global__ void k_spec(GridView<R> g, const C2<R> *__restrict__ in, C2<R> *out, 
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nc) return;
-    const int k = (int)(idx % g.nzc), j = (int)((idx / g.nzc) % g.ny), i = (int)(idx / ((long)g.nzc * g.ny));
+    int i, j, k;
+    dec3c(idx, g.ny, g.nzc, i, j, k);
     C2<R> v = in[idx];
     if (d23 && !keep23(g, i, j, k)) v = mk<C2<R>>((R)0, (R)0);
     if (filt) {
@@ -311,7 +325,8 @@ __global__ void k_lin_init(GridView<R> g, const C2<R> *F0, const C2<R> *F1, cons
 {
     const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= g.nc) return;
-    const int k = (int)(idx % g.nzc), j = (int)((idx / g.nzc) % g.ny), i = (int)(idx / ((long)g.nzc * g.ny));
+    int i, j, k;
+    dec3c(idx, g.ny, g.nzc, i, j, k);
     const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
     const R fac = sv_on ? tau + sv : tau;
     const C2<R> *F[3] = {F0, F1, F2};
@@ -426,6 +441,8 @@ Rhs<R>::~Rhs()
     (void)hipFree(F1_);
     for (auto &d : dF_) (void)hipFree(d);
     (void)hipFree(phiR_);
+    for (auto &st : ph_)
+        for (auto &p : st) (void)hipFree(p);
 }
 
 template <typename R>
@@ -447,6 +464,23 @@ int Rhs<R>::poisson_failures()
     return f;
 }
 
+__global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h1, const double *__restrict__ h0,
+                         long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) phi[i] = 2. * h1[i] - h0[i];
+}
+
+template <typename R>
+void Rhs<R>::set_stage(int s)
+{
+    stage_ = s;
+    if (s >= 0 && !ph_[0][0]) {
+        for (auto &st : ph_)
+            for (auto &p : st) KF_HIP(hipMalloc(&p, sizeof(double) * g_.nr));
+    }
+}
+
 // rhs_standard.cpp:100-113: rho = exp(iFFT(D23 Fn)) + n_d, then Poisson
 template <typename R>
 void Rhs<R>::poisson_step(C2<R> *Fn)
@@ -457,8 +491,17 @@ void Rhs<R>::poisson_step(C2<R> *Fn)
     z.a0 = nd_;
     c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
     if (!phi_) throw Error("RHS potential field not set");
+    const long nr = g_.nr;
+    if (stage_ >= 0 && hc_[stage_] >= 2)
+        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, (const double *)ph_[stage_][1],
+                           (const double *)ph_[stage_][0], nr);
     if constexpr (std::is_same<R, double>::value) ps_.solve(phi_, rho_, nullptr, s_, false);
     else ps_.solve(phi_, nullptr, rho_, s_, false);
+    if (stage_ >= 0) {
+        std::swap(ph_[stage_][0], ph_[stage_][1]);
+        KF_HIP(hipMemcpyAsync(ph_[stage_][1], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        hc_[stage_]++;
+    }
 }
 
 template <typename R>
@@ -691,10 +734,12 @@ void Rk4<R>::step(double2 *y[4])
     const double c[3] = {0.5 * dt_, 0.5 * dt_, 1.0 * dt_};
     const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
     for (int st = 0; st < 4; ++st) {
+        rhs_.set_stage(st);
         rhs_.eval(tt[st], S_);
         if (st < 3)
             hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0);
     }
+    rhs_.set_stage(-1);
     hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, y[0], y[1], y[2], y[3], nc, dt_ / 6.);
     KF_HIP(hipGetLastError());
     t_ += dt_;
@@ -745,7 +790,8 @@ __global__ void k_cfl(const double *__restrict__ Ux, const double *__restrict__ 
     const long p = (long)blockIdx.x * 256 + threadIdx.x;
     double cx = 0., cy = 0., cz = 0.;
     if (p < N) {
-        const int k = (int)(p % nz), j = (int)((p / nz) % ny), i = (int)(p / ((long)nz * ny));
+        int i, j, k;
+        dec3c(p, ny, nz, i, j, k);
         cx = dt * fabs(Ux[p]) / dx[i];
         cy = dt * fabs(Uy[p]) / dy[j];
         cz = dt * fabs(Uz[p]) / dz[k];
