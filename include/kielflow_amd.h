@@ -168,6 +168,37 @@ int kf_rk4_time(kf_rk4 k, double *t);
 int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
                double *dt_new);
 
+/* ---------------------------------------------------------------- adjoint (FP64)
+ * Discrete adjoint of rhs_standard::solve, of Runge_kutta_O4::solve and of
+ * the grain drag J = -sum n_d dV d_x phi (phi of the final state), with
+ * checkpointed reverse sweeps.  No counterpart in the reference (it has no
+ * differentiation); the forward operators are the ones of the entries
+ * above.  Cotangents use the Euclidean inner product of the stored numbers
+ * (spectral fields as 2 Nx Ny (Nz/2+1) reals). */
+typedef struct kf_adjoint_s *kf_adjoint;
+int kf_adjoint_create(kf_rhs r, double dt, kf_adjoint *out);
+int kf_adjoint_destroy(kf_adjoint a);
+/* tolerance of the transposed Poisson solves (default 1e-12) */
+int kf_adjoint_set_tolerance(kf_adjoint a, double tol);
+int kf_adjoint_reset(kf_adjoint a);
+/* S: stage input (4 spectral fields), ob: cotangent of the RHS output;
+ * Fb receives the cotangent of S; M_bar and n_d_bar accumulate */
+int kf_rhs_adjoint(kf_adjoint a, const double *const S[4], const double *const ob[4], double *const Fb[4]);
+/* y0: state before the step; yb: cotangent after the step on entry, before
+ * the step on return */
+int kf_rk4_step_adjoint(kf_adjoint a, const double *const y0[4], double *const yb[4]);
+/* J of a state (dV: cell volumes of the quadrature, real field) */
+int kf_objective(kf_adjoint a, const double *const F[4], const double *dV, double *J);
+/* dJ/dF of the state of the last kf_objective call; n_d_bar accumulates */
+int kf_objective_adjoint(kf_adjoint a, const double *const F[4], const double *dV, double *const Fb[4]);
+/* accumulated M_bar (scalar) and n_d_bar (device real field, copied to nd_bar) */
+int kf_adjoint_accumulators(kf_adjoint a, double *M_bar, double *nd_bar);
+/* forward nsteps from y (in place), checkpoints every k steps, J at the end,
+ * reverse sweep with stage recomputation: J, dJ/dM, dJ/dQ (n_d = Q G, G the
+ * unit-charge grain shape) and dJ/dFn0 (spectral, device) */
+int kf_adjoint_run(kf_adjoint a, double *const y[4], int nsteps, int k, const double *dV, const double *G,
+                   double *J, double *dJdM, double *dJdQ, double *dJdFn0);
+
 /* ---------------------------------------------------------------- launch timing
  * Diagnostic for roofline figures (no counterpart in the reference, which
  * has no profiler hooks, logger.hpp:13-77 aside).  family: 0 FFT x passes,

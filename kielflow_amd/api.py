@@ -89,6 +89,16 @@ SIGNATURES = {
     "kf_rk4_set_dt": (C.c_int, [_vp, C.c_double]),
     "kf_rk4_time": (C.c_int, [_vp, _dp]),
     "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
+    "kf_adjoint_create": (C.c_int, [_vp, C.c_double, C.POINTER(_vp)]),
+    "kf_adjoint_destroy": (C.c_int, [_vp]),
+    "kf_adjoint_set_tolerance": (C.c_int, [_vp, C.c_double]),
+    "kf_adjoint_reset": (C.c_int, [_vp]),
+    "kf_rhs_adjoint": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp), C.POINTER(_vp)]),
+    "kf_rk4_step_adjoint": (C.c_int, [_vp, C.POINTER(_vp), C.POINTER(_vp)]),
+    "kf_objective": (C.c_int, [_vp, C.POINTER(_vp), _vp, _dp]),
+    "kf_objective_adjoint": (C.c_int, [_vp, C.POINTER(_vp), _vp, C.POINTER(_vp)]),
+    "kf_adjoint_accumulators": (C.c_int, [_vp, _dp, _vp]),
+    "kf_adjoint_run": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp, _dp, _dp, _dp, _vp]),
     "kf_profile_enable": (C.c_int, [C.c_int]),
     "kf_profile_read": (C.c_int, [_ip, _dp]),
     "kf_setup_host": (C.c_int, [_AX3, C.c_ulonglong, C.c_double, C.c_double, C.c_double, _vp, _vp,
@@ -364,6 +374,58 @@ class Rk4:
         v = C.c_double(0)
         check(self.rhs.grid.ctx.lib.kf_rk4_time(self.h, C.byref(v)))
         return v.value
+
+
+def _ptrs4(fields):
+    return (_vp * 4)(*[f.ptr for f in fields])
+
+
+class Adjoint:
+    """Discrete adjoint of the FP64 RHS / RK4 step / grain drag objective."""
+
+    def __init__(self, rhs: Rhs, dt: float, tol: float = 1e-12):
+        self.rhs = rhs
+        h = _vp()
+        check(rhs.grid.ctx.lib.kf_adjoint_create(rhs.h, dt, C.byref(h)))
+        self.h = h
+        check(rhs.grid.ctx.lib.kf_adjoint_set_tolerance(h, tol))
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.rhs.grid.ctx.lib.kf_adjoint_destroy(self.h)
+            self.h = None
+
+    @property
+    def lib(self):
+        return self.rhs.grid.ctx.lib
+
+    def reset(self):
+        check(self.lib.kf_adjoint_reset(self.h))
+
+    def rhs_adjoint(self, S, ob, Fb):
+        check(self.lib.kf_rhs_adjoint(self.h, _ptrs4(S), _ptrs4(ob), _ptrs4(Fb)))
+
+    def step_adjoint(self, y0, yb):
+        check(self.lib.kf_rk4_step_adjoint(self.h, _ptrs4(y0), _ptrs4(yb)))
+
+    def objective(self, F, dV):
+        J = C.c_double(0)
+        check(self.lib.kf_objective(self.h, _ptrs4(F), dV.ptr, C.byref(J)))
+        return J.value
+
+    def objective_adjoint(self, F, dV, Fb):
+        check(self.lib.kf_objective_adjoint(self.h, _ptrs4(F), dV.ptr, _ptrs4(Fb)))
+
+    def accumulators(self, nd_bar=None):
+        m = C.c_double(0)
+        check(self.lib.kf_adjoint_accumulators(self.h, C.byref(m), None if nd_bar is None else nd_bar.ptr))
+        return m.value
+
+    def run(self, y, nsteps, k, dV, G, dFn0):
+        J, dM, dQ = C.c_double(0), C.c_double(0), C.c_double(0)
+        check(self.lib.kf_adjoint_run(self.h, _ptrs4(y), nsteps, k, dV.ptr, G.ptr, C.byref(J), C.byref(dM),
+                                      C.byref(dQ), dFn0.ptr))
+        return J.value, dM.value, dQ.value
 
 
 PROF_FFT_X, PROF_FFT_Y, PROF_FFT_ZR2C, PROF_FFT_ZC2R, PROF_POISSON = 0, 1, 2, 3, 4

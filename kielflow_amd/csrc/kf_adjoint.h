@@ -1,0 +1,65 @@
+// kf_adjoint.h -- discrete adjoint of the FP64 RHS, of one RK4 step and of
+// the grain drag objective, plus a checkpointed reverse sweep over a run.
+//
+// Inner products are the Euclidean ones of the stored numbers (a spectral
+// field is 2*Nx*Ny*(Nz/2+1) reals).  With that convention
+//   R2C^T = N * C2R(W .)       W = 1/2 on interior kz planes, 1 on kz = 0, Nz/2
+//   C2R^T = (1/N) W' R2C(.)    W' = 2 on interior kz planes, 1 on kz = 0, Nz/2
+//   (i k)^T = -i k,  D23^T = D23,  (Poisson)^T: rho_bar = -J^{-T} phi_bar
+// and every transform of the forward RHS (rhs_standard.cpp:89-566) is
+// mirrored by one of the reverse sweep.
+#pragma once
+
+#include "kf_rhs.h"
+
+namespace kf {
+
+class Adjoint {
+  public:
+    Adjoint(Rhs<double> &rhs, double dt);
+    ~Adjoint();
+    Adjoint(const Adjoint &) = delete;
+    Adjoint &operator=(const Adjoint &) = delete;
+
+    // cotangent of one RHS evaluation at stage input S: ob -> Fb (written),
+    // and accumulation into M_bar (device scalar) and nd_bar (real field).
+    // Runs the forward evaluation with capture first when `recompute`.
+    void rhs_adjoint(const double2 *S[4], const double2 *ob[4], double2 *Fb[4], bool recompute = true);
+    // adjoint of one RK4 step from state y0: yb holds ybar_{n+1} on entry and
+    // ybar_n on return (M_bar / nd_bar accumulate)
+    void step_adjoint(const double2 *y0[4], double2 *yb[4]);
+
+    // J = -sum_p W(p) d_x phi(p), W = n_d dV, phi = Poisson(exp(iFFT(D23 Fn)) + n_d)
+    double objective(const double2 *F[4], const double *W);
+    // dJ/dF (Fn component only, others zero) and nd_bar accumulation
+    void objective_adjoint(const double2 *F[4], const double *W, double2 *Fb[4]);
+
+    // forward run of nsteps from y (in place) with checkpoints every k steps,
+    // objective at the end, reverse sweep; returns J and the gradients
+    // dJ/dM, dJ/dQ (through n_d = Q G) and dJ/dFn0 (spectral, device).
+    double run(double2 *y[4], int nsteps, int k, const double *W, const double *G, double *dM, double *dQ,
+               double2 *dFn0);
+
+    void reset_accumulators();
+    double M_bar();              // synchronises
+    double *nd_bar() { return ndb_; }
+    void set_tolerance(double t) { ltol_ = t; }
+
+  private:
+    void deriv_T(const double2 *Y, int b, double2 *out, int acc, double coef, int d23_out);
+    Rhs<double> &rhs_;
+    Grid &g_;
+    fft::Plan<double> &p_;
+    hipStream_t s_;
+    double dt_;
+    double ltol_ = 1e-12;
+    Capture cap_[4];
+    double2 *stage_in_[4][4] = {};   // stage inputs of the recomputed step
+    double2 *Fb_[4] = {}, *linb_[4] = {}, *F1b_ = nullptr, *Yt_ = nullptr, *kb_[4][4] = {};
+    double *advb_[4] = {}, *Ub_[3] = {}, *Lb_ = nullptr, *rt_ = nullptr, *Pb_ = nullptr;
+    double *phib_ = nullptr, *rhob_ = nullptr, *ndb_ = nullptr, *phiJ_ = nullptr, *E_ = nullptr;
+    double *Mb_ = nullptr, *part_ = nullptr;
+    double2 *yscratch_[4] = {};
+};
+
+}  // namespace kf

@@ -7,6 +7,7 @@
 #include <memory>
 #include <string>
 
+#include "kf_adjoint.h"
 #include "kf_fft.h"
 #include "kf_grid.h"
 #include "kf_poisson.h"
@@ -45,6 +46,10 @@ struct kf_rhs_s {
     Rhs<double> *r64;
     Rhs<float> *r32;
     float2 *conv[4];
+};
+struct kf_adjoint_s {
+    kf_rhs rhs;
+    Adjoint *a;
 };
 struct kf_rk4_s {
     kf_rhs rhs;
@@ -601,6 +606,133 @@ int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const
         KF_REQUIRE(g && FUx && FUy && FUz && dt_new, "null argument");
         const double2 *F[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
         *dt_new = cfl_all(*g->g, *g->p64, dt, F, g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+// ------------------------------------------------------------------ adjoint
+int kf_adjoint_create(kf_rhs r, double dt, kf_adjoint *out)
+{
+    return guard([&] {
+        KF_REQUIRE(r && out, "null argument");
+        KF_REQUIRE(r->r64, "the adjoint runs in FP64: create the RHS with KF_FP64");
+        set_dev(r->grid->ctx);
+        *out = new kf_adjoint_s{r, new Adjoint(*r->r64, dt)};
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_destroy(kf_adjoint a)
+{
+    return guard([&] {
+        if (!a) return KF_OK;
+        delete a->a;
+        delete a;
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_set_tolerance(kf_adjoint a, double tol)
+{
+    return guard([&] {
+        KF_REQUIRE(a && tol > 0, "bad argument");
+        a->a->set_tolerance(tol);
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_reset(kf_adjoint a)
+{
+    return guard([&] {
+        KF_REQUIRE(a, "null adjoint");
+        a->a->reset_accumulators();
+        return KF_OK;
+    });
+}
+
+int kf_rhs_adjoint(kf_adjoint a, const double *const S[4], const double *const ob[4], double *const Fb[4])
+{
+    return guard([&] {
+        KF_REQUIRE(a && S && ob && Fb, "null argument");
+        const double2 *s4[4], *o4[4];
+        double2 *f4[4];
+        for (int i = 0; i < 4; ++i) {
+            s4[i] = (const double2 *)S[i];
+            o4[i] = (const double2 *)ob[i];
+            f4[i] = (double2 *)Fb[i];
+        }
+        a->a->rhs_adjoint(s4, o4, f4, true);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_rk4_step_adjoint(kf_adjoint a, const double *const y0[4], double *const yb[4])
+{
+    return guard([&] {
+        KF_REQUIRE(a && y0 && yb, "null argument");
+        const double2 *y[4];
+        double2 *b[4];
+        for (int i = 0; i < 4; ++i) {
+            y[i] = (const double2 *)y0[i];
+            b[i] = (double2 *)yb[i];
+        }
+        a->a->step_adjoint(y, b);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_objective(kf_adjoint a, const double *const F[4], const double *dV, double *J)
+{
+    return guard([&] {
+        KF_REQUIRE(a && F && dV && J, "null argument");
+        const double2 *f[4];
+        for (int i = 0; i < 4; ++i) f[i] = (const double2 *)F[i];
+        *J = a->a->objective(f, dV);
+        return KF_OK;
+    });
+}
+
+int kf_objective_adjoint(kf_adjoint a, const double *const F[4], const double *dV, double *const Fb[4])
+{
+    return guard([&] {
+        KF_REQUIRE(a && F && dV && Fb, "null argument");
+        const double2 *f[4];
+        double2 *b[4];
+        for (int i = 0; i < 4; ++i) {
+            f[i] = (const double2 *)F[i];
+            b[i] = (double2 *)Fb[i];
+        }
+        a->a->objective_adjoint(f, dV, b);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_accumulators(kf_adjoint a, double *M_bar, double *nd_bar)
+{
+    return guard([&] {
+        KF_REQUIRE(a, "null adjoint");
+        if (M_bar) *M_bar = a->a->M_bar();
+        if (nd_bar) {
+            Grid &g = *a->rhs->grid->g;
+            KF_HIP(hipMemcpyAsync(nd_bar, a->a->nd_bar(), sizeof(double) * g.nr, hipMemcpyDeviceToDevice,
+                                  a->rhs->grid->ctx->stream));
+            KF_HIP(hipStreamSynchronize(a->rhs->grid->ctx->stream));
+        }
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_run(kf_adjoint a, double *const y[4], int nsteps, int k, const double *dV, const double *G, double *J,
+                   double *dJdM, double *dJdQ, double *dJdFn0)
+{
+    return guard([&] {
+        KF_REQUIRE(a && y && dV && G && J && nsteps >= 0 && k >= 1, "bad argument");
+        double2 *yy[4];
+        for (int i = 0; i < 4; ++i) yy[i] = (double2 *)y[i];
+        *J = a->a->run(yy, nsteps, k, dV, G, dJdM, dJdQ, (double2 *)dJdFn0);
         return KF_OK;
     });
 }

@@ -80,9 +80,14 @@ class Poisson {
     double tolerance() const { return tol_; }
     int max_iterations() const { return max_it_; }
 
-    // Linear solve J x = b (J at the phi of the last solve) for the adjoint:
-    // x, b FP64 device fields; cycles V-cycles at most, relative tol.
-    int linear_solve(double *x, const double *b, double rtol, int max_cycles, hipStream_t s);
+    // J = L_h - diag(exp(phi)) at the given phi (FP64 operator of the adjoint)
+    void set_linearization(const double *phi, hipStream_t s);
+    // J x = b to max|b - J x| <= rtol max|b| (FP64; x overwritten; defect
+    // correction with one V-cycle, or two BiCGStab steps on stretched grids,
+    // per iteration).  Returns iterations, or -1 when max_it is reached.
+    int linear_solve(double *x, const double *b, double rtol, int max_it, hipStream_t s);
+    // J^T x = b through  V J V^{-1} = J^T  (V = cell volumes, V L_h symmetric)
+    int linear_solve_T(double *x, const double *b, double rtol, int max_it, hipStream_t s);
 
     // phi: FP64 in/out (warm start), rho: FP64 or FP32 real field.
     // Enqueues the solve on stream s.  When sync is true the call waits and
@@ -103,6 +108,7 @@ class Poisson {
     int nlj_solve_level(int level, double *phi, const double *rho, hipStream_t s);
 
     int levels() const { return (int)lv_.size(); }
+    bool ax_desc_uniform() const { return ax_[0].type == 1 && ax_[1].type == 1 && ax_[2].type == 1; }
 
   private:
     struct Level {
@@ -156,6 +162,8 @@ class Poisson {
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)
     double *d_scal_ = nullptr;    // device scalars of BiCGStab
     hipEvent_t ev_[2] = {};       // convergence-flag copies of the look-ahead loop
+    double *lsx_ = nullptr, *lsr_ = nullptr, *lsb_ = nullptr;  // linear solve work
+    double *vol_[3] = {};         // per-axis cell volumes (H[i+1]+H[i])/2
     NljState nlj_;
     hipStream_t s0_;
 };

@@ -768,6 +768,10 @@ Poisson::~Poisson()
     (void)hipHostFree(h_ctl_);
     (void)hipFree(corr64_);
     (void)hipFree(d_scal_);
+    (void)hipFree(lsx_);
+    (void)hipFree(lsr_);
+    (void)hipFree(lsb_);
+    for (auto &v : vol_) (void)hipFree(v);
     (void)hipEventDestroy(ev_[0]);
     (void)hipEventDestroy(ev_[1]);
     for (auto &v : kv_) (void)hipFree(v);
@@ -1192,41 +1196,123 @@ int Poisson::solve_cascade(double *phi, const double *rho, hipStream_t s)
     return rc < 0 ? rc : total;
 }
 
-int Poisson::linear_solve(double *x, const double *b, double rtol, int max_cycles, hipStream_t s)
+__global__ void k_exp_field(const double *__restrict__ phi, double *__restrict__ e, long n)
 {
-    // J x = b with J = L_h - diag(e), e = exp(phi) left by the last Newton
-    // residual on every level (FP64 coefficients).
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) e[i] = exp(phi[i]);
+}
+
+// out = in * V (mode 1) or in / V (mode 0), V = vx[i] vy[j] vz[k]
+__global__ void k_vscale(const double *__restrict__ in, double *__restrict__ out, const double *__restrict__ vx,
+                         const double *__restrict__ vy, const double *__restrict__ vz, int nx, int ny, int nz,
+                         int mode)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (long)nx * ny * nz) return;
+    int i, j, k;
+    dec3(p, ny, nz, i, j, k);
+    const double v = vx[i] * vy[j] * vz[k];
+    out[p] = mode ? in[p] * v : in[p] / v;
+}
+
+__global__ void __launch_bounds__(RED_BLOCK)
+k_absmax2(const double *__restrict__ a, const double *__restrict__ b, long n, double *__restrict__ part)
+{
+    __shared__ double sh[RED_BLOCK / 64];
+    double ma = 0., mb = 0.;
+    for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < n; p += (long)gridDim.x * RED_BLOCK) {
+        ma = fmax(ma, fabs(a[p]));
+        mb = fmax(mb, fabs(b[p]));
+    }
+    ma = block_max(ma, sh);
+    mb = block_max(mb, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = ma;
+        part[2 * blockIdx.x + 1] = mb;
+    }
+}
+
+__global__ void k_axpy64(double *__restrict__ y, const double *__restrict__ x, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] += x[i];
+}
+
+void Poisson::set_linearization(const double *phi, hipStream_t s)
+{
+    const long N = lv_[0].N;
+    hipLaunchKernelGGL(k_exp_field, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, lv_[0].e64, N);
+}
+
+int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, hipStream_t s)
+{
     Level &L0 = lv_[0];
     const long N = L0.N;
-    const unsigned nb = nblocks(N, RED_BLOCK);
-    (void)nb;
-    KF_HIP(hipMemcpyAsync(L0.b64, b, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-    KF_HIP(hipMemcpyAsync(L0.x64, x, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    const unsigned nb = nblocks(N, 256);
+    unsigned nbr = nblocks(N, RED_BLOCK);
+    if (nbr > RED_MAX_BLOCKS) nbr = RED_MAX_BLOCKS;
+    if (!lsx_) {
+        KF_HIP(hipMalloc(&lsx_, sizeof(double) * N));
+        KF_HIP(hipMalloc(&lsr_, sizeof(double) * N));
+        KF_HIP(hipMalloc(&lsb_, sizeof(double) * N));
+    }
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
     restrict_e<double>(s, d_ctl_);
-    std::vector<double> hr(N), hb(N);
-    KF_HIP(hipMemcpyAsync(hb.data(), b, sizeof(double) * N, hipMemcpyDeviceToHost, s));
-    KF_HIP(hipStreamSynchronize(s));
-    double bmax = 0.;
-    for (long p = 0; p < N; ++p) bmax = std::max(bmax, fabs(hb[p]));
-    if (bmax == 0.) bmax = 1.;
-    int c = 0;
-    for (; c < max_cycles; ++c) {
-        LevelView<double> v = view<double>(0);
-        hipLaunchKernelGGL(k_lin_resid<double>, dim3(nblocks(N, 256)), dim3(256), 0, s, v,
-                           (const double *)L0.x64, (const double *)L0.b64, (const double *)L0.e64, L0.r64,
-                           (const int *)nullptr);
-        KF_HIP(hipMemcpyAsync(hr.data(), L0.r64, sizeof(double) * N, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, lsx_, N, 0., (const int *)nullptr);
+    LevelView<double> v = view<double>(0);
+    std::vector<double> part(2 * (size_t)nbr);
+    int it = 0;
+    for (;; ++it) {
+        hipLaunchKernelGGL(k_lin_resid<double>, dim3(nb), dim3(256), 0, s, v, (const double *)lsx_, b,
+                           (const double *)L0.e64, lsr_, (const int *)nullptr);
+        hipLaunchKernelGGL(k_absmax2, dim3(nbr), dim3(RED_BLOCK), 0, s, (const double *)lsr_, b, N, d_red_);
+        KF_HIP(hipMemcpyAsync(part.data(), d_red_, sizeof(double) * 2 * nbr, hipMemcpyDeviceToHost, s));
         KF_HIP(hipStreamSynchronize(s));
-        double rmax = 0.;
-        for (long p = 0; p < N; ++p) rmax = std::max(rmax, fabs(hr[p]));
-        h_stat_[0] = rmax / bmax;
-        if (rmax <= rtol * bmax) break;
-        vcycle<double>(0, s);
+        double rm = 0., bm = 0.;
+        for (unsigned q = 0; q < nbr; ++q) {
+            rm = std::max(rm, part[2 * q]);
+            bm = std::max(bm, part[2 * q + 1]);
+        }
+        h_stat_[0] = rm / (bm > 0. ? bm : 1.);
+        if (rm <= rtol * bm || it >= max_it) break;
+        KF_HIP(hipMemcpyAsync(L0.b64, lsr_, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+        if (krylov_) {
+            bicgstab<double>(kiters_, s);
+        } else {
+            hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, L0.x64, N, 0., (const int *)d_ctl_);
+            vcycle<double>(0, s);
+        }
+        hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, lsx_, (const double *)L0.x64, N);
     }
-    KF_HIP(hipMemcpyAsync(x, L0.x64, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    KF_HIP(hipMemcpyAsync(x, lsx_, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    return (h_stat_[0] <= rtol) ? it : -1;
+}
+
+int Poisson::linear_solve_T(double *x, const double *b, double rtol, int max_it, hipStream_t s)
+{
+    const bool uniform = ax_desc_uniform();
+    if (uniform) return linear_solve(x, b, rtol, max_it, s);
+    Level &L0 = lv_[0];
+    const long N = L0.N;
+    if (!vol_[0]) {
+        for (int d = 0; d < 3; ++d) {
+            std::vector<double> v(L0.n[d]);
+            for (int i = 0; i < L0.n[d]; ++i) v[i] = 0.5 * (L0.h[d][i + 1] + L0.h[d][i]);
+            KF_HIP(hipMalloc(&vol_[d], sizeof(double) * v.size()));
+            KF_HIP(hipMemcpy(vol_[d], v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
+        }
+    }
+    double *bs = nullptr;
+    KF_HIP(hipMalloc(&bs, sizeof(double) * N));
+    hipLaunchKernelGGL(k_vscale, dim3(nblocks(N, 256)), dim3(256), 0, s, b, bs, (const double *)vol_[0],
+                       (const double *)vol_[1], (const double *)vol_[2], L0.n[0], L0.n[1], L0.n[2], 0);
+    int it = linear_solve(x, bs, rtol, max_it, s);
+    hipLaunchKernelGGL(k_vscale, dim3(nblocks(N, 256)), dim3(256), 0, s, (const double *)x, x,
+                       (const double *)vol_[0], (const double *)vol_[1], (const double *)vol_[2], L0.n[0],
+                       L0.n[1], L0.n[2], 1);
     KF_HIP(hipStreamSynchronize(s));
-    return c < max_cycles ? c : -1;
+    (void)hipFree(bs);
+    return it;
 }
 
 template void Poisson::newton<float>(double *, const double *, const float *, hipStream_t);

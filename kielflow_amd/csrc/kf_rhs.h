@@ -23,6 +23,17 @@ struct Switches {
     int pen_on;  // _RHS_PENALIZATION (rhs_standard.hpp:29)
 };
 
+// Forward intermediates of one evaluation kept for the adjoint (FP64):
+// real U_b, L = iFFT(D23 Fn), phi, G_ab = iFFT(D23 d_b(D23 F_a)) and the
+// spectral d_x(D23 F_a) of a = x, y, z, n.
+struct Capture {
+    double *U[3];
+    double *L;
+    double *phi;
+    double *G[12];
+    double2 *dF0[4];
+};
+
 // RHS in compute precision R (double: FP64 mode; float: mixed mode).
 template <typename R>
 class Rhs {
@@ -42,8 +53,13 @@ class Rhs {
     // Poisson solve of stage s starts from the linear extrapolation in time
     // of the last two converged potentials of stage s.
     void set_stage(int s);
+    // evaluations store their intermediates into *c (general path) while set
+    void set_capture(Capture *c) { cap_ = c; }
     void set_params(const Params &p) { pa_ = p; }
     const Params &params() const { return pa_; }
+    const Switches &switches() const { return sw_; }
+    const R *mask() const { return mask_; }
+    const R *nd() const { return nd_; }
     Poisson &poisson() { return ps_; }
     fft::Plan<R> &plan() { return plan_; }
     Grid &grid() { return g_; }
@@ -79,6 +95,7 @@ class Rhs {
     C2<R> *F1_ = nullptr;
     C2<R> *dF_[12] = {};
     int stage_ = -1;
+    Capture *cap_ = nullptr;
     double *ph_[4][2] = {};
     int hc_[4] = {0, 0, 0, 0};
 };

@@ -9,6 +9,7 @@
 // d_b X = FFT(dS_b * iFFT(D23 i k_b X)) as the reference does
 // (OP_partial_derivative.cpp:104-136).
 #include "kf_rhs.h"
+#include "kf_rhs_impl.h"
 
 #include <cmath>
 #include <vector>
@@ -17,280 +18,7 @@ namespace kf {
 
 using namespace fft;
 
-enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN };
-enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN };
-
-// ------------------------------------------------------------------ spectral factors
-template <typename R>
-__device__ __forceinline__ bool keep23(const GridView<R> &g, int i, int j, int k)
-{
-    return d23_keep(g.q[0][i], g.q[1][j], g.q[2][k], g.d23_thr);
-}
-
-// spectral viscosity 0.5 (1 - exp(-8 sqrt(knorm)^8)) (rhs_standard.cpp:68-71)
-template <typename R>
-__device__ __forceinline__ R sv_factor(const GridView<R> &g, int i, int j, int k)
-{
-    const R s = (R)((g.q[0][i] + g.q[1][j]) + g.q[2][k]);
-    return (R)0.5 * ((R)1 - exp(-(R)8 * (s * s)));
-}
-
-// exp(-2000 kabs^15) with the z norm of operations.hpp:23 (rhs_standard.cpp:220)
-template <typename R>
-__device__ __forceinline__ R phi_filter(const GridView<R> &g, int i, int j, int k)
-{
-    const R kab = sqrt((R)((g.q[0][i] + g.q[1][j]) + g.qzf[k]));
-    return exp((R)-2000 * pow(kab, (R)15));
-}
-
-template <typename R>
-__device__ __forceinline__ C2<R> ik(R kv, C2<R> v)
-{
-    return mk<C2<R>>(-kv * v.y, kv * v.x);
-}
-
-template <typename R>
-__device__ __forceinline__ void decode_x(const GridView<R> &g, long q, int t, int &i, int &j, int &k,
-                                         long &idx)
-{
-    const unsigned u = (unsigned)q, z = (unsigned)g.nzc;
-    const unsigned r = u / z;
-    i = t;
-    j = (int)r;
-    k = (int)(u - r * z);
-    idx = (long)t * g.ny * g.nzc + q;
-}
-
-__device__ __forceinline__ void dec3c(long p, int ny, int nz, int &i, int &j, int &k)
-{
-    const unsigned u = (unsigned)p, z = (unsigned)nz, y = (unsigned)ny;
-    const unsigned r = u / z;
-    k = (int)(u - r * z);
-    const unsigned q = r / y;
-    j = (int)(r - q * y);
-    i = (int)q;
-}
-
-// ------------------------------------------------------------------ functors
-// first pass of an inverse transform: v = [D23] [i k_dir] src
-template <typename R>
-struct XLoadSpec {
-    const C2<R> *src;
-    GridView<R> g;
-    int d23, dir;
-    __device__ __forceinline__ C2<R> operator()(long q, int t) const
-    {
-        int i, j, k;
-        long idx;
-        decode_x(g, q, t, i, j, k, idx);
-        C2<R> v = src[idx];
-        if (d23 && !keep23(g, i, j, k)) return mk<C2<R>>((R)0, (R)0);
-        if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
-        return v;
-    }
-};
-
-// last pass of an inverse transform (real rows, pairs 2n, 2n+1)
-template <typename R>
-struct ZStore {
-    int kind, nz, ny;
-    R scale;
-    R *o0, *o1;
-    const R *a0, *a1, *a2, *a3;
-    R c0, c1;
-    int flag;
-    const R *prof;
-    int pdir;
-    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const
-    {
-        using C = C2<R>;
-        const R x0 = v.x * scale, x1 = v.y * scale;
-        const long base = row * nz + 2 * n;
-        switch (kind) {
-        case ZK_WRITE:
-            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
-            break;
-        case ZK_PROF: {
-            R p0, p1;
-            if (pdir == 2) {
-                p0 = prof[2 * n];
-                p1 = prof[2 * n + 1];
-            } else {
-                const unsigned rr = (unsigned)row, yy = (unsigned)ny, qq = rr / yy;
-                const int id = (pdir == 1) ? (int)(rr - qq * yy) : (int)qq;
-                p0 = p1 = prof[id];
-            }
-            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0 * p0, x1 * p1);
-            break;
-        }
-        case ZK_RHO: {
-            const C d = *reinterpret_cast<const C *>(a0 + base);
-            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
-            *reinterpret_cast<C *>(o1 + base) = mk<C>(exp(x0) + d.x, exp(x1) + d.y);
-            break;
-        }
-        case ZK_PROD: {
-            const C u = *reinterpret_cast<const C *>(a0 + base);
-            C prev = flag ? *reinterpret_cast<const C *>(o0 + base) : mk<C>((R)0, (R)0);
-            *reinterpret_cast<C *>(o0 + base) = mk<C>(prev.x + u.x * x0, prev.y + u.y * x1);
-            break;
-        }
-        case ZK_PEN: {
-            // rhs_standard.cpp:483-490: eps1 H (f + c H) + (1-H) B + b f
-            const C f = *reinterpret_cast<const C *>(a0 + base);
-            const C ad = *reinterpret_cast<const C *>(a1 + base);
-            const C H = *reinterpret_cast<const C *>(a2 + base);
-            const R b = a3[(unsigned)row / (unsigned)ny];
-            const R t0 = flag ? c1 * H.x : c1, t1 = flag ? c1 * H.y : c1;
-            R p0 = c0 * H.x * (f.x + t0);
-            R p1 = c0 * H.y * (f.y + t1);
-            p0 += ((R)1 - H.x) * (x0 + ad.x);
-            p1 += ((R)1 - H.y) * (x1 + ad.y);
-            p0 += b * f.x;
-            p1 += b * f.y;
-            *reinterpret_cast<C *>(o0 + base) = mk<C>(p0, p1);
-            break;
-        }
-        }
-    }
-};
-
-// last pass of a forward transform
-template <typename R>
-struct XStore {
-    int kind;
-    GridView<R> g;
-    C2<R> *o0, *o1, *o2, *o3;
-    const C2<R> *a0, *a1, *a2, *a3;
-    R tau, M, theta;
-    int sv_on;
-    __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const
-    {
-        using C = C2<R>;
-        int i, j, k;
-        long idx;
-        decode_x(g, q, t, i, j, k, idx);
-        switch (kind) {
-        case XK_WRITE:
-            o0[idx] = X;
-            break;
-        case XK_ACC: {
-            C o = o0[idx];
-            o0[idx] = mk<C>(o.x + X.x, o.y + X.y);
-            break;
-        }
-        case XK_NEG_D23:
-            o0[idx] = keep23(g, i, j, k) ? mk<C>(-X.x, -X.y) : mk<C>((R)0, (R)0);
-            break;
-        case XK_NEG_ACC: {
-            C l = a0[idx];
-            o0[idx] = mk<C>(-(l.x + X.x), -(l.y + X.y));
-            break;
-        }
-        case XK_F1: {
-            const R f = phi_filter(g, i, j, k);
-            C n = a0[idx];
-            o0[idx] = mk<C>(f * X.x + n.x / theta, f * X.y + n.y / theta);
-            break;
-        }
-        case XK_LIN: {
-            // rhs_standard.cpp:119-140, 219-247, 279-319 on an equidistant grid
-            const R f = phi_filter(g, i, j, k);
-            const C fn = a3[idx];
-            const C F1 = mk<C>(f * X.x + fn.x / theta, f * X.y + fn.y / theta);
-            const bool kp = keep23(g, i, j, k);
-            const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
-            const R kx = g.k[0][i], ky = g.k[1][j], kz = g.k[2][k];
-            const C *FU[3] = {a0, a1, a2};
-            C *LO[3] = {o0, o1, o2};
-            const R ka[3] = {kx, ky, kz};
-            C divu = mk<C>((R)0, (R)0);
-            C dnx;
-            {
-                const C fd = kp ? fn : mk<C>((R)0, (R)0);
-                dnx = ik(kx, fd);
-            }
-            C ln = sv_on ? mk<C>(sv * fn.x, sv * fn.y) : mk<C>((R)0, (R)0);
-            ln = mk<C>(ln.x + M * dnx.x, ln.y + M * dnx.y);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const C u = FU[a][idx];
-                const R fac = sv_on ? (tau + sv) : tau;
-                C l = mk<C>(fac * u.x, fac * u.y);
-                const C dF1 = ik(ka[a], F1);
-                l = mk<C>(l.x + dF1.x, l.y + dF1.y);
-                const C ud = kp ? u : mk<C>((R)0, (R)0);
-                const C dx = ik(kx, ud);
-                l = mk<C>(l.x + M * dx.x, l.y + M * dx.y);
-                LO[a][idx] = l;
-                const C da = ik(ka[a], ud);
-                divu = (a == 0) ? da : mk<C>(divu.x + da.x, divu.y + da.y);
-                if (a == 0) ln = mk<C>(ln.x + da.x, ln.y + da.y);
-                else ln = mk<C>(ln.x + da.x, ln.y + da.y);
-            }
-            (void)divu;
-            o3[idx] = ln;
-            break;
-        }
-        }
-    }
-};
-
-template <typename R, typename RS>
-struct RowLoadConv {
-    const RS *p;
-    int nz;
-    __device__ __forceinline__ C2<R> operator()(long row, int n) const
-    {
-        const C2<RS> v = reinterpret_cast<const C2<RS> *>(p + row * nz)[n];
-        return mk<C2<R>>((R)v.x, (R)v.y);
-    }
-};
-
-// ------------------------------------------------------------------ transforms with functors
-template <typename R>
-static void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, const ZStore<R> &st,
-                     hipStream_t s)
-{
-    GridView<R> gv = g.view<R>();
-    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
-    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
-                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X);
-    launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
-    launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc}, st, s);
-}
-
-template <typename R, typename RS>
-static void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, hipStream_t s)
-{
-    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
-    launch_zr2c<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<R, RS>{src, p.nz},
-                   RowStoreC<R>{p.scratch, p.nzc}, s);
-    launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
-    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X);
-}
-
-template <typename R>
-static ZStore<R> zs(int kind, const Plan<R> &p)
-{
-    ZStore<R> z{};
-    z.kind = kind;
-    z.nz = p.nz;
-    z.ny = p.ny;
-    z.scale = (R)(1.0 / (double)p.nr);
-    return z;
-}
-
-template <typename R>
-static XStore<R> xs(int kind, Grid &g)
-{
-    XStore<R> x{};
-    x.kind = kind;
-    x.g = g.view<R>();
-    return x;
-}
+// functors and transform helpers: kf_rhs_impl.h
 
 // ------------------------------------------------------------------ pointwise spectral kernels
 static inline unsigned nbk(long n, int b = 256) { return (unsigned)((n + b - 1) / b); }
@@ -360,6 +88,13 @@ __global__ void k_lin_combine(long nc, C2<R> *L0, C2<R> *L1, C2<R> *L2, C2<R> *L
     l = mk<C2<R>>(l.x + b.x, l.y + b.y);
     l = mk<C2<R>>(l.x + c.x, l.y + c.y);
     L3[idx] = l;
+}
+
+template <typename R>
+__global__ void k_prod_acc(R *__restrict__ o, const R *__restrict__ u, const R *__restrict__ gg, long n, int first)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (first ? (R)0 : o[i]) + u[i] * gg[i];
 }
 
 template <typename A, typename B>
@@ -599,6 +334,12 @@ void Rhs<R>::eval_general(C2<R> *F[4])
         for (auto &d : dF_) d = ralloc<C2<R>>(nc);
     }
     poisson_step(F[3]);
+    Capture *cap = std::is_same<R, double>::value ? cap_ : nullptr;
+    const long nr = g_.nr;
+    if (cap) {
+        KF_HIP(hipMemcpyAsync(cap->L, L_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        KF_HIP(hipMemcpyAsync(cap->phi, phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+    }
     XStore<R> x = xs<R>(XK_F1, g_);
     x.o0 = F1_;
     x.a0 = F[3];
@@ -618,9 +359,21 @@ void Rhs<R>::eval_general(C2<R> *F[4])
         ZStore<R> z = zs<R>(ZK_WRITE, plan_);
         z.o0 = U_[b];
         c2r_spec<R>(g_, plan_, F[b], 1, -1, z, s_);
+        if (cap) KF_HIP(hipMemcpyAsync(cap->U[b], U_[b], sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
     }
+    if (cap)
+        for (int a = 0; a < 4; ++a)
+            KF_HIP(hipMemcpyAsync(cap->dF0[a], dF_[3 * a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 3; ++b) {
+            if (cap) {
+                ZStore<R> z = zs<R>(ZK_WRITE, plan_);
+                z.o0 = (R *)cap->G[3 * a + b];
+                c2r_spec<R>(g_, plan_, dF_[3 * a + b], 1, -1, z, s_);
+                hipLaunchKernelGGL(k_prod_acc<R>, dim3(nbk(nr)), dim3(256), 0, s_, adv_[a], (const R *)U_[b],
+                                   (const R *)cap->G[3 * a + b], nr, b == 0 ? 1 : 0);
+                continue;
+            }
             ZStore<R> z = zs<R>(ZK_PROD, plan_);
             z.o0 = adv_[a];
             z.a0 = U_[b];
@@ -634,7 +387,7 @@ template <typename R>
 void Rhs<R>::eval(double t, C2<R> *F[4])
 {
     (void)t;
-    if (g_.any_stretched() || force_general_) eval_general(F);
+    if (g_.any_stretched() || force_general_ || (cap_ && std::is_same<R, double>::value)) eval_general(F);
     else eval_fused(F);
     KF_HIP(hipGetLastError());
 }

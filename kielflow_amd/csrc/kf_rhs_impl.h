@@ -1,0 +1,333 @@
+// kf_rhs_impl.h -- load/store functors fused into the FFT passes and the
+// transform helpers built from them (shared by the RHS and its adjoint).
+#pragma once
+
+#include "kf_fft.h"
+#include "kf_grid.h"
+
+namespace kf {
+
+using namespace fft;
+
+enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN };
+enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN, XK_ADJ };
+
+// ------------------------------------------------------------------ spectral factors
+template <typename R>
+__device__ __forceinline__ bool keep23(const GridView<R> &g, int i, int j, int k)
+{
+    return d23_keep(g.q[0][i], g.q[1][j], g.q[2][k], g.d23_thr);
+}
+
+// spectral viscosity 0.5 (1 - exp(-8 sqrt(knorm)^8)) (rhs_standard.cpp:68-71)
+template <typename R>
+__device__ __forceinline__ R sv_factor(const GridView<R> &g, int i, int j, int k)
+{
+    const R s = (R)((g.q[0][i] + g.q[1][j]) + g.q[2][k]);
+    return (R)0.5 * ((R)1 - exp(-(R)8 * (s * s)));
+}
+
+// exp(-2000 kabs^15) with the z norm of operations.hpp:23 (rhs_standard.cpp:220)
+template <typename R>
+__device__ __forceinline__ R phi_filter(const GridView<R> &g, int i, int j, int k)
+{
+    const R kab = sqrt((R)((g.q[0][i] + g.q[1][j]) + g.qzf[k]));
+    return exp((R)-2000 * pow(kab, (R)15));
+}
+
+template <typename R>
+__device__ __forceinline__ C2<R> ik(R kv, C2<R> v)
+{
+    return mk<C2<R>>(-kv * v.y, kv * v.x);
+}
+
+template <typename R>
+__device__ __forceinline__ void decode_x(const GridView<R> &g, long q, int t, int &i, int &j, int &k,
+                                         long &idx)
+{
+    const unsigned u = (unsigned)q, z = (unsigned)g.nzc;
+    const unsigned r = u / z;
+    i = t;
+    j = (int)r;
+    k = (int)(u - r * z);
+    idx = (long)t * g.ny * g.nzc + q;
+}
+
+__device__ __forceinline__ void dec3c(long p, int ny, int nz, int &i, int &j, int &k)
+{
+    const unsigned u = (unsigned)p, z = (unsigned)nz, y = (unsigned)ny;
+    const unsigned r = u / z;
+    k = (int)(u - r * z);
+    const unsigned q = r / y;
+    j = (int)(r - q * y);
+    i = (int)q;
+}
+
+// ------------------------------------------------------------------ functors
+// first pass of an inverse transform: v = [D23] [i k_dir] src
+// half-spectrum weight of the transposed transforms: interior kz planes
+// count twice in a C2R (W = 1/2 undoes that, W' = 2 applies it)
+template <typename R>
+__device__ __forceinline__ R wplane(const GridView<R> &g, int k)
+{
+    return (k == 0 || 2 * k == g.nz) ? (R)1 : (R)0.5;
+}
+
+template <typename R>
+struct XLoadSpec {
+    const C2<R> *src;
+    GridView<R> g;
+    int d23, dir;
+    int w = 0;     // multiply by W (adjoint of R2C)
+    int filt = 0;  // multiply by the Phi filter
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const
+    {
+        int i, j, k;
+        long idx;
+        decode_x(g, q, t, i, j, k, idx);
+        C2<R> v = src[idx];
+        if (d23 && !keep23(g, i, j, k)) return mk<C2<R>>((R)0, (R)0);
+        if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+        if (w) v = cscale(v, wplane(g, k));
+        if (filt) v = cscale(v, phi_filter(g, i, j, k));
+        return v;
+    }
+};
+
+// real rows of a product a0*a1 (forward transform of a product)
+template <typename R>
+struct RowLoadProd {
+    const R *a0, *a1;
+    int nz;
+    __device__ __forceinline__ C2<R> operator()(long row, int n) const
+    {
+        const C2<R> u = reinterpret_cast<const C2<R> *>(a0 + row * nz)[n];
+        const C2<R> v = reinterpret_cast<const C2<R> *>(a1 + row * nz)[n];
+        return mk<C2<R>>(u.x * v.x, u.y * v.y);
+    }
+};
+
+// last pass of an inverse transform (real rows, pairs 2n, 2n+1)
+template <typename R>
+struct ZStore {
+    int kind, nz, ny;
+    R scale;
+    R *o0, *o1;
+    const R *a0, *a1, *a2, *a3;
+    R c0, c1;
+    int flag;
+    const R *prof;
+    int pdir;
+    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const
+    {
+        using C = C2<R>;
+        const R x0 = v.x * scale, x1 = v.y * scale;
+        const long base = row * nz + 2 * n;
+        switch (kind) {
+        case ZK_WRITE:
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
+            break;
+        case ZK_PROF: {
+            R p0, p1;
+            if (pdir == 2) {
+                p0 = prof[2 * n];
+                p1 = prof[2 * n + 1];
+            } else {
+                const unsigned rr = (unsigned)row, yy = (unsigned)ny, qq = rr / yy;
+                const int id = (pdir == 1) ? (int)(rr - qq * yy) : (int)qq;
+                p0 = p1 = prof[id];
+            }
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0 * p0, x1 * p1);
+            break;
+        }
+        case ZK_RHO: {
+            const C d = *reinterpret_cast<const C *>(a0 + base);
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
+            *reinterpret_cast<C *>(o1 + base) = mk<C>(exp(x0) + d.x, exp(x1) + d.y);
+            break;
+        }
+        case ZK_PROD: {
+            const C u = *reinterpret_cast<const C *>(a0 + base);
+            C prev = flag ? *reinterpret_cast<const C *>(o0 + base) : mk<C>((R)0, (R)0);
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(prev.x + u.x * x0, prev.y + u.y * x1);
+            break;
+        }
+        case ZK_PEN: {
+            // rhs_standard.cpp:483-490: eps1 H (f + c H) + (1-H) B + b f
+            const C f = *reinterpret_cast<const C *>(a0 + base);
+            const C ad = *reinterpret_cast<const C *>(a1 + base);
+            const C H = *reinterpret_cast<const C *>(a2 + base);
+            const R b = a3[(unsigned)row / (unsigned)ny];
+            const R t0 = flag ? c1 * H.x : c1, t1 = flag ? c1 * H.y : c1;
+            R p0 = c0 * H.x * (f.x + t0);
+            R p1 = c0 * H.y * (f.y + t1);
+            p0 += ((R)1 - H.x) * (x0 + ad.x);
+            p1 += ((R)1 - H.y) * (x1 + ad.y);
+            p0 += b * f.x;
+            p1 += b * f.y;
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(p0, p1);
+            break;
+        }
+        }
+    }
+};
+
+// last pass of a forward transform
+template <typename R>
+struct XStore {
+    int kind;
+    GridView<R> g;
+    C2<R> *o0, *o1, *o2, *o3;
+    const C2<R> *a0, *a1, *a2, *a3;
+    R tau, M, theta;
+    int sv_on;
+    // XK_ADJ: o0 (+)= coef * [D23] * [-i k_dir] * W'/N * X   (adjoint of C2R)
+    int d23, dir, acc;
+    R coef, invN;
+    __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const
+    {
+        using C = C2<R>;
+        int i, j, k;
+        long idx;
+        decode_x(g, q, t, i, j, k, idx);
+        switch (kind) {
+        case XK_ADJ: {
+            C v = mk<C>((R)0, (R)0);
+            if (!d23 || keep23(g, i, j, k)) {
+                const R f = coef * invN / wplane(g, k);
+                v = cscale(X, f);
+                if (dir >= 0) v = ik(-g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+            }
+            if (acc) {
+                C o = o0[idx];
+                v = mk<C>(o.x + v.x, o.y + v.y);
+            }
+            o0[idx] = v;
+            break;
+        }
+        case XK_WRITE:
+            o0[idx] = X;
+            break;
+        case XK_ACC: {
+            C o = o0[idx];
+            o0[idx] = mk<C>(o.x + X.x, o.y + X.y);
+            break;
+        }
+        case XK_NEG_D23:
+            o0[idx] = keep23(g, i, j, k) ? mk<C>(-X.x, -X.y) : mk<C>((R)0, (R)0);
+            break;
+        case XK_NEG_ACC: {
+            C l = a0[idx];
+            o0[idx] = mk<C>(-(l.x + X.x), -(l.y + X.y));
+            break;
+        }
+        case XK_F1: {
+            const R f = phi_filter(g, i, j, k);
+            C n = a0[idx];
+            o0[idx] = mk<C>(f * X.x + n.x / theta, f * X.y + n.y / theta);
+            break;
+        }
+        case XK_LIN: {
+            // rhs_standard.cpp:119-140, 219-247, 279-319 on an equidistant grid
+            const R f = phi_filter(g, i, j, k);
+            const C fn = a3[idx];
+            const C F1 = mk<C>(f * X.x + fn.x / theta, f * X.y + fn.y / theta);
+            const bool kp = keep23(g, i, j, k);
+            const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
+            const R kx = g.k[0][i], ky = g.k[1][j], kz = g.k[2][k];
+            const C *FU[3] = {a0, a1, a2};
+            C *LO[3] = {o0, o1, o2};
+            const R ka[3] = {kx, ky, kz};
+            C divu = mk<C>((R)0, (R)0);
+            C dnx;
+            {
+                const C fd = kp ? fn : mk<C>((R)0, (R)0);
+                dnx = ik(kx, fd);
+            }
+            C ln = sv_on ? mk<C>(sv * fn.x, sv * fn.y) : mk<C>((R)0, (R)0);
+            ln = mk<C>(ln.x + M * dnx.x, ln.y + M * dnx.y);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const C u = FU[a][idx];
+                const R fac = sv_on ? (tau + sv) : tau;
+                C l = mk<C>(fac * u.x, fac * u.y);
+                const C dF1 = ik(ka[a], F1);
+                l = mk<C>(l.x + dF1.x, l.y + dF1.y);
+                const C ud = kp ? u : mk<C>((R)0, (R)0);
+                const C dx = ik(kx, ud);
+                l = mk<C>(l.x + M * dx.x, l.y + M * dx.y);
+                LO[a][idx] = l;
+                const C da = ik(ka[a], ud);
+                divu = (a == 0) ? da : mk<C>(divu.x + da.x, divu.y + da.y);
+                if (a == 0) ln = mk<C>(ln.x + da.x, ln.y + da.y);
+                else ln = mk<C>(ln.x + da.x, ln.y + da.y);
+            }
+            (void)divu;
+            o3[idx] = ln;
+            break;
+        }
+        }
+    }
+};
+
+template <typename R, typename RS>
+struct RowLoadConv {
+    const RS *p;
+    int nz;
+    __device__ __forceinline__ C2<R> operator()(long row, int n) const
+    {
+        const C2<RS> v = reinterpret_cast<const C2<RS> *>(p + row * nz)[n];
+        return mk<C2<R>>((R)v.x, (R)v.y);
+    }
+};
+
+// ------------------------------------------------------------------ transforms with functors
+template <typename R>
+inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, const ZStore<R> &st,
+                     hipStream_t s)
+{
+    GridView<R> gv = g.view<R>();
+    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
+                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X);
+    launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
+    launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc}, st, s);
+}
+
+template <typename R, typename RS>
+inline void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, hipStream_t s)
+{
+    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    launch_zr2c<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<R, RS>{src, p.nz},
+                   RowStoreC<R>{p.scratch, p.nzc}, s);
+    launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
+    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X);
+}
+
+template <typename R>
+inline ZStore<R> zs(int kind, const Plan<R> &p)
+{
+    ZStore<R> z{};
+    z.kind = kind;
+    z.nz = p.nz;
+    z.ny = p.ny;
+    z.scale = (R)(1.0 / (double)p.nr);
+    return z;
+}
+
+template <typename R>
+inline XStore<R> xs(int kind, Grid &g)
+{
+    XStore<R> x{};
+    x.kind = kind;
+    x.g = g.view<R>();
+    x.dir = -1;
+    x.coef = (R)1;
+    x.invN = (R)(1.0 / (double)g.nr);
+    return x;
+}
+
+
+}  // namespace kf
