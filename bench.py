@@ -36,7 +36,27 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
     p.add_argument("--cpu-n", type=int, default=128, help="box size of the CPU sample")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
+    p.add_argument("--adjoint-steps", type=int, default=200)
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run the plain device copy (1 GiB, 5 times) that the HBM byte counters are "
+                        "calibrated against, and exit")
     return p.parse_args()
+
+
+def copy_calibration():
+    """Five launches of the library's streaming copy kernel over 1 GiB: each
+    reads and writes exactly 2^30 bytes, so FETCH_SIZE / WRITE_SIZE per
+    dispatch of this kernel give the counters' scale."""
+    import kielflow_amd as kf
+    ctx = kf.Context(0)
+    nbytes = 1 << 30
+    a = ctx.zeros((nbytes // 8,), np.float64)
+    b = ctx.zeros((nbytes // 8,), np.float64)
+    for _ in range(5):
+        b.copy_from(a)
+    ctx.sync()
+    print(json.dumps({"copy_calibration_bytes_per_launch": nbytes, "launches": 5}))
 
 
 def run_precision(kf, ctx, g, axes, prec, nd_d, mask_d, Fn0, steps, warmup, prof_family=None):
@@ -108,6 +128,60 @@ def cpu_baseline(n_sample, n_target):
             "host_threads_available": os.cpu_count()}
 
 
+def adjoint_cost(kf, ctx, n=64, steps=200, k=20, m=3.0):
+    """Forward run vs checkpointed gradient run on the 64^3 hyperbolic-sinh
+    stretched box (m = 3): 200 FP64 RK4 steps, dt = 1e-3, M = 1, Q = -0.1,
+    checkpoints every 20 steps, J = grain x-drag at the final step."""
+    axes = [kf.axis(kf.HYST, n, -0.5 * L_BOX, L_BOX, m) for _ in range(3)]
+    x0 = kf.api.axis_values(axes[0], n // 2)[0]
+    x1 = kf.api.axis_values(axes[0], n // 2 + 1)[0]
+    Q = -0.1
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=Q, a=2.0 * (x1 - x0))
+    g = kf.Grid(ctx, axes)
+    ps = kf.Poisson(g, kf.KF_FP64, tol=1e-10, max_it=60)
+    rhs = kf.Rhs(g, ps, M=1.0, tau=0.1, theta=100.0, sv_on=False, pen_on=True)
+    phi = g.rfield()
+    rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+    fft = kf.FFT(g, kf.KF_FP64)
+    Fn0 = g.cfield()
+    fft.r2c(ctx.to_device(ln_n), Fn0)
+    dV = ctx.to_device(kf.api.cell_volumes(axes))
+    dG = ctx.to_device(nd / Q)
+    adj = kf.Adjoint(rhs, 1e-3)
+
+    def fresh():
+        phi.upload(np.zeros(g.rshape))
+        y = [g.cfield() for _ in range(4)]
+        y[3].copy_from(Fn0)
+        return y
+
+    rk = kf.Rk4(rhs, 1e-3)
+    rk.step(fresh(), 2)  # warm-up
+    y = fresh()
+    ctx.sync()
+    t0 = time.perf_counter()
+    rk.step(y, steps)
+    ctx.sync()
+    t_fwd = time.perf_counter() - t0
+    J_fwd = adj.objective(y, dV)
+    y = fresh()
+    dF = g.cfield()
+    ctx.sync()
+    t0 = time.perf_counter()
+    J, dM, dQ = adj.run(y, steps, k, dV, dG, dF)
+    ctx.sync()
+    t_adj = time.perf_counter() - t0
+    kept, lin_it = adj.run_info()
+    for o in (adj, rk, rhs, ps, fft):
+        o.close()
+    return {"workload": f"{n}^3 HySt (m={m}) FP64, {steps} RK4 steps, checkpoints every {k}",
+            "forward_s": round(t_fwd, 3), "gradient_run_s": round(t_adj, 3),
+            "gradient_over_forward": round(t_adj / t_fwd, 3),
+            "reverse_sweep_over_forward": round((t_adj - t_fwd) / t_fwd, 3),
+            "J": J, "J_forward_only": J_fwd, "dJ_dM": dM, "dJ_dQ": dQ,
+            "segments_kept_in_hbm": kept, "transposed_poisson_iterations": lin_it}
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
@@ -119,6 +193,9 @@ def main():
               "would shard one per GPU with no exchange; a single box is not split across GPUs).",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import kielflow_amd as kf
     n = args.n
     axes = kf.cube(n, L_BOX)
@@ -162,9 +239,11 @@ def main():
                       "rel_l2_mixed_vs_fp64_after_steps": errs,
                       "poisson_failures_fp64": fp64["poisson_failures"]})
 
+    if not args.no_adjoint:
+        extra["adjoint"] = adjoint_cost(kf, ctx, steps=args.adjoint_steps)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, n)
     line = {
-        "metric": "RK4 steps/s at 256^3 (mixed precision; FP64 rate and ratio alongside)",
+        "metric": "RK4 steps/s at 256^3 (FP64 vs mixed, rel-L2<=1e-5); adjoint/forward cost",
         "value": round(mixed["steps_per_s"], 3),
         "unit": "steps/s",
         "n_gpus": 1,

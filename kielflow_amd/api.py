@@ -99,6 +99,9 @@ SIGNATURES = {
     "kf_objective_adjoint": (C.c_int, [_vp, C.POINTER(_vp), _vp, C.POINTER(_vp)]),
     "kf_adjoint_accumulators": (C.c_int, [_vp, _dp, _vp]),
     "kf_adjoint_run": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp, _dp, _dp, _dp, _vp]),
+    "kf_adjoint_spectral_to_real": (C.c_int, [_vp, _vp, _vp]),
+    "kf_adjoint_set_capture_budget": (C.c_int, [_vp, C.c_size_t]),
+    "kf_adjoint_run_info": (C.c_int, [_vp, _ip, C.POINTER(C.c_long)]),
     "kf_profile_enable": (C.c_int, [C.c_int]),
     "kf_profile_read": (C.c_int, [_ip, _dp]),
     "kf_setup_host": (C.c_int, [_AX3, C.c_ulonglong, C.c_double, C.c_double, C.c_double, _vp, _vp,
@@ -131,6 +134,19 @@ def check(code: int):
 
 def axis(type_, n, l0, L, m=0.0) -> Axis:
     return Axis(type_, n, l0, L, m)
+
+
+def axis_values(ax: Axis, i: int):
+    """(x, k, S') of point i of an axis (axis::val_at, k_val_at, dS)."""
+    v, k, d = C.c_double(), C.c_double(), C.c_double()
+    check(load().kf_axis_values(C.byref(ax), i, C.byref(v), C.byref(k), C.byref(d)))
+    return v.value, k.value, d.value
+
+
+def cell_volumes(axes):
+    """dV = prod_b S'_b(xi) L_b / N_b, the quadrature weights of a box."""
+    w = [np.array([axis_values(a, i)[2] * a.L / a.N for i in range(a.N)]) for a in axes]
+    return w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
 
 
 def cube(n, L=16.0, type_=EQST, m=0.0):
@@ -426,6 +442,19 @@ class Adjoint:
         check(self.lib.kf_adjoint_run(self.h, _ptrs4(y), nsteps, k, dV.ptr, G.ptr, C.byref(J), C.byref(dM),
                                       C.byref(dQ), dFn0.ptr))
         return J.value, dM.value, dQ.value
+
+    def set_capture_budget(self, nbytes):
+        check(self.lib.kf_adjoint_set_capture_budget(self.h, int(nbytes)))
+
+    def run_info(self):
+        """(segments kept in HBM?, transposed Poisson iterations) of the last run."""
+        k, it = C.c_int(0), C.c_long(0)
+        check(self.lib.kf_adjoint_run_info(self.h, C.byref(k), C.byref(it)))
+        return bool(k.value), it.value
+
+    def spectral_to_real(self, dF, out):
+        """out = R2C^T dF: real-space cotangent of the field whose spectrum is F."""
+        check(self.lib.kf_adjoint_spectral_to_real(self.h, dF.ptr, out.ptr))
 
 
 PROF_FFT_X, PROF_FFT_Y, PROF_FFT_ZR2C, PROF_FFT_ZC2R, PROF_POISSON = 0, 1, 2, 3, 4

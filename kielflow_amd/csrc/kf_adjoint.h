@@ -10,6 +10,8 @@
 // mirrored by one of the reverse sweep.
 #pragma once
 
+#include <vector>
+
 #include "kf_rhs.h"
 
 namespace kf {
@@ -40,13 +42,25 @@ class Adjoint {
     double run(double2 *y[4], int nsteps, int k, const double *W, const double *G, double *dM, double *dQ,
                double2 *dFn0);
 
+    // out = R2C^T Y (the real-space cotangent of a field whose spectrum is Y)
+    void spectral_to_real_T(const double2 *Y, double *out);
+
     void reset_accumulators();
     double M_bar();              // synchronises
     double *nd_bar() { return ndb_; }
     void set_tolerance(double t) { ltol_ = t; }
 
+    // capture budget (bytes) of the segment intermediates kept in HBM
+    void set_capture_budget(size_t b) { capture_budget_ = b; }
+    // whether the last run kept its segments' intermediates (no per-step
+    // recomputation) and the transposed Poisson iterations it took
+    bool last_run_kept_segments() const { return used_pool_; }
+    long last_run_linear_iterations() const { return lin_its_; }
+
   private:
     void deriv_T(const double2 *Y, int b, double2 *out, int acc, double coef, int d23_out);
+    void reverse_step(const Capture caps[4], double2 *yb[4]);
+    bool ensure_pool(int steps);
     Rhs<double> &rhs_;
     Grid &g_;
     fft::Plan<double> &p_;
@@ -54,7 +68,13 @@ class Adjoint {
     double dt_;
     double ltol_ = 1e-12;
     Capture cap_[4];
-    double2 *stage_in_[4][4] = {};   // stage inputs of the recomputed step
+    Rk4<double> *rk_ = nullptr;      // forward steps of runs and recomputations
+    std::vector<Capture> pool_;      // per-stage intermediates of a segment
+    size_t capture_budget_ = (size_t)64 << 30;
+    bool used_pool_ = false;
+    double *lam_[4] = {};            // transposed Poisson solutions per stage (warm start)
+    bool lam_valid_ = false;
+    long lin_its_ = 0;
     double2 *Fb_[4] = {}, *linb_[4] = {}, *F1b_ = nullptr, *Yt_ = nullptr, *kb_[4][4] = {};
     double *advb_[4] = {}, *Ub_[3] = {}, *Lb_ = nullptr, *rt_ = nullptr, *Pb_ = nullptr;
     double *phib_ = nullptr, *rhob_ = nullptr, *ndb_ = nullptr, *phiJ_ = nullptr, *E_ = nullptr;

@@ -2,6 +2,7 @@
 // of the RK4 step (RKO4.cpp:50-248) and of the grain drag objective.
 #include "kf_adjoint.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -213,6 +214,15 @@ static T *dalloc(long n)
     return p;
 }
 
+static void free_capture(Capture &c)
+{
+    for (auto &u : c.U) (void)hipFree(u);
+    (void)hipFree(c.L);
+    (void)hipFree(c.phi);
+    for (auto &gg : c.G) (void)hipFree(gg);
+    for (auto &d : c.dF0) (void)hipFree(d);
+}
+
 Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rhs.plan()), s_(rhs.stream()), dt_(dt)
 {
     const long nr = g_.nr, nc = g_.nc;
@@ -223,10 +233,10 @@ Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rh
         for (auto &gg : c.G) gg = dalloc<double>(nr);
         for (auto &d : c.dF0) d = dalloc<double2>(nc);
     }
-    for (auto &st : stage_in_)
-        for (auto &f : st) f = dalloc<double2>(nc);
     for (auto &st : kb_)
         for (auto &f : st) f = dalloc<double2>(nc);
+    for (auto &l : lam_) l = dalloc<double>(nr);
+    rk_ = new Rk4<double>(rhs_, dt_);
     for (int a = 0; a < 4; ++a) {
         Fb_[a] = dalloc<double2>(nc);
         linb_[a] = dalloc<double2>(nc);
@@ -258,10 +268,11 @@ Adjoint::~Adjoint()
         for (auto &gg : c.G) (void)hipFree(gg);
         for (auto &d : c.dF0) (void)hipFree(d);
     }
-    for (auto &st : stage_in_)
-        for (auto &f : st) (void)hipFree(f);
     for (auto &st : kb_)
         for (auto &f : st) (void)hipFree(f);
+    for (auto &l : lam_) (void)hipFree(l);
+    for (auto &c : pool_) free_capture(c);
+    delete rk_;
     for (int a = 0; a < 4; ++a) {
         (void)hipFree(Fb_[a]);
         (void)hipFree(linb_[a]);
@@ -275,6 +286,7 @@ Adjoint::~Adjoint()
 
 void Adjoint::reset_accumulators()
 {
+    lam_valid_ = false;
     KF_HIP(hipMemsetAsync(Mb_, 0, sizeof(double) * 4, s_));
     KF_HIP(hipMemsetAsync(ndb_, 0, sizeof(double) * g_.nr, s_));
 }
@@ -364,7 +376,7 @@ static void r2c_adj(Grid &g, Plan<double> &p, const double *a0, const double *a1
 static void rhs_adjoint_core(Adjoint *A, Rhs<double> &rhs, Grid &g, Plan<double> &p, hipStream_t s, const Capture &c,
                              const double2 *ob[4], double2 *Fb[4], double2 **linb, double **advb, double **Ub,
                              double *Lb, double2 *F1b, double2 *Yt, double *Pb, double *phib, double *rhob,
-                             double *ndb, double *Mb, double *part, double ltol,
+                             double *ndb, double *Mb, double *part, double ltol, bool guess,
                              void (Adjoint::*derivT)(const double2 *, int, double2 *, int, double, int))
 {
     const long nr = g.nr, nc = g.nc;
@@ -430,7 +442,7 @@ static void rhs_adjoint_core(Adjoint *A, Rhs<double> &rhs, Grid &g, Plan<double>
     // Poisson: rho_bar = -J^{-T} phi_bar
     Poisson &ps = rhs.poisson();
     ps.set_linearization(c.phi, s);
-    int it = ps.linear_solve_T(rhob, phib, ltol, 200, s);
+    int it = ps.linear_solve_T(rhob, phib, ltol, 200, s, guess);
     if (it < 0) throw Error("adjoint Poisson solve did not converge");
     hipLaunchKernelGGL(k_L_adj, dim3(nbk(nr)), dim3(256), 0, s, Lb, ndb, (const double *)rhob,
                        (const double *)c.L, nr);
@@ -449,39 +461,39 @@ void Adjoint::rhs_adjoint(const double2 *S[4], const double2 *ob[4], double2 *Fb
         rhs_.set_capture(nullptr);
     }
     rhs_adjoint_core(this, rhs_, g_, p_, s_, cap_[0], ob, Fb, linb_, advb_, Ub_, Lb_, F1b_, Yt_, Pb_, phib_, rhob_,
-                     ndb_, Mb_, part_, ltol_, &Adjoint::deriv_T);
+                     ndb_, Mb_, part_, ltol_, false, &Adjoint::deriv_T);
 }
 
 void Adjoint::step_adjoint(const double2 *y0[4], double2 *yb[4])
 {
+    // recompute the step from y0 through the forward RK4, keeping the
+    // intermediates of its four stages
+    for (int a = 0; a < 4; ++a)
+        KF_HIP(hipMemcpyAsync(yscratch_[a], y0[a], sizeof(double2) * g_.nc, hipMemcpyDeviceToDevice, s_));
+    rhs_.reset_history();
+    rhs_.set_capture_stages(cap_);
+    rk_->step(yscratch_);
+    rhs_.set_capture_stages(nullptr);
+    reverse_step(cap_, yb);
+}
+
+// y1 = y0 + dt/6 (k1 + 2 k2 + 2 k3 + k4), k_s = f(S_s), S_1 = y0,
+// S_{s+1} = y0 + c_s k_s (c = dt/2, dt/2, dt) run backwards: ybar_0 = ybar_1 +
+// sum_s Sbar_s, kbar_s = w_s ybar_1 + c_s Sbar_{s+1}, Sbar_s = f'(S_s)^T kbar_s
+void Adjoint::reverse_step(const Capture caps[4], double2 *yb[4])
+{
     const long nc = g_.nc;
     const unsigned nb = nbk(nc);
-    // recompute the step, keeping stage inputs and intermediates
-    rhs_.set_stage(-1);
-    for (int a = 0; a < 4; ++a)
-        KF_HIP(hipMemcpyAsync(stage_in_[0][a], y0[a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
-    const double c[3] = {0.5 * dt_, 0.5 * dt_, 1.0 * dt_};
-    for (int st = 0; st < 4; ++st) {
-        for (int a = 0; a < 4; ++a)
-            KF_HIP(hipMemcpyAsync(yscratch_[a], stage_in_[st][a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
-        rhs_.set_capture(&cap_[st]);
-        rhs_.eval(0., yscratch_);
-        rhs_.set_capture(nullptr);
-        if (st < 3)
-            for (int a = 0; a < 4; ++a)
-                hipLaunchKernelGGL(k_lincomb2, dim3(nb), dim3(256), 0, s_, stage_in_[st + 1][a], y0[a],
-                                   (const double2 *)yscratch_[a], c[st], nc);
-    }
-    // reverse: y1 = y0 + dt/6 (k1 + 2 k2 + 2 k3 + k4)
     const double wk[4] = {dt_ / 6., dt_ / 3., dt_ / 3., dt_ / 6.};
     for (int st = 0; st < 4; ++st)
         for (int a = 0; a < 4; ++a)
             hipLaunchKernelGGL(k_scale2, dim3(nb), dim3(256), 0, s_, kb_[st][a], (const double2 *)yb[a], wk[st], nc);
-    const double back[4] = {0., 0.5 * dt_, 0.5 * dt_, 1.0 * dt_};  // s_st = y0 + back[st] k_{st-1}
+    const double back[4] = {0., 0.5 * dt_, 0.5 * dt_, 1.0 * dt_};  // S_st = y0 + back[st] k_{st-1}
     for (int st = 3; st >= 0; --st) {
         const double2 *ob[4] = {kb_[st][0], kb_[st][1], kb_[st][2], kb_[st][3]};
-        rhs_adjoint_core(this, rhs_, g_, p_, s_, cap_[st], ob, Fb_, linb_, advb_, Ub_, Lb_, F1b_, Yt_, Pb_, phib_,
-                         rhob_, ndb_, Mb_, part_, ltol_, &Adjoint::deriv_T);
+        rhs_adjoint_core(this, rhs_, g_, p_, s_, caps[st], ob, Fb_, linb_, advb_, Ub_, Lb_, F1b_, Yt_, Pb_, phib_,
+                         lam_[st], ndb_, Mb_, part_, ltol_, lam_valid_, &Adjoint::deriv_T);
+        lin_its_ += rhs_.poisson().last_linear_iterations();
         for (int a = 0; a < 4; ++a) {
             hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(256), 0, s_, yb[a], (const double2 *)Fb_[a], 1.0, nc);
             if (st > 0)
@@ -489,6 +501,7 @@ void Adjoint::step_adjoint(const double2 *y0[4], double2 *yb[4])
                                    back[st], nc);
         }
     }
+    lam_valid_ = true;
     KF_HIP(hipGetLastError());
 }
 
@@ -561,62 +574,124 @@ void Adjoint::objective_adjoint(const double2 *F[4], const double *dV, double2 *
     r2c_adj(g_, p_, E_, nullptr, Fb[3], 1, -1, 0, 1.0, s_);
 }
 
+void Adjoint::spectral_to_real_T(const double2 *Y, double *out) { c2r_w(g_, p_, Y, 0, 0, 1.0, out, s_); }
+
+// intermediates of one RK4 step (four stages) in bytes
+static size_t step_capture_bytes(const Grid &g)
+{
+    return 4 * (17 * sizeof(double) * (size_t)g.nr + 4 * sizeof(double2) * (size_t)g.nc);
+}
+
+bool Adjoint::ensure_pool(int steps)
+{
+    if ((int)pool_.size() >= 4 * steps) return true;
+    size_t fr = 0, tot = 0;
+    KF_HIP(hipMemGetInfo(&fr, &tot));
+    const size_t have = pool_.size() / 4 * step_capture_bytes(g_);
+    const size_t need = (size_t)steps * step_capture_bytes(g_) - have;
+    if (need > fr / 2 || (size_t)steps * step_capture_bytes(g_) > capture_budget_) return false;
+    const long nr = g_.nr, nc = g_.nc;
+    while ((int)pool_.size() < 4 * steps) {
+        Capture c{};
+        for (auto &u : c.U) u = dalloc<double>(nr);
+        c.L = dalloc<double>(nr);
+        c.phi = dalloc<double>(nr);
+        for (auto &gg : c.G) gg = dalloc<double>(nr);
+        for (auto &d : c.dF0) d = dalloc<double2>(nc);
+        pool_.push_back(c);
+    }
+    return true;
+}
+
 double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const double *G, double *dM, double *dQ,
                     double2 *dFn0)
 {
     if (k < 1) k = 1;
     const long nc = g_.nc, nr = g_.nr;
     const int nck = (nsteps + k - 1) / k;
-    // checkpoints (one state per k steps), the states of one segment and ybar
-    struct Fields {
-        std::vector<double2 *> v;
-        ~Fields()
+    // checkpoints: state and converged potential every k steps
+    struct Owned {
+        std::vector<void *> v;
+        ~Owned()
         {
             for (auto p : v) (void)hipFree(p);
         }
-    } ck, seg, ybs;
-    ck.v.assign((size_t)nck * 4, nullptr);
-    seg.v.assign((size_t)k * 4, nullptr);
-    ybs.v.assign(4, nullptr);
-    for (auto &p : ck.v) p = dalloc<double2>(nc);
-    for (auto &p : seg.v) p = dalloc<double2>(nc);
-    for (auto &p : ybs.v) p = dalloc<double2>(nc);
-    double2 *yb[4] = {ybs.v[0], ybs.v[1], ybs.v[2], ybs.v[3]};
-    Rk4<double> rk(rhs_, dt_);
-    // forward with checkpoints
+    } own;
+    auto alloc = [&](size_t bytes) {
+        void *p = nullptr;
+        KF_HIP(hipMalloc(&p, bytes));
+        own.v.push_back(p);
+        return p;
+    };
+    std::vector<double2 *> ck((size_t)nck * 4);
+    std::vector<double *> ckphi(nck);
+    for (auto &p : ck) p = (double2 *)alloc(sizeof(double2) * nc);
+    for (auto &p : ckphi) p = (double *)alloc(sizeof(double) * nr);
+    double2 *yb[4];
+    for (auto &p : yb) p = (double2 *)alloc(sizeof(double2) * nc);
+    // a segment's stage intermediates are kept in HBM when they fit, so each
+    // segment is recomputed once; otherwise every step is recomputed from the
+    // segment's stored states
+    const bool keep = ensure_pool(std::min(k, nsteps));
+    std::vector<double2 *> seg;
+    if (!keep) {
+        seg.resize((size_t)k * 4);
+        for (auto &p : seg) p = (double2 *)alloc(sizeof(double2) * nc);
+    }
+    rk_->set_dt(dt_);
+    rhs_.reset_history();
     for (int n = 0; n < nsteps; ++n) {
-        if (n % k == 0)
+        if (n % k == 0) {
+            const int c = n / k;
             for (int a = 0; a < 4; ++a)
-                KF_HIP(hipMemcpyAsync(ck.v[(size_t)(n / k) * 4 + a], y[a], sizeof(double2) * nc, hipMemcpyDeviceToDevice,
-                                      s_));
-        rk.step(y);
+                KF_HIP(hipMemcpyAsync(ck[(size_t)c * 4 + a], y[a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
+            KF_HIP(hipMemcpyAsync(ckphi[c], rhs_.phi(), sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        }
+        rk_->step(y);
     }
     const double2 *yc[4] = {y[0], y[1], y[2], y[3]};
     const double J = objective(yc, dV);
     reset_accumulators();
+    lam_valid_ = false;
+    lin_its_ = 0;
     objective_adjoint(yc, dV, yb);
-    // reverse sweep over segments, recomputing the states of each segment
     for (int c = nck - 1; c >= 0; --c) {
         const int n0 = c * k, n1 = std::min(nsteps, n0 + k);
-        for (int a = 0; a < 4; ++a)
-            KF_HIP(hipMemcpyAsync(seg.v[a], ck.v[(size_t)c * 4 + a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
-        for (int n = n0 + 1; n < n1; ++n) {
-            double2 *cur[4];
-            for (int a = 0; a < 4; ++a) {
-                cur[a] = seg.v[(size_t)(n - n0) * 4 + a];
-                KF_HIP(hipMemcpyAsync(cur[a], seg.v[(size_t)(n - n0 - 1) * 4 + a], sizeof(double2) * nc,
+        KF_HIP(hipMemcpyAsync(rhs_.phi(), ckphi[c], sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        rhs_.reset_history();
+        if (keep) {
+            for (int a = 0; a < 4; ++a)
+                KF_HIP(hipMemcpyAsync(yscratch_[a], ck[(size_t)c * 4 + a], sizeof(double2) * nc,
                                       hipMemcpyDeviceToDevice, s_));
+            for (int n = n0; n < n1; ++n) {
+                rhs_.set_capture_stages(&pool_[(size_t)(n - n0) * 4]);
+                rk_->step(yscratch_);
             }
-            rk.step(cur);
-        }
-        for (int n = n1 - 1; n >= n0; --n) {
-            const double2 *st[4];
-            for (int a = 0; a < 4; ++a) st[a] = seg.v[(size_t)(n - n0) * 4 + a];
-            step_adjoint(st, yb);
+            rhs_.set_capture_stages(nullptr);
+            for (int n = n1 - 1; n >= n0; --n) reverse_step(&pool_[(size_t)(n - n0) * 4], yb);
+        } else {
+            for (int a = 0; a < 4; ++a)
+                KF_HIP(hipMemcpyAsync(seg[a], ck[(size_t)c * 4 + a], sizeof(double2) * nc, hipMemcpyDeviceToDevice,
+                                      s_));
+            for (int n = n0 + 1; n < n1; ++n) {
+                double2 *cur[4];
+                for (int a = 0; a < 4; ++a) {
+                    cur[a] = seg[(size_t)(n - n0) * 4 + a];
+                    KF_HIP(hipMemcpyAsync(cur[a], seg[(size_t)(n - n0 - 1) * 4 + a], sizeof(double2) * nc,
+                                          hipMemcpyDeviceToDevice, s_));
+                }
+                rk_->step(cur);
+            }
+            for (int n = n1 - 1; n >= n0; --n) {
+                const double2 *st[4];
+                for (int a = 0; a < 4; ++a) st[a] = seg[(size_t)(n - n0) * 4 + a];
+                step_adjoint(st, yb);
+            }
         }
     }
     if (dM) *dM = M_bar();
     if (dQ) {
+        // n_d = Q G: the dynamics and the weight of J both carry Q
         const unsigned nbr = nbk(nr) > AMAXB ? AMAXB : nbk(nr);
         hipLaunchKernelGGL(k_dotp, dim3(nbr), dim3(AB), 0, s_, (const double *)ndb_, G, nr, part_);
         hipLaunchKernelGGL(k_zero, dim3(1), dim3(1), 0, s_, Mb_ + 3, 1L);
@@ -625,6 +700,7 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
     }
     if (dFn0) KF_HIP(hipMemcpyAsync(dFn0, yb[3], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
     KF_HIP(hipStreamSynchronize(s_));
+    used_pool_ = keep;
     return J;
 }
 

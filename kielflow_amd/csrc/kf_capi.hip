@@ -165,11 +165,30 @@ int kf_memcpy_d2h(kf_ctx ctx, void *dst, const void *src, size_t bytes)
     });
 }
 
+// plain streaming copy, 16 bytes per lane and iteration (also the kernel the
+// HBM byte counters are calibrated against: it reads and writes `bytes` once)
+__global__ void __launch_bounds__(256) k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 int kf_memcpy_d2d(kf_ctx ctx, void *dst, const void *src, size_t bytes)
 {
     return guard([&] {
         KF_REQUIRE(ctx, "null context");
-        KF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        if (bytes == 0) return KF_OK;
+        const bool al = (((uintptr_t)dst | (uintptr_t)src | bytes) & 15) == 0;
+        if (!al) {
+            KF_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+            return KF_OK;
+        }
+        const size_t n = bytes / 16;
+        size_t nb = (n + 255) / 256;
+        if (nb > 8192) nb = 8192;
+        hipLaunchKernelGGL(k_copy16, dim3((unsigned)nb), dim3(256), 0, ctx->stream, (const uint4 *)src, (uint4 *)dst,
+                           n);
+        KF_HIP(hipGetLastError());
         return KF_OK;
     });
 }
@@ -733,6 +752,35 @@ int kf_adjoint_run(kf_adjoint a, double *const y[4], int nsteps, int k, const do
         double2 *yy[4];
         for (int i = 0; i < 4; ++i) yy[i] = (double2 *)y[i];
         *J = a->a->run(yy, nsteps, k, dV, G, dJdM, dJdQ, (double2 *)dJdFn0);
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_set_capture_budget(kf_adjoint a, size_t bytes)
+{
+    return guard([&] {
+        KF_REQUIRE(a, "null adjoint");
+        a->a->set_capture_budget(bytes);
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_run_info(kf_adjoint a, int *kept_segments, long *linear_iterations)
+{
+    return guard([&] {
+        KF_REQUIRE(a, "null adjoint");
+        if (kept_segments) *kept_segments = a->a->last_run_kept_segments() ? 1 : 0;
+        if (linear_iterations) *linear_iterations = a->a->last_run_linear_iterations();
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_spectral_to_real(kf_adjoint a, const double *dJdF, double *dJdf)
+{
+    return guard([&] {
+        KF_REQUIRE(a && dJdF && dJdf, "null argument");
+        a->a->spectral_to_real_T((const double2 *)dJdF, dJdf);
+        KF_HIP(hipStreamSynchronize(a->rhs->grid->ctx->stream));
         return KF_OK;
     });
 }

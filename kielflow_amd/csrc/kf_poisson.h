@@ -85,9 +85,12 @@ class Poisson {
     // J x = b to max|b - J x| <= rtol max|b| (FP64; x overwritten; defect
     // correction with one V-cycle, or two BiCGStab steps on stretched grids,
     // per iteration).  Returns iterations, or -1 when max_it is reached.
-    int linear_solve(double *x, const double *b, double rtol, int max_it, hipStream_t s);
+    // With guess, x holds the initial iterate on entry.
+    int linear_solve(double *x, const double *b, double rtol, int max_it, hipStream_t s, bool guess = false);
     // J^T x = b through  V J V^{-1} = J^T  (V = cell volumes, V L_h symmetric)
-    int linear_solve_T(double *x, const double *b, double rtol, int max_it, hipStream_t s);
+    int linear_solve_T(double *x, const double *b, double rtol, int max_it, hipStream_t s, bool guess = false);
+    // iterations of the last linear solve
+    int last_linear_iterations() const { return lin_it_; }
 
     // phi: FP64 in/out (warm start), rho: FP64 or FP32 real field.
     // Enqueues the solve on stream s.  When sync is true the call waits and
@@ -163,6 +166,8 @@ class Poisson {
     double *d_scal_ = nullptr;    // device scalars of BiCGStab
     hipEvent_t ev_[2] = {};       // convergence-flag copies of the look-ahead loop
     double *lsx_ = nullptr, *lsr_ = nullptr, *lsb_ = nullptr;  // linear solve work
+    double *lsT_ = nullptr;  // right-hand side of the transposed solve
+    int lin_it_ = 0;
     double *vol_[3] = {};         // per-axis cell volumes (H[i+1]+H[i])/2
     NljState nlj_;
     hipStream_t s0_;

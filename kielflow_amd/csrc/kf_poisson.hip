@@ -771,6 +771,7 @@ Poisson::~Poisson()
     (void)hipFree(lsx_);
     (void)hipFree(lsr_);
     (void)hipFree(lsb_);
+    (void)hipFree(lsT_);
     for (auto &v : vol_) (void)hipFree(v);
     (void)hipEventDestroy(ev_[0]);
     (void)hipEventDestroy(ev_[1]);
@@ -1244,7 +1245,7 @@ void Poisson::set_linearization(const double *phi, hipStream_t s)
     hipLaunchKernelGGL(k_exp_field, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, lv_[0].e64, N);
 }
 
-int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, hipStream_t s)
+int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, hipStream_t s, bool guess)
 {
     Level &L0 = lv_[0];
     const long N = L0.N;
@@ -1258,7 +1259,8 @@ int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, h
     }
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
     restrict_e<double>(s, d_ctl_);
-    hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, lsx_, N, 0., (const int *)nullptr);
+    if (guess) KF_HIP(hipMemcpyAsync(lsx_, x, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    else hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, lsx_, N, 0., (const int *)nullptr);
     LevelView<double> v = view<double>(0);
     std::vector<double> part(2 * (size_t)nbr);
     int it = 0;
@@ -1285,13 +1287,14 @@ int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, h
         hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, lsx_, (const double *)L0.x64, N);
     }
     KF_HIP(hipMemcpyAsync(x, lsx_, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    lin_it_ = it;
     return (h_stat_[0] <= rtol) ? it : -1;
 }
 
-int Poisson::linear_solve_T(double *x, const double *b, double rtol, int max_it, hipStream_t s)
+int Poisson::linear_solve_T(double *x, const double *b, double rtol, int max_it, hipStream_t s, bool guess)
 {
     const bool uniform = ax_desc_uniform();
-    if (uniform) return linear_solve(x, b, rtol, max_it, s);
+    if (uniform) return linear_solve(x, b, rtol, max_it, s, guess);
     Level &L0 = lv_[0];
     const long N = L0.N;
     if (!vol_[0]) {
@@ -1302,16 +1305,18 @@ int Poisson::linear_solve_T(double *x, const double *b, double rtol, int max_it,
             KF_HIP(hipMemcpy(vol_[d], v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice));
         }
     }
-    double *bs = nullptr;
-    KF_HIP(hipMalloc(&bs, sizeof(double) * N));
+    if (!lsT_) KF_HIP(hipMalloc(&lsT_, sizeof(double) * N));
+    double *bs = lsT_;
     hipLaunchKernelGGL(k_vscale, dim3(nblocks(N, 256)), dim3(256), 0, s, b, bs, (const double *)vol_[0],
                        (const double *)vol_[1], (const double *)vol_[2], L0.n[0], L0.n[1], L0.n[2], 0);
-    int it = linear_solve(x, bs, rtol, max_it, s);
+    if (guess)  // the iterate of J y = V^{-1} b is y = V^{-1} x
+        hipLaunchKernelGGL(k_vscale, dim3(nblocks(N, 256)), dim3(256), 0, s, (const double *)x, x,
+                           (const double *)vol_[0], (const double *)vol_[1], (const double *)vol_[2], L0.n[0],
+                           L0.n[1], L0.n[2], 0);
+    int it = linear_solve(x, bs, rtol, max_it, s, guess);
     hipLaunchKernelGGL(k_vscale, dim3(nblocks(N, 256)), dim3(256), 0, s, (const double *)x, x,
                        (const double *)vol_[0], (const double *)vol_[1], (const double *)vol_[2], L0.n[0],
                        L0.n[1], L0.n[2], 1);
-    KF_HIP(hipStreamSynchronize(s));
-    (void)hipFree(bs);
     return it;
 }
 

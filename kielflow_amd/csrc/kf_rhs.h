@@ -55,6 +55,13 @@ class Rhs {
     void set_stage(int s);
     // evaluations store their intermediates into *c (general path) while set
     void set_capture(Capture *c) { cap_ = c; }
+    // evaluations at RK stage s store their intermediates into c4[s] while set
+    void set_capture_stages(Capture *c4) { cap_stages_ = c4; }
+    // forget the per-stage potential history (the run jumped in time)
+    void reset_history()
+    {
+        for (int &h : hc_) h = 0;
+    }
     void set_params(const Params &p) { pa_ = p; }
     const Params &params() const { return pa_; }
     const Switches &switches() const { return sw_; }
@@ -96,6 +103,12 @@ class Rhs {
     C2<R> *dF_[12] = {};
     int stage_ = -1;
     Capture *cap_ = nullptr;
+    Capture *cap_stages_ = nullptr;
+    Capture *active_capture() const
+    {
+        if (cap_) return cap_;
+        return (cap_stages_ && stage_ >= 0) ? &cap_stages_[stage_] : nullptr;
+    }
     double *ph_[4][2] = {};
     int hc_[4] = {0, 0, 0, 0};
 };

@@ -334,7 +334,7 @@ void Rhs<R>::eval_general(C2<R> *F[4])
         for (auto &d : dF_) d = ralloc<C2<R>>(nc);
     }
     poisson_step(F[3]);
-    Capture *cap = std::is_same<R, double>::value ? cap_ : nullptr;
+    Capture *cap = std::is_same<R, double>::value ? active_capture() : nullptr;
     const long nr = g_.nr;
     if (cap) {
         KF_HIP(hipMemcpyAsync(cap->L, L_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
@@ -387,7 +387,8 @@ template <typename R>
 void Rhs<R>::eval(double t, C2<R> *F[4])
 {
     (void)t;
-    if (g_.any_stretched() || force_general_ || (cap_ && std::is_same<R, double>::value)) eval_general(F);
+    if (g_.any_stretched() || force_general_ || (active_capture() && std::is_same<R, double>::value))
+        eval_general(F);
     else eval_fused(F);
     KF_HIP(hipGetLastError());
 }

@@ -1376,3 +1376,320 @@ void or_init_density(const or_grid *g, uint64_t seed, double amp, double *Fn)
     or_fft_r2c(g, v, Fn);
     free(v);
 }
+
+/* ======================================================================== */
+/* Tangent-linear model of the RHS, the RK4 step and the grain drag         */
+/* objective (forward-mode derivative, FP64).  It linearises each line of  */
+/* or_rhs_eval above; it checks the GPU adjoint, which is its transpose.   */
+/* ======================================================================== */
+
+/* (e^phi - L_h) x = b: Jacobi-preconditioned CG on the system multiplied by
+ * the cell volumes V = prod_d (H[i+1] + H[i]) / 2, which makes it symmetric
+ * positive definite (V L_h is symmetric for the stencil above). */
+static int tl_poisson(const or_grid *g, const double *phi, const double *b, double *x, double tol)
+{
+    fas_level lv;
+    lv.n[0] = g->n[0]; lv.n[1] = g->n[1]; lv.n[2] = g->n[2];
+    spacing_make(&lv.sp, g->ax);
+    const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
+    const size_t N = g->nr;
+    double *V = (double *)malloc(sizeof(double) * N), *E = (double *)malloc(sizeof(double) * N);
+    double *dg = (double *)malloc(sizeof(double) * N), *r = (double *)malloc(sizeof(double) * N);
+    double *z = (double *)malloc(sizeof(double) * N), *p = (double *)malloc(sizeof(double) * N);
+    double *Ap = (double *)malloc(sizeof(double) * N);
+    double bmax = 0.;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const size_t q = PIDX(i, j, k);
+                double A, D;
+                stencil(&lv, phi, i, j, k, &A, &D);
+                V[q] = 0.125 * (lv.sp.H[0][i + 1] + lv.sp.H[0][i]) * (lv.sp.H[1][j + 1] + lv.sp.H[1][j]) *
+                       (lv.sp.H[2][k + 1] + lv.sp.H[2][k]);
+                E[q] = exp(phi[q]);
+                dg[q] = V[q] * (E[q] + D);
+                x[q] = 0.;
+                r[q] = V[q] * b[q];
+                bmax = omax(bmax, fabs(r[q]));
+            }
+    double rz = 0.;
+    for (size_t q = 0; q < N; ++q) {
+        z[q] = r[q] / dg[q];
+        p[q] = z[q];
+        rz += r[q] * z[q];
+    }
+    int it = 0, ok = (bmax == 0.);
+    for (; !ok && it < 20000; ++it) {
+        double pAp = 0.;
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    const size_t q = PIDX(i, j, k);
+                    double A, D;
+                    stencil(&lv, p, i, j, k, &A, &D);
+                    Ap[q] = V[q] * (E[q] * p[q] - (A - D * p[q]));
+                    pAp += p[q] * Ap[q];
+                }
+        const double al = rz / pAp;
+        double rmax = 0., rz2 = 0.;
+        for (size_t q = 0; q < N; ++q) {
+            x[q] += al * p[q];
+            r[q] -= al * Ap[q];
+            rmax = omax(rmax, fabs(r[q]));
+            z[q] = r[q] / dg[q];
+            rz2 += r[q] * z[q];
+        }
+        if (rmax <= tol * bmax) ok = 1;
+        const double be = rz2 / rz;
+        rz = rz2;
+        for (size_t q = 0; q < N; ++q) p[q] = z[q] + be * p[q];
+    }
+    spacing_free(&lv.sp);
+    free(V); free(E); free(dg); free(r); free(z); free(p); free(Ap);
+    return ok ? it : -1;
+}
+
+static double *tl_alloc(size_t n) { return (double *)calloc(n, sizeof(double)); }
+
+/* out = C2R(D23 d_b X) (real) */
+static void tl_dd_c2r(or_grid *g, int b, const double *X, double *tmp, double *out)
+{
+    or_deriv(g, b, X, tmp);
+    or_dealias23(g, tmp);
+    or_fft_c2r(g, tmp, out);
+}
+
+int or_rhs_eval_tl(or_rhs *r, double *F[4], double *T[4], double dM, const double *dnd)
+{
+    or_grid *g = r->g;
+    const size_t nc = g->nc, nr = g->nr, n2 = 2 * nc;
+    const or_params *P = &r->p;
+    double *FB[4], *TB[4], *U[3], *dU[3], *dF[3][3], *dT[3][3];
+    for (int a = 0; a < 4; ++a) { FB[a] = tl_alloc(n2); TB[a] = tl_alloc(n2); }
+    for (int a = 0; a < 3; ++a) {
+        U[a] = tl_alloc(nr); dU[a] = tl_alloc(nr);
+        for (int b = 0; b < 3; ++b) { dF[a][b] = tl_alloc(n2); dT[a][b] = tl_alloc(n2); }
+    }
+    double *c1 = tl_alloc(n2), *c2 = tl_alloc(n2), *c3 = tl_alloc(n2);
+    double *L = tl_alloc(nr), *dL = tl_alloc(nr), *rho = tl_alloc(nr), *drho = tl_alloc(nr), *dphi = tl_alloc(nr);
+    double *B1 = tl_alloc(nr), *B2 = tl_alloc(nr), *B3 = tl_alloc(nr), *TB2 = tl_alloc(nr);
+
+    /* Poisson and its linearisation: (e^phi - L_h) dphi = e^L dL + dnd */
+    memcpy(c1, F[3], sizeof(double) * n2);
+    or_dealias23(g, c1);
+    or_fft_c2r(g, c1, L);
+    memcpy(c1, T[3], sizeof(double) * n2);
+    or_dealias23(g, c1);
+    or_fft_c2r(g, c1, dL);
+    for (size_t q = 0; q < nr; ++q) {
+        rho[q] = exp(L[q]) + r->nd[q];
+        drho[q] = exp(L[q]) * dL[q] + (dnd ? dnd[q] : 0.);
+    }
+    r->status = poisson_call(r, r->phi, rho);
+    int st = tl_poisson(g, r->phi, drho, dphi, 1e-15);
+    if (st < 0) r->status = -7;
+
+    /* dissipation and spectral viscosity */
+    for (size_t i = 0; i < nc; ++i) {
+        const double f = P->sv_on ? P->tau + r->sv[i] : P->tau;
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 2; ++c) {
+                FB[a][2 * i + c] = f * F[a][2 * i + c];
+                TB[a][2 * i + c] = f * T[a][2 * i + c];
+            }
+        for (int c = 0; c < 2; ++c) {
+            FB[3][2 * i + c] = P->sv_on ? r->sv[i] * F[3][2 * i + c] : 0.;
+            TB[3][2 * i + c] = P->sv_on ? r->sv[i] * T[3][2 * i + c] : 0.;
+        }
+    }
+    /* E force and pressure: grad(filter FFT(phi) + Fn / theta) */
+    or_fft_r2c(g, r->phi, c1);
+    or_filter_phi(g, c1);
+    or_fft_r2c(g, dphi, c2);
+    or_filter_phi(g, c2);
+    for (size_t i = 0; i < n2; ++i) {
+        c1[i] += F[3][i] / P->theta;
+        c2[i] += T[3][i] / P->theta;
+    }
+    for (int a = 0; a < 3; ++a) {
+        or_deriv(g, a, c1, c3);
+        cadd(FB[a], c3, nc);
+        or_deriv(g, a, c2, c3);
+        cadd(TB[a], c3, nc);
+    }
+    /* dealiased inputs, velocities and their gradients */
+    for (int a = 0; a < 4; ++a) {
+        or_dealias23(g, F[a]);
+        or_dealias23(g, T[a]);
+    }
+    for (int a = 0; a < 3; ++a) {
+        or_fft_c2r(g, F[a], U[a]);
+        or_fft_c2r(g, T[a], dU[a]);
+        for (int b = 0; b < 3; ++b) {
+            or_deriv(g, b, F[a], dF[a][b]);
+            or_deriv(g, b, T[a], dT[a][b]);
+        }
+    }
+    /* translation M d_x */
+    for (int a = 0; a < 3; ++a)
+        for (size_t i = 0; i < n2; ++i) {
+            FB[a][i] += P->M * dF[a][0][i];
+            TB[a][i] += P->M * dT[a][0][i] + dM * dF[a][0][i];
+        }
+    or_deriv(g, 0, F[3], c1);
+    or_deriv(g, 0, T[3], c2);
+    for (size_t i = 0; i < n2; ++i) {
+        FB[3][i] += P->M * c1[i];
+        TB[3][i] += P->M * c2[i] + dM * c1[i];
+    }
+    /* continuity: div u + FFT(u . grad ln n) */
+    for (int a = 0; a < 3; ++a) {
+        cadd(FB[3], dF[a][a], nc);
+        cadd(TB[3], dT[a][a], nc);
+    }
+    for (size_t q = 0; q < nr; ++q) { B2[q] = 0.; TB2[q] = 0.; }
+    for (int b = 0; b < 3; ++b) {
+        tl_dd_c2r(g, b, F[3], c3, B1);
+        tl_dd_c2r(g, b, T[3], c3, B3);
+        for (size_t q = 0; q < nr; ++q) {
+            B2[q] += B1[q] * U[b][q];
+            TB2[q] += B3[q] * U[b][q] + B1[q] * dU[b][q];
+        }
+    }
+    or_fft_r2c(g, B2, c1);
+    cadd(FB[3], c1, nc);
+    or_fft_r2c(g, TB2, c1);
+    cadd(TB[3], c1, nc);
+    /* advection FFT(u . grad u_a) */
+    for (int a = 0; a < 3; ++a) {
+        for (size_t q = 0; q < nr; ++q) { B2[q] = 0.; TB2[q] = 0.; }
+        for (int b = 0; b < 3; ++b) {
+            or_dealias23(g, dF[a][b]);
+            or_dealias23(g, dT[a][b]);
+            or_fft_c2r(g, dF[a][b], B1);
+            or_fft_c2r(g, dT[a][b], B3);
+            for (size_t q = 0; q < nr; ++q) {
+                B2[q] += U[b][q] * B1[q];
+                TB2[q] += dU[b][q] * B1[q] + U[b][q] * B3[q];
+            }
+        }
+        or_fft_r2c(g, B2, c1);
+        cadd(FB[a], c1, nc);
+        or_fft_r2c(g, TB2, c1);
+        cadd(TB[a], c1, nc);
+    }
+    /* penalization blend */
+    if (P->pen_on) {
+        const double eps1 = 2., target[4] = {P->M, 0., 0., 10.};
+        double *TB1r = dL, *TB2r = drho; /* reuse */
+        for (int a = 0; a < 4; ++a) {
+            or_fft_c2r(g, F[a], B1);
+            or_fft_c2r(g, FB[a], B2);
+            or_fft_c2r(g, T[a], TB1r);
+            or_fft_c2r(g, TB[a], TB2r);
+            for (size_t q = 0; q < nr; ++q) {
+                const double H = r->mask[q], bar = r->barrier[q];
+                double p = (a == 0) ? eps1 * H * (B1[q] + target[a] * H) : eps1 * H * (B1[q] + target[a]);
+                p += (1. - H) * B2[q] + bar * B1[q];
+                double tp = eps1 * H * TB1r[q] + (1. - H) * TB2r[q] + bar * TB1r[q];
+                if (a == 0) tp += eps1 * H * H * dM;
+                B1[q] = p;
+                B3[q] = tp;
+            }
+            or_fft_r2c(g, B1, FB[a]);
+            or_dealias23(g, FB[a]);
+            or_fft_r2c(g, B3, TB[a]);
+            or_dealias23(g, TB[a]);
+        }
+    }
+    for (int a = 0; a < 4; ++a)
+        for (size_t i = 0; i < n2; ++i) {
+            F[a][i] = -FB[a][i];
+            T[a][i] = -TB[a][i];
+        }
+    for (int a = 0; a < 4; ++a) { free(FB[a]); free(TB[a]); }
+    for (int a = 0; a < 3; ++a) {
+        free(U[a]); free(dU[a]);
+        for (int b = 0; b < 3; ++b) { free(dF[a][b]); free(dT[a][b]); }
+    }
+    free(c1); free(c2); free(c3); free(L); free(dL); free(rho); free(drho); free(dphi);
+    free(B1); free(B2); free(B3); free(TB2);
+    return r->status;
+}
+
+/* one RK4 step of the state y and of its tangent ty (RKO4.cpp:50-248) */
+int or_rk4_step_tl(or_rhs *r, double dt, double *y[4], double *ty[4], double dM, const double *dnd)
+{
+    const size_t n2 = 2 * r->g->nc;
+    double *S[4], *TS[4], *acc[4], *tacc[4];
+    for (int f = 0; f < 4; ++f) {
+        S[f] = tl_alloc(n2); TS[f] = tl_alloc(n2); acc[f] = tl_alloc(n2); tacc[f] = tl_alloc(n2);
+        memcpy(S[f], y[f], sizeof(double) * n2);
+        memcpy(TS[f], ty[f], sizeof(double) * n2);
+    }
+    const double c[4] = {0.5, 0.5, 1.0, 0.}, w[4] = {1., 2., 2., 1.};
+    int st = 0;
+    for (int s = 0; s < 4; ++s) {
+        st |= or_rhs_eval_tl(r, S, TS, dM, dnd);
+        for (int f = 0; f < 4; ++f)
+            for (size_t i = 0; i < n2; ++i) {
+                acc[f][i] += w[s] * S[f][i];
+                tacc[f][i] += w[s] * TS[f][i];
+                if (s < 3) {
+                    S[f][i] = y[f][i] + c[s] * dt * S[f][i];
+                    TS[f][i] = ty[f][i] + c[s] * dt * TS[f][i];
+                }
+            }
+    }
+    for (int f = 0; f < 4; ++f) {
+        for (size_t i = 0; i < n2; ++i) {
+            y[f][i] += (dt / 6.) * acc[f][i];
+            ty[f][i] += (dt / 6.) * tacc[f][i];
+        }
+        free(S[f]); free(TS[f]); free(acc[f]); free(tacc[f]);
+    }
+    return st;
+}
+
+/* J = -sum n_d dV E_x with E_x = C2R(d_x FFT(phi)), phi the Poisson solution
+ * of rho = exp(C2R(D23 Fn)) + n_d (warm start: the RHS's phi), and its
+ * tangent dJ along (dFn, dnd).  Returns J. */
+double or_objective_tl(or_rhs *r, const double *Fn, const double *Tn, const double *dV, const double *dnd,
+                       double *dJ)
+{
+    or_grid *g = r->g;
+    const size_t nc = g->nc, nr = g->nr, n2 = 2 * nc;
+    double *c1 = tl_alloc(n2), *c2 = tl_alloc(n2);
+    double *L = tl_alloc(nr), *dL = tl_alloc(nr), *rho = tl_alloc(nr), *drho = tl_alloc(nr);
+    double *phi = tl_alloc(nr), *dphi = tl_alloc(nr), *E = tl_alloc(nr), *dE = tl_alloc(nr);
+    memcpy(c1, Fn, sizeof(double) * n2);
+    or_dealias23(g, c1);
+    or_fft_c2r(g, c1, L);
+    if (Tn) {
+        memcpy(c1, Tn, sizeof(double) * n2);
+        or_dealias23(g, c1);
+        or_fft_c2r(g, c1, dL);
+    }
+    for (size_t q = 0; q < nr; ++q) {
+        rho[q] = exp(L[q]) + r->nd[q];
+        drho[q] = exp(L[q]) * dL[q] + (dnd ? dnd[q] : 0.);
+    }
+    memcpy(phi, r->phi, sizeof(double) * nr);
+    double res;
+    if (or_poisson_fas(g, phi, rho, r->p.poisson_tol, r->p.poisson_max, &res) < 0) r->status = -1;
+    if (tl_poisson(g, phi, drho, dphi, 1e-15) < 0) r->status = -7;
+    or_fft_r2c(g, phi, c1);
+    or_deriv(g, 0, c1, c2);
+    or_fft_c2r(g, c2, E);
+    or_fft_r2c(g, dphi, c1);
+    or_deriv(g, 0, c1, c2);
+    or_fft_c2r(g, c2, dE);
+    double J = 0., t = 0.;
+    for (size_t q = 0; q < nr; ++q) {
+        J -= r->nd[q] * dV[q] * E[q];
+        t -= r->nd[q] * dV[q] * dE[q] + (dnd ? dnd[q] : 0.) * dV[q] * E[q];
+    }
+    if (dJ) *dJ = t;
+    free(c1); free(c2); free(L); free(dL); free(rho); free(drho); free(phi); free(dphi); free(E); free(dE);
+    return J;
+}

@@ -113,6 +113,14 @@ int  or_rhs_poisson_status(const or_rhs *r, double *last_resid, int *last_iters)
 /* ---- RK4 (RKO4.cpp:50-248) -------------------------------------------- */
 int or_rk4_step(or_rhs *r, double dt, double *t_io,
                 double *FUx, double *FUy, double *FUz, double *Fn);
+
+/* tangent-linear model (forward-mode derivative): F is the state (in place,
+ * becomes the RHS as or_rhs_eval), T the tangent direction (becomes the
+ * tangent of the RHS) for parameter directions dM and dnd (may be NULL) */
+int or_rhs_eval_tl(or_rhs *r, double *F[4], double *T[4], double dM, const double *dnd);
+int or_rk4_step_tl(or_rhs *r, double dt, double *y[4], double *ty[4], double dM, const double *dnd);
+double or_objective_tl(or_rhs *r, const double *Fn, const double *Tn, const double *dV, const double *dnd,
+                       double *dJ);
 /* CFL.cpp:29-82 */
 double or_cfl(const or_grid *g, double dt, const double *FUx, const double *FUy, const double *FUz);
 

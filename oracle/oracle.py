@@ -96,6 +96,12 @@ def lib():
         L.or_rhs_poisson_status.argtypes = [vp, dp, C.POINTER(C.c_int)]
         L.or_rk4_step.restype = C.c_int
         L.or_rk4_step.argtypes = [vp, C.c_double, dp, vp, vp, vp, vp]
+        L.or_rhs_eval_tl.restype = C.c_int
+        L.or_rhs_eval_tl.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.c_double, vp]
+        L.or_rk4_step_tl.restype = C.c_int
+        L.or_rk4_step_tl.argtypes = [vp, C.c_double, C.POINTER(vp), C.POINTER(vp), C.c_double, vp]
+        L.or_objective_tl.restype = C.c_double
+        L.or_objective_tl.argtypes = [vp, vp, vp, vp, vp, dp]
         L.or_cfl.restype = C.c_double
         L.or_cfl.argtypes = [vp, C.c_double, vp, vp, vp]
         L.or_philox4x32_10.argtypes = [vp, vp, vp]
@@ -291,6 +297,32 @@ class Rhs:
         it = C.c_int(0)
         st = lib().or_rhs_poisson_status(self.h, C.byref(res), C.byref(it))
         return st, res.value, it.value
+
+    def eval_tl(self, state, tangent, dM=0.0, dnd=None):
+        """RHS and its tangent along (tangent, dM, dnd): returns (F, T, status)."""
+        s = [np.array(f, dtype=np.complex128, order="C") for f in state]
+        t = [np.array(f, dtype=np.complex128, order="C") for f in tangent]
+        d = None if dnd is None else np.ascontiguousarray(dnd, np.float64)
+        st = lib().or_rhs_eval_tl(self.h, (C.c_void_p * 4)(*[_p(a) for a in s]),
+                                  (C.c_void_p * 4)(*[_p(a) for a in t]), dM, None if d is None else _p(d))
+        return s, t, st
+
+    def rk4_tl(self, dt, y, ty, dM=0.0, dnd=None):
+        """One RK4 step of y and of its tangent ty, both in place; returns status."""
+        d = None if dnd is None else np.ascontiguousarray(dnd, np.float64)
+        return lib().or_rk4_step_tl(self.h, dt, (C.c_void_p * 4)(*[_p(a) for a in y]),
+                                    (C.c_void_p * 4)(*[_p(a) for a in ty]), dM, None if d is None else _p(d))
+
+    def objective_tl(self, Fn, Tn, dV, dnd=None):
+        """(J, dJ) of the grain drag at spectral ln n Fn along (Tn, dnd)."""
+        Fn = np.ascontiguousarray(Fn, np.complex128)
+        Tn = None if Tn is None else np.ascontiguousarray(Tn, np.complex128)
+        dV = np.ascontiguousarray(dV, np.float64)
+        d = None if dnd is None else np.ascontiguousarray(dnd, np.float64)
+        dJ = C.c_double(0)
+        J = lib().or_objective_tl(self.h, _p(Fn), None if Tn is None else _p(Tn), _p(dV),
+                                  None if d is None else _p(d), C.byref(dJ))
+        return J, dJ.value
 
     def rk4(self, dt, t, state):
         """In place on the 4 arrays of `state`; returns (t_new, status)."""

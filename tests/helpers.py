@@ -47,3 +47,10 @@ def initial_state(kind, n, seed=0, amp=1e-3, Q=-0.1, m=None):
     Fn = og.r2c(ln_n)
     z = np.zeros(og.cshape, complex)
     return [z.copy(), z.copy(), z.copy(), Fn], nd, mask, og
+
+
+def cell_volumes(og):
+    """dV = prod_b S'_b(xi) L_b / N_b on the oracle grid's axes."""
+    from oracle import oracle as O
+    d = [np.array([O.dS(ax, i) * ax.L / ax.N for i in range(ax.N)]) for ax in og.axes]
+    return d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :]

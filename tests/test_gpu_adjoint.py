@@ -4,7 +4,7 @@ operators, and a checkpointed run against finite differences of J."""
 import numpy as np
 import pytest
 
-from helpers import axes_for, grain_radius, initial_state
+from helpers import cell_volumes, initial_state
 
 pytestmark = pytest.mark.gpu
 
@@ -112,9 +112,7 @@ def test_rk4_step_adjoint_dot_product(ctx, kind):
 
 
 def _dV(og):
-    from oracle import oracle as O
-    d = [np.array([O.dS(ax, i) * ax.L / ax.N for i in range(ax.N)]) for ax in og.axes]
-    return d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :]
+    return cell_volumes(og)
 
 
 @pytest.mark.parametrize("kind", ["eqst", "sist"])
@@ -138,14 +136,30 @@ def test_objective_adjoint(ctx, kind):
     assert abs(fd - _ip(v, Fb[3])) <= 1e-6 * abs(fd), (fd, _ip(v, Fb[3]), J)
 
 
+def _tangent_dJ(og, F0, nd, mask, steps, dV, dM=0.0, dnd=None, dFn=None, dt=1e-3):
+    """dJ along (dM, dnd, dFn0) from the oracle's FP64 tangent-linear run."""
+    from oracle import oracle as O
+    r = O.Rhs(og, M=1.0, tau=0.1, theta=100.0, sv_on=False, pen_on=True, poisson_mode=O.POISSON_FAS,
+              poisson_tol=1e-14, poisson_max=400)
+    r.set_fields(np.zeros(og.rshape), nd, mask)
+    y = [np.array(f, dtype=np.complex128) for f in F0]
+    ty = [np.zeros(og.cshape, complex) for _ in range(3)] + [
+        np.zeros(og.cshape, complex) if dFn is None else np.array(dFn, dtype=np.complex128)]
+    for _ in range(steps):
+        assert r.rk4_tl(dt, y, ty, dM, dnd) == 0
+    return r.objective_tl(y[3], ty[3], dV, dnd)
+
+
 def test_checkpointed_run_gradients(ctx):
-    """dJ/dM, dJ/dQ and dJ/dFn0 of a short run against central differences."""
+    """dJ/dM, dJ/dQ and dJ/dFn0 of a short run: kept and recomputed segments
+    agree, and both match central differences and the CPU tangent run."""
     import kielflow_amd as kf
     print("adjoint run", flush=True)
     kind, n, steps, k = "hyst", 16, 6, 4
     g, ps, rhs, phi, F0, nd, mask, og = _setup(ctx, kind, n, Q=-0.2)
     G = nd / -0.2
-    dV = ctx.to_device(_dV(og))
+    dVh = _dV(og)
+    dV = ctx.to_device(dVh)
     dG = ctx.to_device(G)
     adj = kf.Adjoint(rhs, 1e-3)
     phi0 = phi.get()
@@ -163,12 +177,22 @@ def test_checkpointed_run_gradients(ctx):
         rhs.set_fields(None, ctx.to_device(-0.2 * G), None)
         return J
 
-    phi.upload(phi0)
-    y = [ctx.to_device(s) for s in F0]
-    dFn0 = g.cfield()
-    J, dM, dQ = adj.run(y, steps, k, dV, dG, dFn0)
+    def run_adj():
+        phi.upload(phi0)
+        y = [ctx.to_device(s) for s in F0]
+        dFn0 = g.cfield()
+        J, dM, dQ = adj.run(y, steps, k, dV, dG, dFn0)
+        return J, dM, dQ, dFn0.get()
+
+    J, dM, dQ, dFn0 = run_adj()
+    assert adj.run_info()[0]
+    adj.set_capture_budget(0)
+    J2, dM2, dQ2, dFn02 = run_adj()
+    assert not adj.run_info()[0]
+    assert abs(dM2 - dM) <= 1e-9 * abs(dM) and abs(dQ2 - dQ) <= 1e-9 * abs(dQ)
+    assert np.abs(dFn02 - dFn0).max() <= 1e-9 * np.abs(dFn0).max()
     J0 = run_J()
-    assert abs(J - J0) <= 1e-10 * abs(J0)
+    assert abs(J - J0) <= 1e-8 * abs(J0)
     hM, hQ = 1e-3, 1e-4
     fdM = (run_J(M=1.0 + hM) - run_J(M=1.0 - hM)) / (2 * hM)
     fdQ = (run_J(Q=-0.2 + hQ) - run_J(Q=-0.2 - hQ)) / (2 * hQ)
@@ -181,4 +205,14 @@ def test_checkpointed_run_gradients(ctx):
     Fp = [F0[0], F0[1], F0[2], F0[3] + hv * v]
     Fm = [F0[0], F0[1], F0[2], F0[3] - hv * v]
     fdv = (run_J(F=Fp) - run_J(F=Fm)) / (2 * hv)
-    assert abs(fdv - _ip(v, dFn0.get())) <= 1e-4 * abs(fdv), (fdv, _ip(v, dFn0.get()))
+    assert abs(fdv - _ip(v, dFn0)) <= 1e-4 * abs(fdv), (fdv, _ip(v, dFn0))
+    # the CPU FP64 tangent-linear run along the same three directions
+    Jt, tM = _tangent_dJ(og, F0, nd, mask, steps, dVh, dM=1.0)
+    _, tQ = _tangent_dJ(og, F0, nd, mask, steps, dVh, dnd=G)
+    _, tv = _tangent_dJ(og, F0, nd, mask, steps, dVh, dFn=v)
+    print(f"tangent: J={Jt:.10e} dM={tM:.12e}/{dM:.12e} dQ={tQ:.12e}/{dQ:.12e} v={tv:.12e}/{_ip(v, dFn0):.12e}",
+          flush=True)
+    assert abs(Jt - J) <= 1e-8 * abs(J)
+    assert abs(tM - dM) <= 1e-7 * abs(tM)
+    assert abs(tQ - dQ) <= 1e-7 * abs(tQ)
+    assert abs(tv - _ip(v, dFn0)) <= 1e-7 * abs(tv)
