@@ -117,6 +117,45 @@ int kf_dealias23(kf_grid g, double *inout);
 /* dealiasing_undesignated(F, exp(-2000 k^15)) (operations.hpp:20-46, rhs_standard.cpp:220) */
 int kf_filter_phi(kf_grid g, double *inout);
 
+/* ---------------------------------------------------------------- effects
+ * kielflow's effect_* interfaces on FP64 spectral device fields.  Each adds
+ * its term of d/dt to the output buffer(s), as the reference's effect
+ * classes are used; the arithmetic is the term rhs_standard::solve inlines,
+ * so the sequence below reproduces kf_rhs_solve (rhs_standard.cpp:89-566):
+ *   out_a  = force_linear(-tau) FU_a [+ spectral_viscosity FU_a]
+ *   out_n  = [spectral_viscosity Fn]
+ *   force_E(FFT(Phi)); diffusion_u(theta, Fn)       (Fn not yet dealiased)
+ *   dealias FU, Fn; translation(M) on FU_a and Fn
+ *   out_n += continuity(FU, Fn); advection(FU, FU_a) into out_a
+ *   penalization(mask, -2, fix, FU_a or Fn) on each buffer
+ *     (fix = -M scaled by the mask for u_x, 0 for u_y, u_z, -10 for ln n) */
+/* out -= FFT(u . grad(D23 FXX)), u = iFFT(D23 FU)
+ * (effect_advection::execute, effect_advection.hpp:13-15; rhs_standard.cpp:388-437) */
+int kf_effect_advection(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *FXX,
+                        double *out);
+/* out = -(div(D23 FU) + FFT(u . grad(D23 Fni)))
+ * (effect_continuityEQ::execute, effect_continuityEQ.hpp:14-15; rhs_standard.cpp:315-345) */
+int kf_effect_continuity(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *Fni,
+                         double *out);
+/* out_a -= d_a(filter(FPhi)), filter = exp(-2000 khat^15)
+ * (effect_force_E::execute, effect_force_E.hpp:14-15; rhs_standard.cpp:219-247) */
+int kf_effect_force_E(kf_grid g, const double *FPhi, double *outx, double *outy, double *outz);
+/* out_a -= d_a(Fni) / theta, log form
+ * (effect_diffusion_u::execute, effect_diffusion_u.hpp:15-16; rhs_standard.cpp:219-247) */
+int kf_effect_diffusion_u(kf_grid g, double theta, const double *Fni, double *outx, double *outy, double *outz);
+/* out -= SV in, SV = 0.5 (1 - exp(-8 khat^4))
+ * (effect_spectral_viscosity::execute, effect_spectral_viscosity.hpp:20-21; rhs_standard.cpp:51-77,119-140) */
+int kf_effect_spectral_viscosity(kf_grid g, const double *in, double *out);
+/* out -= v d_x(D23 in)
+ * (effect_translation::execute, effect_translation.hpp:15-16; rhs_standard.cpp:279-310) */
+int kf_effect_translation(kf_grid g, double v, const double *in, double *out);
+/* out += k in   (effect_force_linear::execute, effect_force_linear.hpp:14-15; k = -tau in rhs_standard.cpp:119-140) */
+int kf_effect_force_linear(kf_grid g, double k, const double *in, double *out);
+/* out = D23 FFT((1-H) iFFT(out) + eta H (iFFT(in) - fix [H]) - b iFFT(in)), b the upstream barrier
+ * (effect_penalization::execute, effect_penalization.hpp:16-18; rhs_standard.cpp:459-533); eta <= 0 */
+int kf_effect_penalization(kf_grid g, const double *mask, double eta, double fix, int fix_scaled_by_mask,
+                           const double *in, double *out);
+
 /* ---------------------------------------------------------------- Poisson
  * interface_relaxation_solver (interface_poisson_solver.hpp:14-34) and
  * solver_poisson_jacobi_nlin / solver_poisson_multigrid (their .hpp:38-56). */

@@ -65,6 +65,14 @@ SIGNATURES = {
     "kf_fft_c2r": (C.c_int, [_vp, _vp, _vp]),
     "kf_partial_derivative": (C.c_int, [_vp, C.c_int, _vp, _vp]),
     "kf_dealias23": (C.c_int, [_vp, _vp]),
+    "kf_effect_advection": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "kf_effect_continuity": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "kf_effect_force_E": (C.c_int, [_vp, _vp, _vp, _vp, _vp]),
+    "kf_effect_diffusion_u": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _vp]),
+    "kf_effect_spectral_viscosity": (C.c_int, [_vp, _vp, _vp]),
+    "kf_effect_translation": (C.c_int, [_vp, C.c_double, _vp, _vp]),
+    "kf_effect_force_linear": (C.c_int, [_vp, C.c_double, _vp, _vp]),
+    "kf_effect_penalization": (C.c_int, [_vp, _vp, C.c_double, C.c_double, C.c_int, _vp, _vp]),
     "kf_filter_phi": (C.c_int, [_vp, _vp]),
     "kf_poisson_create": (C.c_int, [_vp, C.c_int, C.c_int, C.POINTER(_vp)]),
     "kf_poisson_destroy": (C.c_int, [_vp]),
@@ -260,6 +268,32 @@ class Grid:
 
     def filter_phi(self, X: DeviceArray):
         check(self.ctx.lib.kf_filter_phi(self.h, X.ptr))
+
+    # effect_* interfaces (terms of d/dt added to the output buffers)
+    def effect_advection(self, FU, FXX, out):
+        check(self.ctx.lib.kf_effect_advection(self.h, FU[0].ptr, FU[1].ptr, FU[2].ptr, FXX.ptr, out.ptr))
+
+    def effect_continuity(self, FU, Fni, out):
+        check(self.ctx.lib.kf_effect_continuity(self.h, FU[0].ptr, FU[1].ptr, FU[2].ptr, Fni.ptr, out.ptr))
+
+    def effect_force_E(self, FPhi, out):
+        check(self.ctx.lib.kf_effect_force_E(self.h, FPhi.ptr, out[0].ptr, out[1].ptr, out[2].ptr))
+
+    def effect_diffusion_u(self, theta, Fni, out):
+        check(self.ctx.lib.kf_effect_diffusion_u(self.h, theta, Fni.ptr, out[0].ptr, out[1].ptr, out[2].ptr))
+
+    def effect_spectral_viscosity(self, X, out):
+        check(self.ctx.lib.kf_effect_spectral_viscosity(self.h, X.ptr, out.ptr))
+
+    def effect_translation(self, v, X, out):
+        check(self.ctx.lib.kf_effect_translation(self.h, v, X.ptr, out.ptr))
+
+    def effect_force_linear(self, k, X, out):
+        check(self.ctx.lib.kf_effect_force_linear(self.h, k, X.ptr, out.ptr))
+
+    def effect_penalization(self, mask, eta, fix, fix_scaled_by_mask, X, out):
+        check(self.ctx.lib.kf_effect_penalization(self.h, mask.ptr, eta, fix, int(fix_scaled_by_mask), X.ptr,
+                                                  out.ptr))
 
     def cfl(self, dt, FUx, FUy, FUz):
         out = C.c_double(0)

@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libkielflow_amd.so")
-SOURCES = ["kf_fft.hip", "kf_grid.hip", "kf_poisson.hip", "kf_rhs.hip", "kf_adjoint.hip",
+SOURCES = ["kf_fft.hip", "kf_grid.hip", "kf_poisson.hip", "kf_rhs.hip", "kf_adjoint.hip", "kf_effects.hip",
            "kf_batch.hip", "kf_prof.hip", "kf_capi.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KF_OFFLOAD_ARCH", "gfx950")
@@ -46,7 +46,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "host", "kf_driver.cpp")
+DRIVER = os.path.join(HERE, "host", "kf_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The kielflow-style C++ driver over the reference-interface adaptors
+    (host code only: g++ against the C ABI of the library)."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", inc, DRIVER_SRC, "-L", HERE,
+           "-lkielflow_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stderr}")
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

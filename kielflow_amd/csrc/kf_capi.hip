@@ -8,6 +8,7 @@
 #include <string>
 
 #include "kf_adjoint.h"
+#include "kf_effects.h"
 #include "kf_fft.h"
 #include "kf_grid.h"
 #include "kf_poisson.h"
@@ -26,6 +27,12 @@ struct kf_grid_s {
     kf_ctx ctx;
     Grid *g;
     fft::Plan<double> *p64;  // FP64 plan for the OP_* operators and CFL
+    EffectWork *ew;          // scratch of the effect_* entries (on first use)
+    EffectWork &work()
+    {
+        if (!ew) ew = new EffectWork(*g);
+        return *ew;
+    }
 };
 struct kf_fft_s {
     kf_grid grid;
@@ -220,7 +227,7 @@ int kf_grid_create(kf_ctx ctx, const kf_axis axes[3], kf_grid *out)
         std::unique_ptr<Grid> g(new Grid(ad, ctx->stream));
         std::unique_ptr<fft::Plan<double>> p(new fft::Plan<double>());
         fft::plan_init<double>(*p, ad[0].n, ad[1].n, ad[2].n, ctx->stream);
-        *out = new kf_grid_s{ctx, g.release(), p.release()};
+        *out = new kf_grid_s{ctx, g.release(), p.release(), nullptr};
         return KF_OK;
     });
 }
@@ -232,6 +239,7 @@ int kf_grid_destroy(kf_grid g)
         (void)hipStreamSynchronize(g->ctx->stream);
         fft::plan_free<double>(*g->p64);
         delete g->p64;
+        delete g->ew;
         delete g->g;
         delete g;
         return KF_OK;
@@ -323,6 +331,97 @@ int kf_partial_derivative(kf_grid g, int dir, const double *in, double *out)
         KF_REQUIRE(g && in && out, "null argument");
         KF_REQUIRE(dir >= 0 && dir < 3, "direction must be 0, 1 or 2");
         op_partial_derivative(*g->g, *g->p64, dir, (const double2 *)in, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+// ---------------------------------------------------------------- effect_* interfaces
+int kf_effect_advection(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *FXX,
+                        double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FUx && FUy && FUz && FXX && out, "null argument");
+        const double2 *FU[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
+        effect_advection(*g->g, *g->p64, g->work(), FU, (const double2 *)FXX, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_continuity(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *Fni,
+                         double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FUx && FUy && FUz && Fni && out, "null argument");
+        const double2 *FU[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
+        effect_continuity(*g->g, *g->p64, g->work(), FU, (const double2 *)Fni, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_force_E(kf_grid g, const double *FPhi, double *outx, double *outy, double *outz)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FPhi && outx && outy && outz, "null argument");
+        double2 *o[3] = {(double2 *)outx, (double2 *)outy, (double2 *)outz};
+        effect_force_E(*g->g, *g->p64, g->work(), (const double2 *)FPhi, o, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_diffusion_u(kf_grid g, double theta, const double *Fni, double *outx, double *outy, double *outz)
+{
+    return guard([&] {
+        KF_REQUIRE(g && Fni && outx && outy && outz, "null argument");
+        KF_REQUIRE(theta != 0., "theta must be nonzero");
+        double2 *o[3] = {(double2 *)outx, (double2 *)outy, (double2 *)outz};
+        effect_diffusion_u(*g->g, *g->p64, g->work(), theta, (const double2 *)Fni, o, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_spectral_viscosity(kf_grid g, const double *in, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && in && out, "null argument");
+        effect_spectral_viscosity(*g->g, (const double2 *)in, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_translation(kf_grid g, double v, const double *in, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && in && out, "null argument");
+        effect_translation(*g->g, *g->p64, g->work(), v, (const double2 *)in, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_force_linear(kf_grid g, double k, const double *in, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && in && out, "null argument");
+        effect_force_linear(*g->g, k, (const double2 *)in, (double2 *)out, g->ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_effect_penalization(kf_grid g, const double *mask, double eta, double fix, int fix_scaled_by_mask,
+                           const double *in, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && mask && in && out, "null argument");
+        KF_REQUIRE(eta <= 0., "penalization damping must not be positive (effect_penalization.cpp:9-15)");
+        effect_penalization(*g->g, *g->p64, g->work(), mask, eta, fix, fix_scaled_by_mask ? 1 : 0,
+                            (const double2 *)in, (double2 *)out, g->ctx->stream);
         KF_HIP(hipGetLastError());
         return KF_OK;
     });

@@ -50,8 +50,9 @@ class Rhs {
     void eval(double t, C2<R> *F[4]);
     void set_force_general(bool v) { force_general_ = v; }
     // RK stage (0..3) of the next evaluations, or -1.  Within an RK run the
-    // Poisson solve of stage s starts from the linear extrapolation in time
-    // of the last two converged potentials of stage s.
+    // Poisson solve of stage s starts from the quadratic (linear while only
+    // two exist) extrapolation in time of the last converged potentials of
+    // stage s.
     void set_stage(int s);
     // evaluations store their intermediates into *c (general path) while set
     void set_capture(Capture *c) { cap_ = c; }
@@ -109,7 +110,7 @@ class Rhs {
         if (cap_) return cap_;
         return (cap_stages_ && stage_ >= 0) ? &cap_stages_[stage_] : nullptr;
     }
-    double *ph_[4][2] = {};
+    double *ph_[4][3] = {};  // per RK stage: the last three converged potentials, newest last
     int hc_[4] = {0, 0, 0, 0};
 };
 

@@ -199,11 +199,14 @@ int Rhs<R>::poisson_failures()
     return f;
 }
 
-__global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h1, const double *__restrict__ h0,
-                         long n)
+// phi = polynomial extrapolation in time of the last converged potentials
+// h2 (newest), h1, h0 (h0 unused when order 1)
+__global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h2, const double *__restrict__ h1,
+                         const double *__restrict__ h0, int order, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) phi[i] = 2. * h1[i] - h0[i];
+    if (i >= n) return;
+    phi[i] = order == 2 ? 3. * (h2[i] - h1[i]) + h0[i] : 2. * h2[i] - h1[i];
 }
 
 template <typename R>
@@ -227,14 +230,20 @@ void Rhs<R>::poisson_step(C2<R> *Fn)
     c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
     if (!phi_) throw Error("RHS potential field not set");
     const long nr = g_.nr;
-    if (stage_ >= 0 && hc_[stage_] >= 2)
-        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, (const double *)ph_[stage_][1],
-                           (const double *)ph_[stage_][0], nr);
+    if (stage_ >= 0 && hc_[stage_] >= 2) {
+        double **h = ph_[stage_];
+        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, (const double *)h[2], (const double *)h[1],
+                           (const double *)h[0], hc_[stage_] >= 3 ? 2 : 1, nr);
+    }
     if constexpr (std::is_same<R, double>::value) ps_.solve(phi_, rho_, nullptr, s_, false);
     else ps_.solve(phi_, nullptr, rho_, s_, false);
     if (stage_ >= 0) {
-        std::swap(ph_[stage_][0], ph_[stage_][1]);
-        KF_HIP(hipMemcpyAsync(ph_[stage_][1], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        double **h = ph_[stage_];
+        double *oldest = h[0];
+        h[0] = h[1];
+        h[1] = h[2];
+        h[2] = oldest;
+        KF_HIP(hipMemcpyAsync(h[2], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
         hc_[stage_]++;
     }
 }

@@ -1693,3 +1693,10 @@ double or_objective_tl(or_rhs *r, const double *Fn, const double *Tn, const doub
     free(c1); free(c2); free(L); free(dL); free(rho); free(drho); free(phi); free(dphi); free(E); free(dE);
     return J;
 }
+
+/* the cascade's transfer operators, exported for their known-answer tests */
+void or_mg_inject(const double *in, const int nin[3], double *out, int f) { inject(in, nin, out, f); }
+void or_mg_prolong(const double *in, const int nin[3], double *out, const or_axis out_ax[3])
+{
+    prolong_hto2h(in, nin, out, out_ax);
+}

@@ -114,6 +114,10 @@ int  or_rhs_poisson_status(const or_rhs *r, double *last_resid, int *last_iters)
 int or_rk4_step(or_rhs *r, double dt, double *t_io,
                 double *FUx, double *FUy, double *FUz, double *Fn);
 
+/* OP_8htoh-style injection by f and OP_hto2h prolongation (field_interpolation.cpp:56-147,233-315) */
+void or_mg_inject(const double *in, const int nin[3], double *out, int f);
+void or_mg_prolong(const double *in, const int nin[3], double *out, const or_axis out_ax[3]);
+
 /* tangent-linear model (forward-mode derivative): F is the state (in place,
  * becomes the RHS as or_rhs_eval), T the tangent direction (becomes the
  * tangent of the RHS) for parameter directions dM and dnd (may be NULL) */

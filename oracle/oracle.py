@@ -96,6 +96,8 @@ def lib():
         L.or_rhs_poisson_status.argtypes = [vp, dp, C.POINTER(C.c_int)]
         L.or_rk4_step.restype = C.c_int
         L.or_rk4_step.argtypes = [vp, C.c_double, dp, vp, vp, vp, vp]
+        L.or_mg_inject.argtypes = [vp, vp, vp, C.c_int]
+        L.or_mg_prolong.argtypes = [vp, vp, vp, C.POINTER(Axis)]
         L.or_rhs_eval_tl.restype = C.c_int
         L.or_rhs_eval_tl.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.c_double, vp]
         L.or_rk4_step_tl.restype = C.c_int
@@ -329,6 +331,23 @@ class Rhs:
         tt = C.c_double(t)
         st = lib().or_rk4_step(self.h, dt, C.byref(tt), *[_p(a) for a in state])
         return tt.value, st
+
+
+def mg_inject(a, f):
+    a = np.ascontiguousarray(a, np.float64)
+    nin = np.array(a.shape, np.int32)
+    out = np.empty(tuple(n // f for n in a.shape), np.float64)
+    lib().or_mg_inject(_p(a), _p(nin), _p(out), f)
+    return out
+
+
+def mg_prolong(a, fine_axes):
+    a = np.ascontiguousarray(a, np.float64)
+    nin = np.array(a.shape, np.int32)
+    out = np.empty(tuple(2 * n for n in a.shape), np.float64)
+    ax = (Axis * 3)(*fine_axes)
+    lib().or_mg_prolong(_p(a), _p(nin), _p(out), ax)
+    return out
 
 
 def philox(ctr, key):

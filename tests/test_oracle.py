@@ -312,3 +312,42 @@ def test_cfl_closed_form(oracle):
     Ux = g.r2c(np.full(g.rshape, 50.0))
     dt_new = g.cfl(0.01, Ux, z, z)
     assert dt_new == pytest.approx(0.4 * (L / n) / 50.0, rel=1e-12)
+
+
+def test_mg_injection_known_answer(oracle):
+    O = oracle
+    a = np.arange(8 * 8 * 8, dtype=np.float64).reshape(8, 8, 8)
+    assert np.array_equal(O.mg_inject(a, 2), a[::2, ::2, ::2])
+    assert np.array_equal(O.mg_inject(a, 8), a[::8, ::8, ::8])
+
+
+@pytest.mark.parametrize("kind", [1, 2])
+def test_mg_prolongation_known_answer(oracle, kind):
+    """OP_hto2h: even points copy, odd points interpolate linearly in the
+    fine grid's coordinates along z, then y, then x (periodic wrap)."""
+    O = oracle
+    n = 8
+    fine = [O.axis(kind, n, -8.0, 16.0, 3.0 if kind == 2 else 0.0) for _ in range(3)]
+    rng = np.random.default_rng(0)
+    c = rng.standard_normal((n // 2,) * 3)
+    got = O.mg_prolong(c, fine)
+    x = [np.array([O.val_at(ax, i) for i in range(-1, n + 2)]) for ax in fine]
+
+    def w(d, i):  # weight of the right neighbour at odd fine index i on axis d
+        xs = x[d]
+        return (xs[i + 1] - xs[i]) / (xs[i + 2] - xs[i])
+
+    ref = np.zeros((n, n, n))
+    ref[::2, ::2, ::2] = c
+    for k in range(1, n, 2):
+        g = w(2, k)
+        ref[::2, ::2, k] = (1 - g) * ref[::2, ::2, k - 1] + g * ref[::2, ::2, (k + 1) % n]
+    for j in range(1, n, 2):
+        g = w(1, j)
+        ref[::2, j, :] = (1 - g) * ref[::2, j - 1, :] + g * ref[::2, (j + 1) % n, :]
+    for i in range(1, n, 2):
+        g = w(0, i)
+        ref[i] = (1 - g) * ref[i - 1] + g * ref[(i + 1) % n]
+    assert np.allclose(got, ref, rtol=0, atol=1e-14)
+    if kind == 1:  # uniform grid: odd points are plain averages
+        assert np.allclose(got[1::2, ::2, ::2], 0.5 * (c + np.roll(c, -1, 0)), atol=1e-14)
