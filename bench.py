@@ -38,6 +38,13 @@ def parse():
     p.add_argument("--cpu-n", type=int, default=128, help="box size of the CPU sample")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     p.add_argument("--adjoint-steps", type=int, default=200)
+    p.add_argument("--no-batch", action="store_true", help="skip the batched-boxes leg")
+    p.add_argument("--smoothing", type=str, default="",
+                   help="Poisson smoothing sweeps 'pre,post,coarse' (library default when empty)")
+    p.add_argument("--long", type=int, default=0,
+                   help="only run the mixed and FP64 steppers side by side for this many steps, printing the "
+                        "per-field rel-L2 every 100 steps, and exit")
+    p.add_argument("--batch-steps", type=int, default=100)
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the plain device copy (1 GiB, 5 times) that the HBM byte counters are "
                         "calibrated against, and exit")
@@ -59,9 +66,14 @@ def copy_calibration():
     print(json.dumps({"copy_calibration_bytes_per_launch": nbytes, "launches": 5}))
 
 
+SMOOTHING = None
+
+
 def run_precision(kf, ctx, g, axes, prec, nd_d, mask_d, Fn0, steps, warmup, prof_family=None):
     P = kf.KF_MIXED if prec == "mixed" else kf.KF_FP64
     ps = kf.Poisson(g, P, tol=1e-10, max_it=60)
+    if SMOOTHING:
+        ps.set_smoothing(*SMOOTHING)
     rhs = kf.Rhs(g, ps, M=1.5, tau=0.1, theta=100.0, sv_on=False, pen_on=True, precision=P)
     phi = g.rfield()
     rhs.set_fields(phi, nd_d, mask_d)
@@ -182,6 +194,116 @@ def adjoint_cost(kf, ctx, n=64, steps=200, k=20, m=3.0):
             "segments_kept_in_hbm": kept, "transposed_poisson_iterations": lin_it}
 
 
+def batch_sweep(kf, ctx, nbox=64, steps=100, warm=3, seed=0, nstreams=4, poisson_iterations=6, check_boxes=4):
+    """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
+    M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
+    precision, one captured graph per box and step over `nstreams` streams.
+    Error: the first box of each size against an FP64 solo run."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.choice([32, 48, 64, 96], nbox)
+    Ms = rng.uniform(0.5, 2.0, nbox)
+    Qs = rng.uniform(-0.5, -0.05, nbox)
+    boxes, fields = [], []
+    for b in range(nbox):
+        n = int(sizes[b])
+        axes = kf.cube(n, L_BOX)
+        x0 = kf.api.axis_values(axes[0], n // 2)[0]
+        x1 = kf.api.axis_values(axes[0], n // 2 + 1)[0]
+        fields.append((axes, kf.setup_host(axes, seed=1000 + b, amp=1e-3, Q=Qs[b], a=2.0 * (x1 - x0))))
+        boxes.append((axes, kf.api.Parameters(Ms[b], 0.1, 100.0, 0.0, 0.0)))
+    B = kf.Batch(ctx, boxes, precision=kf.KF_MIXED, nstreams=nstreams)
+    for b, (axes, (ln_n, nd, mask)) in enumerate(fields):
+        B.set_fields(b, ctx.to_device(nd), ctx.to_device(mask))
+        F, _ = B.box(b)
+        g = kf.Grid(ctx, axes)
+        fft = kf.FFT(g, kf.KF_FP64)
+        fft.r2c(ctx.to_device(ln_n), F[3])
+        fft.close()
+        g.close()
+    B.steps(warm)
+    B.sync()
+    B.capture(poisson_iterations)
+    ctx.sync()
+    t0 = time.perf_counter()
+    B.steps(steps)
+    B.sync()
+    t = time.perf_counter() - t0
+    fails = B.poisson_failures()
+    errs = {}
+    for n in (32, 48, 64, 96):
+        idx = [b for b in range(nbox) if sizes[b] == n]
+        if not idx or len(errs) >= check_boxes:
+            continue
+        b = idx[0]
+        axes, (ln_n, nd, mask) = fields[b]
+        g = kf.Grid(ctx, axes)
+        ps = kf.Poisson(g, kf.KF_FP64, tol=1e-10, max_it=100)
+        rhs = kf.Rhs(g, ps, M=Ms[b], tau=0.1, theta=100.0, sv_on=False, pen_on=True)
+        phi = g.rfield()
+        rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+        F = [g.cfield() for _ in range(4)]
+        fft = kf.FFT(g, kf.KF_FP64)
+        fft.r2c(ctx.to_device(ln_n), F[3])
+        rk = kf.Rk4(rhs, 1e-3)
+        rk.step(F, warm + steps)
+        Fb, _ = B.box(b)
+        errs[f"box{b}_n{n}"] = {f: rel_l2(Fb[i].get(), F[i].get()) for i, f in
+                                 enumerate(("FUx", "FUy", "FUz", "ln_n"))}
+        for o in (rk, fft, rhs, ps, g):
+            o.close()
+    B.close()
+    pts = float(sum(int(n) ** 3 for n in sizes))
+    return {"workload": f"{nbox} boxes, sizes uniform in {{32,48,64,96}}^3 (seed {seed}), M in [0.5,2], "
+                        f"Q in [-0.5,-0.05], mixed precision, {steps} graph-replayed RK4 steps",
+            "box_steps_per_s": round(nbox * steps / t, 2), "batch_steps_per_s": round(steps / t, 3),
+            "grid_point_steps_per_s": pts * steps / t, "sizes": {str(n): int((sizes == n).sum()) for n in
+                                                                (32, 48, 64, 96)},
+            "streams": nstreams, "poisson_newton_iterations_in_graph": poisson_iterations,
+            "poisson_failures": fails, "rel_l2_vs_fp64_solo": errs}
+
+
+def long_run(kf, n, steps, every=100):
+    """Mixed and FP64 runs of the bench workload side by side; per-field
+    rel-L2 of mixed against FP64 every `every` steps."""
+    ctx = kf.Context(0)
+    axes = kf.cube(n, L_BOX)
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=-0.3, a=2.0 * L_BOX / n)
+    g = kf.Grid(ctx, axes)
+    fft = kf.FFT(g, kf.KF_FP64)
+    runs = {}
+    for name, P in (("mixed", kf.KF_MIXED), ("fp64", kf.KF_FP64)):
+        ps = kf.Poisson(g, P, tol=1e-10, max_it=60)
+        rhs = kf.Rhs(g, ps, M=1.5, tau=0.1, theta=100.0, sv_on=False, pen_on=True, precision=P)
+        phi = g.rfield()
+        rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+        F = [g.cfield() for _ in range(4)]
+        fft.r2c(ctx.to_device(ln_n), F[3])
+        runs[name] = dict(ps=ps, rhs=rhs, phi=phi, F=F, rk=kf.Rk4(rhs, 1e-3), t=0.0)
+    rows = []
+    done = 0
+    while done < steps:
+        k = min(every, steps - done)
+        for r in runs.values():
+            ctx.sync()
+            t0 = time.perf_counter()
+            r["rk"].step(r["F"], k)
+            ctx.sync()
+            r["t"] += time.perf_counter() - t0
+        done += k
+        a, b = runs["mixed"], runs["fp64"]
+        row = {"step": done}
+        for i, f in enumerate(("FUx", "FUy", "FUz", "ln_n")):
+            row[f] = rel_l2(a["F"][i].get(), b["F"][i].get())
+        row["phi"] = rel_l2(a["phi"].get(), b["phi"].get())
+        rows.append(row)
+        print(json.dumps(row), flush=True)
+    out = {"workload": f"{n}^3 equidistant box, M=1.5, Q=-0.3, dt=1e-3, {steps} RK4 steps",
+           "mixed_steps_per_s": steps / runs["mixed"]["t"], "fp64_steps_per_s": steps / runs["fp64"]["t"],
+           "poisson_failures": {k: r["rhs"].poisson_failures() for k, r in runs.items()},
+           "max_rel_l2": max(max(v for kk, v in r.items() if kk != "step") for r in rows)}
+    print(json.dumps(out), flush=True)
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
@@ -193,8 +315,15 @@ def main():
               "would shard one per GPU with no exchange; a single box is not split across GPUs).",
               file=sys.stderr)
         sys.exit(2)
+    global SMOOTHING
+    if args.smoothing:
+        SMOOTHING = tuple(int(v) for v in args.smoothing.split(","))
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.long:
+        import kielflow_amd as kf
+        long_run(kf, args.n, args.long)
         return
     import kielflow_amd as kf
     n = args.n
@@ -241,6 +370,8 @@ def main():
 
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_cost(kf, ctx, steps=args.adjoint_steps)
+    if not args.no_batch:
+        extra["batch"] = batch_sweep(kf, ctx, steps=args.batch_steps)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, n)
     line = {
         "metric": "RK4 steps/s at 256^3 (FP64 vs mixed, rel-L2<=1e-5); adjoint/forward cost",

@@ -207,6 +207,28 @@ int kf_rk4_time(kf_rk4 k, double *t);
 int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
                double *dt_new);
 
+/* ---------------------------------------------------------------- batched boxes
+ * Parameter sweeps over many independent boxes of uneven sizes (no
+ * counterpart in the reference, which runs one box per process).  Each box
+ * is a complete solver; after kf_batch_capture one RK4 step of every box is
+ * a HIP graph, and kf_batch_steps replays them round-robin over nstreams
+ * streams.  precision: KF_FP64 or KF_MIXED. */
+typedef struct kf_batch_s *kf_batch;
+int kf_batch_create(kf_ctx ctx, int nbox, const kf_axis *axes /* 3 per box */, const kf_parameters *params,
+                    const kf_rhs_switches *sw, int precision, double dt, double poisson_tol, int nstreams,
+                    kf_batch *out);
+int kf_batch_destroy(kf_batch b);
+/* device pointers of a box's state (spectral FP64) and potential (real FP64) */
+int kf_batch_box(kf_batch b, int box, double **FUx, double **FUy, double **FUz, double **Fn, double **phi);
+/* n_d and solid mask of a box (device FP64 real fields, copied) */
+int kf_batch_set_fields(kf_batch b, int box, const double *nd, const double *mask);
+/* record the graphs; Poisson solves inside run exactly poisson_iterations
+ * Newton iterations (unconverged solves are counted as failures) */
+int kf_batch_capture(kf_batch b, int poisson_iterations);
+int kf_batch_steps(kf_batch b, int nsteps);
+int kf_batch_synchronize(kf_batch b);
+int kf_batch_poisson_failures(kf_batch b, int *count);
+
 /* ---------------------------------------------------------------- adjoint (FP64)
  * Discrete adjoint of rhs_standard::solve, of Runge_kutta_O4::solve and of
  * the grain drag J = -sum n_d dV d_x phi (phi of the final state), with

@@ -108,6 +108,15 @@ SIGNATURES = {
     "kf_adjoint_accumulators": (C.c_int, [_vp, _dp, _vp]),
     "kf_adjoint_run": (C.c_int, [_vp, C.POINTER(_vp), C.c_int, C.c_int, _vp, _vp, _dp, _dp, _dp, _vp]),
     "kf_adjoint_spectral_to_real": (C.c_int, [_vp, _vp, _vp]),
+    "kf_batch_create": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_int, C.c_double, C.c_double, C.c_int,
+                                  C.POINTER(_vp)]),
+    "kf_batch_destroy": (C.c_int, [_vp]),
+    "kf_batch_box": (C.c_int, [_vp, C.c_int] + [C.POINTER(_vp)] * 5),
+    "kf_batch_set_fields": (C.c_int, [_vp, C.c_int, _vp, _vp]),
+    "kf_batch_capture": (C.c_int, [_vp, C.c_int]),
+    "kf_batch_steps": (C.c_int, [_vp, C.c_int]),
+    "kf_batch_synchronize": (C.c_int, [_vp]),
+    "kf_batch_poisson_failures": (C.c_int, [_vp, _ip]),
     "kf_adjoint_set_capture_budget": (C.c_int, [_vp, C.c_size_t]),
     "kf_adjoint_run_info": (C.c_int, [_vp, _ip, C.POINTER(C.c_long)]),
     "kf_profile_enable": (C.c_int, [C.c_int]),
@@ -153,7 +162,7 @@ def axis_values(ax: Axis, i: int):
 
 def cell_volumes(axes):
     """dV = prod_b S'_b(xi) L_b / N_b, the quadrature weights of a box."""
-    w = [np.array([axis_values(a, i)[2] * a.L / a.N for i in range(a.N)]) for a in axes]
+    w = [np.array([axis_values(a, i)[2] * a.L / a.n for i in range(a.n)]) for a in axes]
     return w[0][:, None, None] * w[1][None, :, None] * w[2][None, None, :]
 
 
@@ -489,6 +498,67 @@ class Adjoint:
     def spectral_to_real(self, dF, out):
         """out = R2C^T dF: real-space cotangent of the field whose spectrum is F."""
         check(self.lib.kf_adjoint_spectral_to_real(self.h, dF.ptr, out.ptr))
+
+
+class _View:
+    """A device field owned by the library (no free on close)."""
+
+    def __init__(self, ctx, ptr, shape, dtype):
+        self.ctx, self.ptr, self.shape, self.dtype = ctx, ptr, tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+
+    upload = DeviceArray.upload
+    get = DeviceArray.get
+    copy_from = DeviceArray.copy_from
+
+
+class Batch:
+    """Independent boxes advanced together (kf_batch_*)."""
+
+    def __init__(self, ctx: Context, boxes, sv_on=False, pen_on=True, precision=KF_MIXED, dt=1e-3,
+                 poisson_tol=1e-10, nstreams=4):
+        """boxes: list of (axes[3], Parameters)."""
+        self.ctx = ctx
+        n = len(boxes)
+        ax = (Axis * (3 * n))(*[a for axes, _ in boxes for a in axes])
+        pa = (Parameters * n)(*[p for _, p in boxes])
+        sw = Switches(int(sv_on), int(pen_on))
+        h = _vp()
+        check(ctx.lib.kf_batch_create(ctx.h, n, C.cast(ax, _vp), C.cast(pa, _vp), C.byref(sw), precision, dt,
+                                      poisson_tol, nstreams, C.byref(h)))
+        self.h = h
+        self.shapes = [tuple(a.n for a in axes) for axes, _ in boxes]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.ctx.lib.kf_batch_destroy(self.h)
+            self.h = None
+
+    def box(self, b):
+        """(state views FUx, FUy, FUz, Fn; potential view) of box b."""
+        p = [_vp() for _ in range(5)]
+        check(self.ctx.lib.kf_batch_box(self.h, b, *[C.byref(x) for x in p]))
+        nx, ny, nz = self.shapes[b]
+        cs, rs = (nx, ny, nz // 2 + 1), (nx, ny, nz)
+        return [_View(self.ctx, x.value, cs, np.complex128) for x in p[:4]], _View(self.ctx, p[4].value, rs,
+                                                                                    np.float64)
+
+    def set_fields(self, b, nd, mask):
+        check(self.ctx.lib.kf_batch_set_fields(self.h, b, nd.ptr, mask.ptr))
+
+    def capture(self, poisson_iterations=4):
+        check(self.ctx.lib.kf_batch_capture(self.h, poisson_iterations))
+
+    def steps(self, n):
+        check(self.ctx.lib.kf_batch_steps(self.h, n))
+
+    def sync(self):
+        check(self.ctx.lib.kf_batch_synchronize(self.h))
+
+    def poisson_failures(self):
+        c = C.c_int(0)
+        check(self.ctx.lib.kf_batch_poisson_failures(self.h, C.byref(c)))
+        return c.value
 
 
 PROF_FFT_X, PROF_FFT_Y, PROF_FFT_ZR2C, PROF_FFT_ZC2R, PROF_POISSON = 0, 1, 2, 3, 4

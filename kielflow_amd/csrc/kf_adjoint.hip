@@ -317,9 +317,9 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
         GridView<double> gv = g_.view<double>();
         StridedAddr ax = addr_x(p_.nx, p_.ny, p_.nzc), ay = addr_y(p_.nx, p_.ny, p_.nzc);
         launch_strided<double, +1>(p_.nx, (long)p_.ny * p_.nzc, p_.tw_x, ld, StorePlain<double>{p_.scratch, ax}, s_,
-                                   prof::FFT_X);
+                                   prof::FFT_X, p_.vx);
         launch_strided<double, +1>(p_.ny, (long)p_.nx * p_.nzc, p_.tw_y, LoadPlain<double>{p_.scratch, ay},
-                                   StorePlain<double>{p_.scratch, ay}, s_, prof::FFT_Y);
+                                   StorePlain<double>{p_.scratch, ay}, s_, prof::FFT_Y, p_.vy);
         launch_zc2r<double>(p_.nz / 2, (long)p_.nx * p_.ny, p_.tw_m, p_.tw_n, RowLoadC<double>{p_.scratch, p_.nzc}, z,
                             s_);
         (void)gv;
@@ -344,9 +344,9 @@ static void c2r_w(Grid &g, Plan<double> &p, const double2 *src, int d23, int fil
     z.o0 = out;
     z.scale = scale;
     StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
-    launch_strided<double, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, ld, StorePlain<double>{p.scratch, ax}, s, prof::FFT_X);
+    launch_strided<double, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, ld, StorePlain<double>{p.scratch, ax}, s, prof::FFT_X, p.vx);
     launch_strided<double, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<double>{p.scratch, ay},
-                               StorePlain<double>{p.scratch, ay}, s, prof::FFT_Y);
+                               StorePlain<double>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
     launch_zc2r<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<double>{p.scratch, p.nzc}, z, s);
 }
 
@@ -368,8 +368,8 @@ static void r2c_adj(Grid &g, Plan<double> &p, const double *a0, const double *a1
         launch_zr2c<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<double, double>{a0, p.nz},
                             RowStoreC<double>{p.scratch, p.nzc}, s);
     launch_strided<double, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<double>{p.scratch, ay},
-                               StorePlain<double>{p.scratch, ay}, s, prof::FFT_Y);
-    launch_strided<double, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<double>{p.scratch, ax}, x, s, prof::FFT_X);
+                               StorePlain<double>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
+    launch_strided<double, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<double>{p.scratch, ax}, x, s, prof::FFT_X, p.vx);
 }
 
 // one RHS evaluation transposed, at the forward intermediates in c
@@ -584,12 +584,13 @@ static size_t step_capture_bytes(const Grid &g)
 
 bool Adjoint::ensure_pool(int steps)
 {
+    if ((size_t)steps * step_capture_bytes(g_) > capture_budget_) return false;
     if ((int)pool_.size() >= 4 * steps) return true;
     size_t fr = 0, tot = 0;
     KF_HIP(hipMemGetInfo(&fr, &tot));
     const size_t have = pool_.size() / 4 * step_capture_bytes(g_);
     const size_t need = (size_t)steps * step_capture_bytes(g_) - have;
-    if (need > fr / 2 || (size_t)steps * step_capture_bytes(g_) > capture_budget_) return false;
+    if (need > fr / 2) return false;
     const long nr = g_.nr, nc = g_.nc;
     while ((int)pool_.size() < 4 * steps) {
         Capture c{};

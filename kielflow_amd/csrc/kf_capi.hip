@@ -8,6 +8,7 @@
 #include <string>
 
 #include "kf_adjoint.h"
+#include "kf_batch.h"
 #include "kf_effects.h"
 #include "kf_fft.h"
 #include "kf_grid.h"
@@ -880,6 +881,105 @@ int kf_adjoint_spectral_to_real(kf_adjoint a, const double *dJdF, double *dJdf)
         KF_REQUIRE(a && dJdF && dJdf, "null argument");
         a->a->spectral_to_real_T((const double2 *)dJdF, dJdf);
         KF_HIP(hipStreamSynchronize(a->rhs->grid->ctx->stream));
+        return KF_OK;
+    });
+}
+
+// ---------------------------------------------------------------- batch
+struct kf_batch_s {
+    kf_ctx ctx;
+    Batch *b;
+};
+
+int kf_batch_create(kf_ctx ctx, int nbox, const kf_axis *axes, const kf_parameters *params, const kf_rhs_switches *sw,
+                    int precision, double dt, double poisson_tol, int nstreams, kf_batch *out)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx && axes && params && sw && out && nbox > 0, "bad argument");
+        KF_REQUIRE(precision == KF_FP64 || precision == KF_MIXED, "precision must be KF_FP64 or KF_MIXED");
+        KF_REQUIRE(dt > 0 && poisson_tol > 0, "dt and tolerance must be positive");
+        set_dev(ctx);
+        std::vector<BatchBox> boxes((size_t)nbox);
+        for (int b = 0; b < nbox; ++b) {
+            for (int d = 0; d < 3; ++d) {
+                boxes[b].ax[d] = to_desc(axes[3 * b + d]);
+                KF_REQUIRE(boxes[b].ax[d].type >= 1 && boxes[b].ax[d].type <= 3, "unknown axis type");
+            }
+            KF_REQUIRE(fft::grid_supported(boxes[b].ax[0].n, boxes[b].ax[1].n, boxes[b].ax[2].n),
+                       "grid extents not supported (2^a 3^b <= 256; Nz even)");
+            boxes[b].pa = Params{params[b].M, params[b].tau, params[b].theta, params[b].mu, params[b].beta};
+        }
+        Switches s{sw->sv_on, sw->pen_on};
+        *out = new kf_batch_s{ctx, new Batch(boxes, s, precision == KF_MIXED ? 1 : 0, dt, poisson_tol,
+                                             nstreams)};
+        return KF_OK;
+    });
+}
+
+int kf_batch_destroy(kf_batch b)
+{
+    return guard([&] {
+        if (!b) return KF_OK;
+        delete b->b;
+        delete b;
+        return KF_OK;
+    });
+}
+
+int kf_batch_box(kf_batch b, int box, double **FUx, double **FUy, double **FUz, double **Fn, double **phi)
+{
+    return guard([&] {
+        KF_REQUIRE(b && box >= 0 && box < b->b->size(), "bad box index");
+        double **o[4] = {FUx, FUy, FUz, Fn};
+        for (int f = 0; f < 4; ++f)
+            if (o[f]) *o[f] = (double *)b->b->state(box, f);
+        if (phi) *phi = b->b->potential(box);
+        return KF_OK;
+    });
+}
+
+int kf_batch_set_fields(kf_batch b, int box, const double *nd, const double *mask)
+{
+    return guard([&] {
+        KF_REQUIRE(b && box >= 0 && box < b->b->size(), "bad box index");
+        b->b->set_fields(box, nd, mask);
+        return KF_OK;
+    });
+}
+
+int kf_batch_capture(kf_batch b, int poisson_iterations)
+{
+    return guard([&] {
+        KF_REQUIRE(b && poisson_iterations > 0 && poisson_iterations <= 64, "bad argument");
+        b->b->capture(poisson_iterations);
+        return KF_OK;
+    });
+}
+
+int kf_batch_steps(kf_batch b, int nsteps)
+{
+    return guard([&] {
+        KF_REQUIRE(b && nsteps >= 0, "bad argument");
+        b->b->steps(nsteps);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_batch_synchronize(kf_batch b)
+{
+    return guard([&] {
+        KF_REQUIRE(b, "null batch");
+        b->b->sync();
+        return KF_OK;
+    });
+}
+
+int kf_batch_poisson_failures(kf_batch b, int *count)
+{
+    return guard([&] {
+        KF_REQUIRE(b && count, "null argument");
+        *count = b->b->poisson_failures();
         return KF_OK;
     });
 }

@@ -223,6 +223,9 @@ template <typename R>
 struct Plan {
     int nx, ny, nz, nzc;
     long nr, nc;
+    // column-block variant of the x and y passes, picked by the autotuner
+    // at plan creation (0: cb_for columns per workgroup, 1: half of them)
+    int vx = 0, vy = 0;
     C2<R> *tw_x = nullptr, *tw_y = nullptr, *tw_m = nullptr, *tw_n = nullptr;
     C2<R> *scratch = nullptr;  // nc complex, for C2R (input is preserved)
     void *owner_pool = nullptr;
@@ -255,17 +258,31 @@ __host__ __device__ constexpr int cb_for(int n)
 
 #define KF_FFT_LENGTHS(X) X(2) X(3) X(4) X(6) X(8) X(12) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256)
 
+// the two column-block variants of a strided pass (results are identical:
+// only the mapping of columns to workgroups differs)
+template <typename R>
+__host__ __device__ constexpr int cb_variant(int n, int v)
+{
+    return (v == 1 && cb_for<R>(n) / 2 >= 4) ? cb_for<R>(n) / 2 : cb_for<R>(n);
+}
+
 template <typename R, int DIR, class LD, class ST_>
-void launch_strided(int n, long Q, const C2<R> *tw, LD ld, ST_ st, hipStream_t s, int fam)
+void launch_strided(int n, long Q, const C2<R> *tw, LD ld, ST_ st, hipStream_t s, int fam, int var = 0)
 {
     prof::Scope ps_(fam, s);
     switch (n) {
 #define KF_CASE(NN)                                                                       \
     case NN: {                                                                            \
-        constexpr int CB = cb_for<R>(NN);                                                 \
-        dim3 grid((unsigned)((Q + CB - 1) / CB));                                         \
-        hipLaunchKernelGGL((k_strided<R, NN, CB, DIR, LD, ST_>), grid, dim3(tpt_for(NN) * CB), \
-                           0, s, tw, Q, ld, st);                                          \
+        constexpr int CB0 = cb_variant<R>(NN, 0), CB1 = cb_variant<R>(NN, 1);             \
+        if (var == 1 && CB1 != CB0) {                                                     \
+            dim3 grid((unsigned)((Q + CB1 - 1) / CB1));                                   \
+            hipLaunchKernelGGL((k_strided<R, NN, CB1, DIR, LD, ST_>), grid,               \
+                               dim3(tpt_for(NN) * CB1), 0, s, tw, Q, ld, st);             \
+        } else {                                                                          \
+            dim3 grid((unsigned)((Q + CB0 - 1) / CB0));                                   \
+            hipLaunchKernelGGL((k_strided<R, NN, CB0, DIR, LD, ST_>), grid,               \
+                               dim3(tpt_for(NN) * CB0), 0, s, tw, Q, ld, st);             \
+        }                                                                                 \
         break;                                                                            \
     }
         KF_FFT_LENGTHS(KF_CASE)

@@ -3,6 +3,9 @@
 // scaled by 1/N; the caller's input is preserved in both directions).
 #include "kf_fft.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 #include <cmath>
 
 namespace kf {
@@ -44,6 +47,46 @@ static C2<R> *twiddles(int n, int count, hipStream_t s)
     return d;
 }
 
+// Column-block autotuner: time both variants of the x and y passes on the
+// plan's scratch field (median of 3 after a warm-up launch) and keep the
+// faster.  KF_FFT_AUTOTUNE=0 keeps variant 0.  The variants compute the same
+// numbers, so the choice only changes speed.
+template <typename R>
+static void autotune(Plan<R> &p, hipStream_t s)
+{
+    p.vx = p.vy = 0;
+    const char *env = std::getenv("KF_FFT_AUTOTUNE");
+    if (env && env[0] == '0') return;
+    if (cb_variant<R>(p.nx, 1) == cb_variant<R>(p.nx, 0) && cb_variant<R>(p.ny, 1) == cb_variant<R>(p.ny, 0))
+        return;
+    KF_HIP(hipMemsetAsync(p.scratch, 0, sizeof(C2<R>) * p.nc, s));
+    hipEvent_t e0, e1;
+    KF_HIP(hipEventCreate(&e0));
+    KF_HIP(hipEventCreate(&e1));
+    StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
+    auto time_pass = [&](int pass, int var) {
+        float ms[3];
+        for (int r = -1; r < 3; ++r) {
+            KF_HIP(hipEventRecord(e0, s));
+            if (pass == 0)
+                launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax},
+                                      StorePlain<R>{p.scratch, ax}, s, 1000, var);
+            else
+                launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
+                                      StorePlain<R>{p.scratch, ay}, s, 1000, var);
+            KF_HIP(hipEventRecord(e1, s));
+            KF_HIP(hipEventSynchronize(e1));
+            if (r >= 0) KF_HIP(hipEventElapsedTime(&ms[r], e0, e1));
+        }
+        std::sort(ms, ms + 3);
+        return ms[1];
+    };
+    if (cb_variant<R>(p.nx, 1) != cb_variant<R>(p.nx, 0)) p.vx = time_pass(0, 1) < time_pass(0, 0) ? 1 : 0;
+    if (cb_variant<R>(p.ny, 1) != cb_variant<R>(p.ny, 0)) p.vy = time_pass(1, 1) < time_pass(1, 0) ? 1 : 0;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
 template <typename R>
 void plan_init(Plan<R> &p, int nx, int ny, int nz, hipStream_t s)
 {
@@ -61,6 +104,7 @@ void plan_init(Plan<R> &p, int nx, int ny, int nz, hipStream_t s)
     p.tw_m = twiddles<R>(nz / 2, nz / 2, s);
     p.tw_n = twiddles<R>(nz, nz / 2 + 1, s);
     KF_HIP(hipMalloc(&p.scratch, sizeof(C2<R>) * p.nc));
+    autotune(p, s);
 }
 
 template <typename R>
@@ -82,9 +126,9 @@ void r2c(const Plan<R> &p, const R *in, C2<R> *out, hipStream_t s)
                    RowStoreC<R>{out, p.nzc}, s);
     StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
     launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{out, ay},
-                          StorePlain<R>{out, ay}, s, prof::FFT_Y);
+                          StorePlain<R>{out, ay}, s, prof::FFT_Y, p.vy);
     launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{out, ax},
-                          StorePlain<R>{out, ax}, s, prof::FFT_X);
+                          StorePlain<R>{out, ax}, s, prof::FFT_X, p.vx);
 }
 
 template <typename R>
@@ -92,9 +136,9 @@ void c2r(const Plan<R> &p, const C2<R> *in, R *out, hipStream_t s)
 {
     StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
     launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{in, ax},
-                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X);
+                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X, p.vx);
     launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
     const long rows = (long)p.nx * p.ny;
     launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc},
                    RowStoreReal<R>{out, p.nz, (R)(1.0 / (double)p.nr)}, s);

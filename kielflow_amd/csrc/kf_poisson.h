@@ -76,6 +76,11 @@ class Poisson {
     void set_max_iterations(int m) { max_it_ = m; }
     void set_smoothing(int pre, int post, int coarse) { nu1_ = pre; nu2_ = post; nuc_ = coarse; }
     void set_krylov(bool on, int iters) { krylov_ = on; kiters_ = iters; }
+    // k > 0: every Newton-MG solve enqueues exactly k iterations with no
+    // host synchronisation, so a stream capture can record it; iterations
+    // after convergence return at once on the device, and a solve still
+    // unconverged after k counts as a failure.  k = 0: host-driven loop.
+    void set_graph_iterations(int k) { graph_it_ = k; }
     int mode() const { return mode_; }
     double tolerance() const { return tol_; }
     int max_iterations() const { return max_it_; }
@@ -151,6 +156,7 @@ class Poisson {
     double tol_ = 1e-10;
     int max_it_ = 100;
     int nu1_ = 2, nu2_ = 2, nuc_ = 24, inner_ = 1;
+    int graph_it_ = 0;
     std::vector<Level> lv_;
     // device control block: [0] done flag, [1] cycles, [2] fail counter
     int *d_ctl_ = nullptr;

@@ -136,6 +136,9 @@ constexpr unsigned RED_MAX_BLOCKS = 2048;
 // ======================================================================== Newton-MG kernels
 // F = L_h phi - exp(phi) + rho (FP64); b = -F and e = exp(phi) in the sweep
 // precision; per-block max|F| and max|rho|.
+// Each thread takes NR_PTS points spaced RED_BLOCK apart inside its block's
+// contiguous chunk, so the loads of all its points are in flight together.
+constexpr int NR_PTS = 4;
 template <typename RIN, typename R>
 __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *__restrict__ rho,
@@ -145,19 +148,34 @@ k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *_
     if (ctl[0]) return;
     double fmaxv = 0., rmax = 0.;
     const int nz = L.n[2], ny = L.n[1];
-    for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < L.N; p += (long)gridDim.x * RED_BLOCK) {
-        int i, j, k;
-        dec3(p, ny, nz, i, j, k);
-        double A, D;
-        stencil<double>(L, phi, i, j, k, A, D);
-        const double x = phi[p];
-        const double ex = exp(x);
-        const double rh = (double)rho[p];
-        const double F = A - D * x - ex + rh;
-        b[p] = (R)(-F);
-        e[p] = (R)ex;
-        fmaxv = fmax(fmaxv, fabs(F));
-        rmax = fmax(rmax, fabs(rh));
+    const long p0 = (long)blockIdx.x * (RED_BLOCK * NR_PTS) + threadIdx.x;
+    double F[NR_PTS], ex[NR_PTS], rh[NR_PTS];
+#pragma unroll
+    for (int u = 0; u < NR_PTS; ++u) {
+        const long p = p0 + (long)u * RED_BLOCK;
+        F[u] = 0.;
+        ex[u] = 0.;
+        rh[u] = 0.;
+        if (p < L.N) {
+            int i, j, k;
+            dec3(p, ny, nz, i, j, k);
+            double A, D;
+            stencil<double>(L, phi, i, j, k, A, D);
+            const double x = phi[p];
+            ex[u] = exp(x);
+            rh[u] = (double)rho[p];
+            F[u] = A - D * x - ex[u] + rh[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NR_PTS; ++u) {
+        const long p = p0 + (long)u * RED_BLOCK;
+        if (p < L.N) {
+            b[p] = (R)(-F[u]);
+            e[p] = (R)ex[u];
+            fmaxv = fmax(fmaxv, fabs(F[u]));
+            rmax = fmax(rmax, fabs(rh[u]));
+        }
     }
     fmaxv = block_max(fmaxv, sh);
     rmax = block_max(rmax, sh);
@@ -906,14 +924,15 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
 {
     Level &L0 = lv_[0];
     const long N = L0.N;
-    unsigned nbr = nblocks(N, RED_BLOCK);
-    if (nbr > RED_MAX_BLOCKS) nbr = RED_MAX_BLOCKS;
+    const unsigned nbr = nblocks(N, RED_BLOCK * NR_PTS);
     LevelView<double> v64 = view<double>(0);
     R *x0, *b0, *e0;
     if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
     else { x0 = L0.x32; b0 = L0.b32; e0 = L0.e32; }
     const int *ctl = d_ctl_;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
+    const bool graph = graph_it_ > 0;
+    const int max_it = graph ? graph_it_ : max_it_;
     auto check = [&](int it) {
         if (rho64)
             hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
@@ -922,7 +941,8 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
             hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
                                (const double *)phi, rho32, b0, e0, d_red_, ctl);
         hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr,
-                           it, max_it_, tol_, d_ctl_, d_fail_, d_stat_);
+                           it, max_it, tol_, d_ctl_, d_fail_, d_stat_);
+        if (graph) return;
         KF_HIP(hipMemcpyAsync(h_ctl_ + 4 * (it & 1), d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
         KF_HIP(hipEventRecord(ev_[it & 1], s));
     };
@@ -937,9 +957,10 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
         hipLaunchKernelGGL(k_update<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const R *)x0, N, ctl);
     };
     check(0);
-    for (int it = 0; it < max_it_; ++it) {
+    for (int it = 0; it < max_it; ++it) {
         correct();
         check(it + 1);
+        if (graph) continue;
         KF_HIP(hipEventSynchronize(ev_[it & 1]));
         if (h_ctl_[4 * (it & 1)]) break;
     }

@@ -58,6 +58,9 @@ class Rhs {
     void set_capture(Capture *c) { cap_ = c; }
     // evaluations at RK stage s store their intermediates into c4[s] while set
     void set_capture_stages(Capture *c4) { cap_stages_ = c4; }
+    // graph-safe history: rotate the per-stage potentials by copying
+    // (fixed buffers) rather than by swapping pointers on the host
+    void set_graph_safe(bool v) { graph_safe_ = v; }
     // forget the per-stage potential history (the run jumped in time)
     void reset_history()
     {
@@ -105,6 +108,7 @@ class Rhs {
     int stage_ = -1;
     Capture *cap_ = nullptr;
     Capture *cap_stages_ = nullptr;
+    bool graph_safe_ = false;
     Capture *active_capture() const
     {
         if (cap_) return cap_;

@@ -209,6 +209,17 @@ __global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h2
     phi[i] = order == 2 ? 3. * (h2[i] - h1[i]) + h0[i] : 2. * h2[i] - h1[i];
 }
 
+// h0 <- h1 <- h2 <- phi
+__global__ void k_shift3(double *__restrict__ h0, double *__restrict__ h1, double *__restrict__ h2,
+                         const double *__restrict__ phi, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    h0[i] = h1[i];
+    h1[i] = h2[i];
+    h2[i] = phi[i];
+}
+
 template <typename R>
 void Rhs<R>::set_stage(int s)
 {
@@ -239,11 +250,15 @@ void Rhs<R>::poisson_step(C2<R> *Fn)
     else ps_.solve(phi_, nullptr, rho_, s_, false);
     if (stage_ >= 0) {
         double **h = ph_[stage_];
-        double *oldest = h[0];
-        h[0] = h[1];
-        h[1] = h[2];
-        h[2] = oldest;
-        KF_HIP(hipMemcpyAsync(h[2], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        if (graph_safe_) {
+            hipLaunchKernelGGL(k_shift3, dim3(nbk(nr)), dim3(256), 0, s_, h[0], h[1], h[2], (const double *)phi_, nr);
+        } else {
+            double *oldest = h[0];
+            h[0] = h[1];
+            h[1] = h[2];
+            h[2] = oldest;
+            KF_HIP(hipMemcpyAsync(h[2], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        }
         hc_[stage_]++;
     }
 }

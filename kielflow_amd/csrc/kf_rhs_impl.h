@@ -289,9 +289,9 @@ inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
     GridView<R> gv = g.view<R>();
     StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
     launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
-                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X);
+                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X, p.vx);
     launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
     launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc}, st, s);
 }
 
@@ -302,8 +302,8 @@ inline void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, h
     launch_zr2c<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<R, RS>{src, p.nz},
                    RowStoreC<R>{p.scratch, p.nzc}, s);
     launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y);
-    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X);
+                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
+    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X, p.vx);
 }
 
 template <typename R>

@@ -147,7 +147,12 @@ class grid {
         : x_axis(Ax.clone()), y_axis(Ay.clone()), z_axis(Az.clone()), Nx(x_axis->N), Ny(y_axis->N), Nz(z_axis->N)
     {
     }
-    grid(const grid &that) : grid(*that.x_axis, *that.y_axis, *that.z_axis) { h_ = that.h_; }
+    // copies share one device grid (created here if it does not exist yet)
+    grid(const grid &that) : grid(*that.x_axis, *that.y_axis, *that.z_axis)
+    {
+        that.handle();
+        h_ = that.h_;
+    }
     ~grid()
     {
         delete x_axis;

@@ -268,25 +268,47 @@ size_t or_grid_ncplx(const or_grid *g) { return g->nc; }
 static void fft3_full(or_grid *g, int sign)
 {
     const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+    const int mx = nx > ny ? (nx > nz ? nx : nz) : (ny > nz ? ny : nz);
     cplx *a = g->full;
-    for (int i = 0; i < nx; ++i)
-        for (int j = 0; j < ny; ++j) {
-            cplx *row = a + ((size_t)i * ny + j) * nz;
-            fft_line(g->pl[2], row, g->line2, sign);
-        }
-    for (int i = 0; i < nx; ++i)
-        for (int k = 0; k < nz; ++k) {
-            for (int j = 0; j < ny; ++j) g->line[j] = a[((size_t)i * ny + j) * nz + k];
-            fft_line(g->pl[1], g->line, g->line2, sign);
-            for (int j = 0; j < ny; ++j) a[((size_t)i * ny + j) * nz + k] = g->line[j];
-        }
-    for (int j = 0; j < ny; ++j)
-        for (int k = 0; k < nz; ++k) {
-            for (int i = 0; i < nx; ++i) g->line[i] = a[((size_t)i * ny + j) * nz + k];
-            fft_line(g->pl[0], g->line, g->line2, sign);
-            for (int i = 0; i < nx; ++i) a[((size_t)i * ny + j) * nz + k] = g->line[i];
-        }
+    /* independent lines: threads share nothing but the plans (read only) */
+#pragma omp parallel
+    {
+        cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)mx);
+        cplx *line2 = (cplx *)malloc(sizeof(cplx) * (size_t)mx);
+#pragma omp for collapse(2)
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j) {
+                cplx *row = a + ((size_t)i * ny + j) * nz;
+                fft_line(g->pl[2], row, line2, sign);
+            }
+#pragma omp for collapse(2)
+        for (int i = 0; i < nx; ++i)
+            for (int k = 0; k < nz; ++k) {
+                for (int j = 0; j < ny; ++j) line[j] = a[((size_t)i * ny + j) * nz + k];
+                fft_line(g->pl[1], line, line2, sign);
+                for (int j = 0; j < ny; ++j) a[((size_t)i * ny + j) * nz + k] = line[j];
+            }
+#pragma omp for collapse(2)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                for (int i = 0; i < nx; ++i) line[i] = a[((size_t)i * ny + j) * nz + k];
+                fft_line(g->pl[0], line, line2, sign);
+                for (int i = 0; i < nx; ++i) a[((size_t)i * ny + j) * nz + k] = line[i];
+            }
+        free(line);
+        free(line2);
+    }
 }
+
+/* host threads of the oracle (1 = the reference's single thread) */
+#ifdef _OPENMP
+#include <omp.h>
+void or_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int or_get_threads(void) { return omp_get_max_threads(); }
+#else
+void or_set_threads(int n) { (void)n; }
+int or_get_threads(void) { return 1; }
+#endif
 
 /* OP_FFT::operator() (OP_FFT.cpp:49-63): unnormalised R2C */
 void or_fft_r2c(const or_grid *gc, const double *in, double *out)
@@ -777,6 +799,7 @@ static void fas_smooth(fas_level *lv, int sweeps)
     const int nx = lv->n[0], ny = lv->n[1], nz = lv->n[2];
     for (int s = 0; s < sweeps; ++s)
         for (int color = 0; color < 2; ++color)
+#pragma omp parallel for collapse(2)
             for (int i = 0; i < nx; ++i)
                 for (int j = 0; j < ny; ++j)
                     for (int k = (i + j + color) & 1; k < nz; k += 2) {
@@ -796,6 +819,7 @@ static void fas_smooth(fas_level *lv, int sweeps)
 static void fas_residual(const fas_level *lv, double *r)
 {
     const int nx = lv->n[0], ny = lv->n[1], nz = lv->n[2];
+#pragma omp parallel for
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -809,6 +833,7 @@ static void fas_residual(const fas_level *lv, double *r)
 static void fas_apply_N(const fas_level *lv, const double *phi, double *out)
 {
     const int nx = lv->n[0], ny = lv->n[1], nz = lv->n[2];
+#pragma omp parallel for
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -824,6 +849,7 @@ static void restrict_fw(const double *in, const int *nf, double *out)
 {
     const int nx = nf[0] / 2, ny = nf[1] / 2, nz = nf[2] / 2;
     static const double w[3] = {0.25, 0.5, 0.25};
+#pragma omp parallel for
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -846,6 +872,7 @@ static void restrict_fw(const double *in, const int *nf, double *out)
 static void prolong_add(const double *ec, const int *nc, double *fine)
 {
     const int nx = 2 * nc[0], ny = 2 * nc[1], nz = 2 * nc[2];
+#pragma omp parallel for
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {
@@ -919,6 +946,7 @@ double or_poisson_residual(const or_grid *g, const double *phi, const double *rh
     spacing_make(&lv.sp, g->ax);
     const int nx = lv.n[0], ny = lv.n[1], nz = lv.n[2];
     double rmax = 0., qmax = 0.;
+#pragma omp parallel for reduction(max : rmax, qmax)
     for (int i = 0; i < nx; ++i)
         for (int j = 0; j < ny; ++j)
             for (int k = 0; k < nz; ++k) {

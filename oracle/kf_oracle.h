@@ -35,6 +35,10 @@ typedef struct {
 
 typedef struct or_grid_s or_grid;
 
+/* host threads (OpenMP) of the oracle; 1 is the reference's single thread */
+void or_set_threads(int n);
+int or_get_threads(void);
+
 /* ---- axes ------------------------------------------------------------- */
 double or_axis_val_at(const or_axis *a, int index);
 double or_axis_k_val_at(const or_axis *a, int index);

@@ -96,6 +96,8 @@ def lib():
         L.or_rhs_poisson_status.argtypes = [vp, dp, C.POINTER(C.c_int)]
         L.or_rk4_step.restype = C.c_int
         L.or_rk4_step.argtypes = [vp, C.c_double, dp, vp, vp, vp, vp]
+        L.or_set_threads.argtypes = [C.c_int]
+        L.or_get_threads.restype = C.c_int
         L.or_mg_inject.argtypes = [vp, vp, vp, C.c_int]
         L.or_mg_prolong.argtypes = [vp, vp, vp, C.POINTER(Axis)]
         L.or_rhs_eval_tl.restype = C.c_int
@@ -331,6 +333,14 @@ class Rhs:
         tt = C.c_double(t)
         st = lib().or_rk4_step(self.h, dt, C.byref(tt), *[_p(a) for a in state])
         return tt.value, st
+
+
+def set_threads(n: int):
+    lib().or_set_threads(n)
+
+
+def get_threads() -> int:
+    return lib().or_get_threads()
 
 
 def mg_inject(a, f):

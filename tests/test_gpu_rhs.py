@@ -144,3 +144,67 @@ def test_cfl_matches_oracle(ctx, oracle):
     g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
     d = [ctx.to_device(s) for s in state[:3]]
     assert g.cfl(0.01, *d) == pytest.approx(og.cfl(0.01, *state[:3]), rel=1e-13)
+
+
+def test_config0_full_run_matches_oracle(ctx, oracle):
+    """32^3 EqSt, M = 1, Q = -0.1, u = 0, seed-0 noise, SV off, penalization
+    on, dt = 1e-3: the full 100 steps, FP64, checked every 25 steps."""
+    import kielflow_amd as kf
+    print("config0 100 steps", flush=True)
+    state, nd, mask, og = initial_state("eqst", 32, seed=0, Q=-0.1)
+    g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+    ps = kf.Poisson(g, kf.KF_FP64, tol=1e-13, max_it=80)
+    rhs = kf.Rhs(g, ps, M=1.0, tau=0.1, theta=100.0, sv_on=False, pen_on=True)
+    phi = g.rfield()
+    rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+    orhs = oracle.Rhs(og, M=1.0, tau=0.1, theta=100.0, sv_on=False, pen_on=True,
+                      poisson_mode=oracle.POISSON_FAS, poisson_tol=1e-14, poisson_max=400)
+    orhs.set_fields(np.zeros(og.rshape), nd, mask)
+    dF = [ctx.to_device(s) for s in state]
+    ost = [s.copy() for s in state]
+    rk = kf.Rk4(rhs, 1e-3)
+    t = 0.0
+    for block in range(4):
+        rk.step(dF, 25)
+        for _ in range(25):
+            t, s = orhs.rk4(1e-3, t, ost)
+            assert s == 0
+        for a in range(4):
+            e = rel_l2(dF[a].get(), ost[a])
+            assert e < FP64_TOL, (25 * (block + 1), a, e)
+        assert rel_l2(phi.get(), orhs.phi()) < FP64_TOL
+    assert rhs.poisson_failures() == 0
+
+
+def test_config1_sist_with_cfl_dt_matches_oracle(ctx, oracle):
+    """Sine-stretched axes (S'(0) = 1/3), M = 1.2, Q = -0.2, spectral
+    viscosity on, dt from cfl_all every step (CFL.cpp:29-82), FP64: three
+    steps at 64^3 against the oracle."""
+    import kielflow_amd as kf
+    print("config1 sist 64", flush=True)
+    state, nd, mask, og = initial_state("sist", 64, seed=0, Q=-0.2)
+    state = _perturb(og, state)
+    g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+    ps = kf.Poisson(g, kf.KF_FP64, tol=1e-13, max_it=80)
+    rhs = kf.Rhs(g, ps, M=1.2, tau=0.1, theta=100.0, sv_on=True, pen_on=True)
+    phi = g.rfield()
+    rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+    orhs = oracle.Rhs(og, M=1.2, tau=0.1, theta=100.0, sv_on=True, pen_on=True,
+                      poisson_mode=oracle.POISSON_FAS, poisson_tol=1e-14, poisson_max=400)
+    orhs.set_fields(np.zeros(og.rshape), nd, mask)
+    dF = [ctx.to_device(s) for s in state]
+    ost = [s.copy() for s in state]
+    rk = kf.Rk4(rhs, 1e-3)
+    dt, dt_o, t = 1e-3, 1e-3, 0.0
+    for step in range(3):
+        dt = g.cfl(dt, dF[0], dF[1], dF[2])
+        dt_o = og.cfl(dt_o, ost[0], ost[1], ost[2])
+        assert dt == pytest.approx(dt_o, rel=1e-12)
+        rk.set_dt(dt)
+        rk.step(dF, 1)
+        t, s = orhs.rk4(dt_o, t, ost)
+        assert s == 0
+    for a in range(4):
+        e = rel_l2(dF[a].get(), ost[a])
+        assert e < FP64_TOL, (a, e)
+    assert rel_l2(phi.get(), orhs.phi()) < FP64_TOL
