@@ -159,7 +159,8 @@ def adjoint_cost(kf, ctx, n=64, steps=200, k=20, m=3.0):
     fft.r2c(ctx.to_device(ln_n), Fn0)
     dV = ctx.to_device(kf.api.cell_volumes(axes))
     dG = ctx.to_device(nd / Q)
-    adj = kf.Adjoint(rhs, 1e-3)
+    # transposed Poisson solves to the forward solves' tolerance
+    adj = kf.Adjoint(rhs, 1e-3, tol=1e-10)
 
     def fresh():
         phi.upload(np.zeros(g.rshape))

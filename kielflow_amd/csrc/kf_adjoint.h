@@ -74,6 +74,8 @@ class Adjoint {
     bool used_pool_ = false;
     double *lam_[4] = {};            // transposed Poisson solutions per stage (warm start)
     bool lam_valid_ = false;
+    double *lamh_[4][3] = {};        // per stage: the last three solutions, newest last
+    int lam_count_[4] = {0, 0, 0, 0};
     long lin_its_ = 0;
     double2 *Fb_[4] = {}, *linb_[4] = {}, *F1b_ = nullptr, *Yt_ = nullptr, *kb_[4][4] = {};
     double *advb_[4] = {}, *Ub_[3] = {}, *Lb_ = nullptr, *rt_ = nullptr, *Pb_ = nullptr;

@@ -204,6 +204,26 @@ __global__ void k_mul3(double *__restrict__ out, const double *__restrict__ a, c
     if (i < n) out[i] = s * a[i] * b[i] * (c ? c[i] : 1.);
 }
 
+// x = 3 (h2 - h1) + h0 (order 2) or 2 h2 - h1 (order 1)
+__global__ void k_lam_extrap(double *__restrict__ x, const double *__restrict__ h2, const double *__restrict__ h1,
+                             const double *__restrict__ h0, int order, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    x[i] = order == 2 ? 3. * (h2[i] - h1[i]) + h0[i] : 2. * h2[i] - h1[i];
+}
+
+// h0 <- h1 <- h2 <- x
+__global__ void k_lam_shift(double *__restrict__ h0, double *__restrict__ h1, double *__restrict__ h2,
+                            const double *__restrict__ x, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    h0[i] = h1[i];
+    h1[i] = h2[i];
+    h2[i] = x[i];
+}
+
 // ------------------------------------------------------------------ class
 template <typename T>
 static T *dalloc(long n)
@@ -236,6 +256,8 @@ Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rh
     for (auto &st : kb_)
         for (auto &f : st) f = dalloc<double2>(nc);
     for (auto &l : lam_) l = dalloc<double>(nr);
+    for (auto &st : lamh_)
+        for (auto &l : st) l = dalloc<double>(nr);
     rk_ = new Rk4<double>(rhs_, dt_);
     for (int a = 0; a < 4; ++a) {
         Fb_[a] = dalloc<double2>(nc);
@@ -271,6 +293,8 @@ Adjoint::~Adjoint()
     for (auto &st : kb_)
         for (auto &f : st) (void)hipFree(f);
     for (auto &l : lam_) (void)hipFree(l);
+    for (auto &st : lamh_)
+        for (auto &l : st) (void)hipFree(l);
     for (auto &c : pool_) free_capture(c);
     delete rk_;
     for (int a = 0; a < 4; ++a) {
@@ -287,6 +311,7 @@ Adjoint::~Adjoint()
 void Adjoint::reset_accumulators()
 {
     lam_valid_ = false;
+    for (int &c : lam_count_) c = 0;
     KF_HIP(hipMemsetAsync(Mb_, 0, sizeof(double) * 4, s_));
     KF_HIP(hipMemsetAsync(ndb_, 0, sizeof(double) * g_.nr, s_));
 }
@@ -315,12 +340,12 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
     ld.w = 1;
     {
         GridView<double> gv = g_.view<double>();
-        StridedAddr ax = addr_x(p_.nx, p_.ny, p_.nzc), ay = addr_y(p_.nx, p_.ny, p_.nzc);
+        const StridedAddr ax = saddr_x(p_), ay = saddr_y(p_);
         launch_strided<double, +1>(p_.nx, (long)p_.ny * p_.nzc, p_.tw_x, ld, StorePlain<double>{p_.scratch, ax}, s_,
                                    prof::FFT_X, p_.vx);
-        launch_strided<double, +1>(p_.ny, (long)p_.nx * p_.nzc, p_.tw_y, LoadPlain<double>{p_.scratch, ay},
-                                   StorePlain<double>{p_.scratch, ay}, s_, prof::FFT_Y, p_.vy);
-        launch_zc2r<double>(p_.nz / 2, (long)p_.nx * p_.ny, p_.tw_m, p_.tw_n, RowLoadC<double>{p_.scratch, p_.nzc}, z,
+        launch_strided<double, +1>(p_.ny, scratch_cols_y(p_), p_.tw_y, LoadMasked<double>{p_.scratch, ay, (unsigned)p_.nzc},
+                               StoreMasked<double>{p_.scratch, ay, (unsigned)p_.nzc}, s_, prof::FFT_Y, p_.vy);
+        launch_zc2r<double>(p_.nz / 2, (long)p_.nx * p_.ny, p_.tw_m, p_.tw_n, RowLoadC<double>{p_.scratch, p_.nzs}, z,
                             s_);
         (void)gv;
     }
@@ -343,11 +368,11 @@ static void c2r_w(Grid &g, Plan<double> &p, const double2 *src, int d23, int fil
     ZStore<double> z = zs<double>(ZK_WRITE, p);
     z.o0 = out;
     z.scale = scale;
-    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
     launch_strided<double, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, ld, StorePlain<double>{p.scratch, ax}, s, prof::FFT_X, p.vx);
-    launch_strided<double, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<double>{p.scratch, ay},
-                               StorePlain<double>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
-    launch_zc2r<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<double>{p.scratch, p.nzc}, z, s);
+    launch_strided<double, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<double>{p.scratch, ay, (unsigned)p.nzc},
+                               StoreMasked<double>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
+    launch_zc2r<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<double>{p.scratch, p.nzs}, z, s);
 }
 
 // o (+)= coef [D23] [-ik_dir] W'/N R2C(a0 [* a1])
@@ -360,15 +385,15 @@ static void r2c_adj(Grid &g, Plan<double> &p, const double *a0, const double *a1
     x.dir = dir;
     x.acc = acc;
     x.coef = coef;
-    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
     if (a1)
         launch_zr2c<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadProd<double>{a0, a1, p.nz},
-                            RowStoreC<double>{p.scratch, p.nzc}, s);
+                            RowStoreC<double>{p.scratch, p.nzs}, s);
     else
         launch_zr2c<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<double, double>{a0, p.nz},
-                            RowStoreC<double>{p.scratch, p.nzc}, s);
-    launch_strided<double, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<double>{p.scratch, ay},
-                               StorePlain<double>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
+                            RowStoreC<double>{p.scratch, p.nzs}, s);
+    launch_strided<double, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<double>{p.scratch, ay, (unsigned)p.nzc},
+                               StoreMasked<double>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
     launch_strided<double, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<double>{p.scratch, ax}, x, s, prof::FFT_X, p.vx);
 }
 
@@ -491,9 +516,19 @@ void Adjoint::reverse_step(const Capture caps[4], double2 *yb[4])
     const double back[4] = {0., 0.5 * dt_, 0.5 * dt_, 1.0 * dt_};  // S_st = y0 + back[st] k_{st-1}
     for (int st = 3; st >= 0; --st) {
         const double2 *ob[4] = {kb_[st][0], kb_[st][1], kb_[st][2], kb_[st][3]};
+        // warm start: polynomial extrapolation (in the reverse time direction)
+        // of this stage's last transposed Poisson solutions
+        const long nr = g_.nr;
+        double **h = lamh_[st];
+        if (lam_count_[st] >= 2)
+            hipLaunchKernelGGL(k_lam_extrap, dim3(nbk(nr)), dim3(256), 0, s_, lam_[st], (const double *)h[2],
+                               (const double *)h[1], (const double *)h[0], lam_count_[st] >= 3 ? 2 : 1, nr);
         rhs_adjoint_core(this, rhs_, g_, p_, s_, caps[st], ob, Fb_, linb_, advb_, Ub_, Lb_, F1b_, Yt_, Pb_, phib_,
                          lam_[st], ndb_, Mb_, part_, ltol_, lam_valid_, &Adjoint::deriv_T);
         lin_its_ += rhs_.poisson().last_linear_iterations();
+        hipLaunchKernelGGL(k_lam_shift, dim3(nbk(nr)), dim3(256), 0, s_, h[0], h[1], h[2], (const double *)lam_[st],
+                           nr);
+        if (lam_count_[st] < 3) ++lam_count_[st];
         for (int a = 0; a < 4; ++a) {
             hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(256), 0, s_, yb[a], (const double2 *)Fb_[a], 1.0, nc);
             if (st > 0)
@@ -654,6 +689,7 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
     const double J = objective(yc, dV);
     reset_accumulators();
     lam_valid_ = false;
+    for (int &c : lam_count_) c = 0;
     lin_its_ = 0;
     objective_adjoint(yc, dV, yb);
     for (int c = nck - 1; c >= 0; --c) {

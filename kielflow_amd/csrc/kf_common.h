@@ -36,6 +36,18 @@ __host__ __device__ inline C mk(decltype(C().x) a, decltype(C().x) b)
     return c;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD
+// b mod 8).  xcd_block maps the hardware index to a logical one so that each
+// XCD works through one contiguous eighth of the logical blocks: neighbouring
+// tiles then share that XCD's L2.  The tail beyond a multiple of 8 keeps its
+// index.
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned n)
+{
+    const unsigned n8 = n >> 3;
+    if (b >= (n8 << 3)) return b;
+    return (b & 7u) * n8 + (b >> 3);
+}
+
 template <typename C>
 __device__ __forceinline__ C cadd(C a, C b) { return mk<C>(a.x + b.x, a.y + b.y); }
 template <typename C>

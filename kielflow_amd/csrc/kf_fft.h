@@ -70,7 +70,7 @@ k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
     __shared__ C lds[N * CB];
     __shared__ C tw[N];
     const int tid = threadIdx.x;
-    const long q0 = (long)blockIdx.x * CB;
+    const long q0 = (long)xcd_block(blockIdx.x, gridDim.x) * CB;
     for (int m = tid; m < N; m += WG) tw[m] = tw_g[m];
     // each thread owns one column c and rows t0, t0+TPT, ... (the column's
     // address arithmetic is loop invariant)
@@ -122,7 +122,7 @@ k_zr2c(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long ro
     __shared__ C tw[M];
     __shared__ C twN[M + 1];  // exp(-2 pi i k / (2M)), k = 0..M
     const int tid = threadIdx.x;
-    const long r0 = (long)blockIdx.x * RB;
+    const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
     for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
     for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
     for (int e = tid; e < RB * M; e += WG) {
@@ -188,7 +188,7 @@ k_zc2r(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long ro
     __shared__ C tw[M];
     __shared__ C twN[M + 1];
     const int tid = threadIdx.x;
-    const long r0 = (long)blockIdx.x * RB;
+    const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
     for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
     for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
     for (int e = tid; e < RB * (M + 1); e += WG) {
@@ -222,6 +222,9 @@ k_zc2r(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long ro
 template <typename R>
 struct Plan {
     int nx, ny, nz, nzc;
+    // row length (complex) of the scratch field: Nz/2+1 rounded up to 128
+    // bytes, so that the y pass reads and writes whole cache lines
+    int nzs;
     long nr, nc;
     // column-block variant of the x and y passes, picked by the autotuner
     // at plan creation (0: cb_for columns per workgroup, 1: half of them)
@@ -342,6 +345,50 @@ void launch_zc2r(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST
     default:
         throw Error("unsupported z length " + std::to_string(2 * m));
     }
+}
+
+// Strided passes over the padded scratch: columns kz >= nvalid are padding
+// (loaded as zero, never stored)
+template <typename R>
+struct LoadMasked {
+    const C2<R> *p;
+    StridedAddr a;
+    unsigned nvalid;
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const
+    {
+        if ((unsigned)q % (unsigned)a.qdiv >= nvalid) return mk<C2<R>>((R)0, (R)0);
+        return p[a(q, t)];
+    }
+};
+
+template <typename R>
+struct StoreMasked {
+    C2<R> *p;
+    StridedAddr a;
+    unsigned nvalid;
+    __device__ __forceinline__ void operator()(long q, int t, C2<R> v) const
+    {
+        if ((unsigned)q % (unsigned)a.qdiv >= nvalid) return;
+        p[a(q, t)] = v;
+    }
+};
+
+// scratch-side addressing: y pass over Nx*nzs columns (padding masked),
+// x pass over the Ny*Nzc columns of the unpadded spectral layout
+template <typename R>
+inline StridedAddr saddr_y(const Plan<R> &p)
+{
+    return StridedAddr{p.nzs, (long)p.ny * p.nzs, p.nzs};
+}
+template <typename R>
+inline StridedAddr saddr_x(const Plan<R> &p)
+{
+    return StridedAddr{p.nzc, p.nzs, (long)p.ny * p.nzs};
+}
+template <typename R>
+inline long scratch_cols_y(const Plan<R> &p)
+{
+    return (long)p.nx * p.nzs;
 }
 
 // address helpers for the y and x passes on the [Nx][Ny][Nzc] layout

@@ -59,11 +59,11 @@ static void autotune(Plan<R> &p, hipStream_t s)
     if (env && env[0] == '0') return;
     if (cb_variant<R>(p.nx, 1) == cb_variant<R>(p.nx, 0) && cb_variant<R>(p.ny, 1) == cb_variant<R>(p.ny, 0))
         return;
-    KF_HIP(hipMemsetAsync(p.scratch, 0, sizeof(C2<R>) * p.nc, s));
+    KF_HIP(hipMemsetAsync(p.scratch, 0, sizeof(C2<R>) * (size_t)p.nx * p.ny * p.nzs, s));
     hipEvent_t e0, e1;
     KF_HIP(hipEventCreate(&e0));
     KF_HIP(hipEventCreate(&e1));
-    StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
+    const StridedAddr ay = saddr_y(p), ax = saddr_x(p);
     auto time_pass = [&](int pass, int var) {
         float ms[3];
         for (int r = -1; r < 3; ++r) {
@@ -72,8 +72,8 @@ static void autotune(Plan<R> &p, hipStream_t s)
                 launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax},
                                       StorePlain<R>{p.scratch, ax}, s, 1000, var);
             else
-                launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                                      StorePlain<R>{p.scratch, ay}, s, 1000, var);
+                launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
+                                      StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, 1000, var);
             KF_HIP(hipEventRecord(e1, s));
             KF_HIP(hipEventSynchronize(e1));
             if (r >= 0) KF_HIP(hipEventElapsedTime(&ms[r], e0, e1));
@@ -103,7 +103,11 @@ void plan_init(Plan<R> &p, int nx, int ny, int nz, hipStream_t s)
     p.tw_y = twiddles<R>(ny, ny, s);
     p.tw_m = twiddles<R>(nz / 2, nz / 2, s);
     p.tw_n = twiddles<R>(nz, nz / 2 + 1, s);
-    KF_HIP(hipMalloc(&p.scratch, sizeof(C2<R>) * p.nc));
+    {
+        const int al = 128 / (int)sizeof(C2<R>);
+        p.nzs = (p.nzc + al - 1) / al * al;
+    }
+    KF_HIP(hipMalloc(&p.scratch, sizeof(C2<R>) * (size_t)nx * ny * p.nzs));
     autotune(p, s);
 }
 
@@ -134,13 +138,13 @@ void r2c(const Plan<R> &p, const R *in, C2<R> *out, hipStream_t s)
 template <typename R>
 void c2r(const Plan<R> &p, const C2<R> *in, R *out, hipStream_t s)
 {
-    StridedAddr ay = addr_y(p.nx, p.ny, p.nzc), ax = addr_x(p.nx, p.ny, p.nzc);
+    const StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), sx = saddr_x(p), sy = saddr_y(p);
     launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{in, ax},
-                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X, p.vx);
-    launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
+                          StorePlain<R>{p.scratch, sx}, s, prof::FFT_X, p.vx);
+    launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, sy, (unsigned)p.nzc},
+                          StoreMasked<R>{p.scratch, sy, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
     const long rows = (long)p.nx * p.ny;
-    launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc},
+    launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzs},
                    RowStoreReal<R>{out, p.nz, (R)(1.0 / (double)p.nr)}, s);
 }
 

@@ -41,6 +41,8 @@ double axis_dS(const AxisDesc &a, int index);
 template <typename R>
 struct AxisCoef {
     const R *cp, *cm, *dd;  // 1/(h+ (h+ + h-)), 1/(h- (h+ + h-)), 2/(h+ h-)
+    int uni;                // uniform spacing: the constants below replace the arrays
+    R ucp, udd;
 };
 
 template <typename R>
@@ -166,6 +168,7 @@ class Poisson {
     double *h_stat_ = nullptr;    // pinned mirror
     int *h_ctl_ = nullptr;
     double *corr64_ = nullptr;    // FP64 scratch (mixed defect correction)
+    double *alt64_ = nullptr;     // second potential buffer of the fused Newton update
     bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
     int kiters_ = 2;
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)

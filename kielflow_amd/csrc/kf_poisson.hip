@@ -77,24 +77,66 @@ __device__ __forceinline__ void dec3(long p, int ny, int nz, int &i, int &j, int
     i = (int)q;
 }
 
+// On an axis of uniform spacing the coefficients are the constants ucp, udd
+// (kernel arguments, so no per-point coefficient loads); other axes gather
+// them per index.  Offsets are 32-bit (every level has fewer than 2^31 points).
+// value of a field at a linear index: x[q]
+template <typename R>
+struct Plain {
+    const R *__restrict__ x;
+    __device__ __forceinline__ R operator()(unsigned q) const { return x[q]; }
+};
+// phi[q] + x[q]: the potential after the pending Newton correction
+template <typename R>
+struct Corrected {
+    const double *__restrict__ phi;
+    const R *__restrict__ x;
+    __device__ __forceinline__ double operator()(unsigned q) const { return phi[q] + (double)x[q]; }
+};
+
+template <typename R, class LD>
+__device__ __forceinline__ void stencil_ld(const LevelView<R> &L, const LD &x, int i, int j, int k, R &A, R &D)
+{
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const unsigned sx = (unsigned)ny * (unsigned)nz;
+    const unsigned ip = wrapi(i + 1, nx), im = wrapi(i - 1, nx);
+    const unsigned jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
+    const unsigned kp = wrapi(k + 1, nz), km = wrapi(k - 1, nz);
+    const unsigned ri = (unsigned)i * sx, rj = (unsigned)j * (unsigned)nz;
+    const R xip = x(ip * sx + rj + (unsigned)k), xim = x(im * sx + rj + (unsigned)k);
+    const R xjp = x(ri + jp * (unsigned)nz + (unsigned)k), xjm = x(ri + jm * (unsigned)nz + (unsigned)k);
+    const R xkp = x(ri + rj + kp), xkm = x(ri + rj + km);
+    R a, d;
+    if (L.c[0].uni) {
+        a = L.c[0].ucp * (xip + xim);
+        d = L.c[0].udd;
+    } else {
+        a = L.c[0].cp[i] * xip + L.c[0].cm[i] * xim;
+        d = L.c[0].dd[i];
+    }
+    if (L.c[1].uni) {
+        a += L.c[1].ucp * (xjp + xjm);
+        d += L.c[1].udd;
+    } else {
+        a += L.c[1].cp[j] * xjp + L.c[1].cm[j] * xjm;
+        d += L.c[1].dd[j];
+    }
+    if (L.c[2].uni) {
+        a += L.c[2].ucp * (xkp + xkm);
+        d += L.c[2].udd;
+    } else {
+        a += L.c[2].cp[k] * xkp + L.c[2].cm[k] * xkm;
+        d += L.c[2].dd[k];
+    }
+    A = (R)2 * a;
+    D = d;
+}
+
 template <typename R>
 __device__ __forceinline__ void stencil(const LevelView<R> &L, const R *__restrict__ x, int i, int j,
                                         int k, R &A, R &D)
 {
-    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
-    const long sx = (long)ny * nz;
-    const int ip = wrapi(i + 1, nx), im = wrapi(i - 1, nx);
-    const int jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
-    const int kp = wrapi(k + 1, nz), km = wrapi(k - 1, nz);
-    const long row = (long)i * sx + (long)j * nz;
-    R a = L.c[0].cp[i] * x[(long)ip * sx + (long)j * nz + k];
-    a += L.c[0].cm[i] * x[(long)im * sx + (long)j * nz + k];
-    a += L.c[1].cp[j] * x[(long)i * sx + (long)jp * nz + k];
-    a += L.c[1].cm[j] * x[(long)i * sx + (long)jm * nz + k];
-    a += L.c[2].cp[k] * x[row + kp];
-    a += L.c[2].cm[k] * x[row + km];
-    A = (R)2 * a;
-    D = L.c[0].dd[i] + L.c[1].dd[j] + L.c[2].dd[k];
+    stencil_ld<R>(L, Plain<R>{x}, i, j, k, A, D);
 }
 
 // block reductions: wavefront (64 lanes) shuffles, then one LDS slot per wave
@@ -148,7 +190,7 @@ k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *_
     if (ctl[0]) return;
     double fmaxv = 0., rmax = 0.;
     const int nz = L.n[2], ny = L.n[1];
-    const long p0 = (long)blockIdx.x * (RED_BLOCK * NR_PTS) + threadIdx.x;
+    const long p0 = (long)xcd_block(blockIdx.x, gridDim.x) * (RED_BLOCK * NR_PTS) + threadIdx.x;
     double F[NR_PTS], ex[NR_PTS], rh[NR_PTS];
 #pragma unroll
     for (int u = 0; u < NR_PTS; ++u) {
@@ -185,7 +227,221 @@ k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *_
     }
 }
 
-// ctl: [0] done, [1] iterations, [2] converged; stat: [0] rel resid
+// Newton update fused with the next residual: phi_out = phi_in + x and
+// F(phi_out) as k_newton_resid (the stencil reads phi_in + x, so phi_in is
+// not modified while neighbours still read it).  ctl[3] records which of the
+// two potential buffers holds the newest iterate.
+template <typename RIN, typename R>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_newton_upd_resid(LevelView<double> L, const double *__restrict__ phi_in, const R *__restrict__ x,
+                   double *__restrict__ phi_out, int out_id, const RIN *__restrict__ rho, R *__restrict__ b,
+                   R *__restrict__ e, double *__restrict__ part, int *ctl)
+{
+    __shared__ double sh[RED_BLOCK / 64];
+    if (ctl[0]) return;
+    double fmaxv = 0., rmax = 0.;
+    const int nz = L.n[2], ny = L.n[1];
+    const long p0 = (long)xcd_block(blockIdx.x, gridDim.x) * (RED_BLOCK * NR_PTS) + threadIdx.x;
+    const Corrected<R> cur{phi_in, x};
+    double F[NR_PTS], ex[NR_PTS], rh[NR_PTS], xn[NR_PTS];
+#pragma unroll
+    for (int u = 0; u < NR_PTS; ++u) {
+        const long p = p0 + (long)u * RED_BLOCK;
+        F[u] = 0.;
+        ex[u] = 0.;
+        rh[u] = 0.;
+        xn[u] = 0.;
+        if (p < L.N) {
+            int i, j, k;
+            dec3(p, ny, nz, i, j, k);
+            double A, D;
+            stencil_ld<double>(L, cur, i, j, k, A, D);
+            xn[u] = cur((unsigned)p);
+            ex[u] = exp(xn[u]);
+            rh[u] = (double)rho[p];
+            F[u] = A - D * xn[u] - ex[u] + rh[u];
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < NR_PTS; ++u) {
+        const long p = p0 + (long)u * RED_BLOCK;
+        if (p < L.N) {
+            phi_out[p] = xn[u];
+            b[p] = (R)(-F[u]);
+            e[p] = (R)ex[u];
+            fmaxv = fmax(fmaxv, fabs(F[u]));
+            rmax = fmax(rmax, fabs(rh[u]));
+        }
+    }
+    fmaxv = block_max(fmaxv, sh);
+    rmax = block_max(rmax, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = fmaxv;
+        part[2 * blockIdx.x + 1] = rmax;
+        if (blockIdx.x == 0) ctl[3] = out_id;
+    }
+}
+
+// Newton residual with each thread on 4 consecutive z points of one line
+// (Nz % 4 == 0): one index decode, vector loads of the line and of its four
+// neighbour lines, and the two z neighbours outside the quad.  With x given,
+// the pending correction is applied first (phi_out = phi_in + x), as in
+// k_newton_upd_resid; with x == nullptr it is k_newton_resid.
+template <typename R>
+__device__ __forceinline__ void load4(const double *__restrict__ phi, const R *__restrict__ x, unsigned q,
+                                      double v[4])
+{
+    const double2 a = *reinterpret_cast<const double2 *>(phi + q);
+    const double2 b = *reinterpret_cast<const double2 *>(phi + q + 2);
+    v[0] = a.x;
+    v[1] = a.y;
+    v[2] = b.x;
+    v[3] = b.y;
+    if (x) {
+        if constexpr (std::is_same<R, float>::value) {
+            const float4 c = *reinterpret_cast<const float4 *>(x + q);
+            v[0] += (double)c.x;
+            v[1] += (double)c.y;
+            v[2] += (double)c.z;
+            v[3] += (double)c.w;
+        } else {
+            const double2 c = *reinterpret_cast<const double2 *>(x + q);
+            const double2 d = *reinterpret_cast<const double2 *>(x + q + 2);
+            v[0] += c.x;
+            v[1] += c.y;
+            v[2] += d.x;
+            v[3] += d.y;
+        }
+    }
+}
+
+template <typename R>
+__device__ __forceinline__ void store4(R *__restrict__ p, unsigned q, const double v[4])
+{
+    if constexpr (std::is_same<R, float>::value) {
+        *reinterpret_cast<float4 *>(p + q) = make_float4((float)v[0], (float)v[1], (float)v[2], (float)v[3]);
+    } else {
+        *reinterpret_cast<double2 *>(p + q) = make_double2(v[0], v[1]);
+        *reinterpret_cast<double2 *>(p + q + 2) = make_double2(v[2], v[3]);
+    }
+}
+
+template <typename RIN, typename R>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_newton_z4(LevelView<double> L, const double *__restrict__ phi_in, const R *__restrict__ x,
+            double *__restrict__ phi_out, int out_id, const RIN *__restrict__ rho, R *__restrict__ b,
+            R *__restrict__ e, double *__restrict__ part, int *ctl)
+{
+    __shared__ double sh[RED_BLOCK / 64];
+    if (ctl[0]) return;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const unsigned nq = (unsigned)(L.N >> 2);
+    const unsigned t = xcd_block(blockIdx.x, gridDim.x) * RED_BLOCK + threadIdx.x;
+    double fmaxv = 0., rmax = 0.;
+    if (t < nq) {
+        const unsigned nz4 = (unsigned)nz >> 2;
+        const unsigned line = t / nz4, k0 = (t - line * nz4) << 2;
+        const unsigned i = line / (unsigned)ny, j = line - i * (unsigned)ny;
+        const unsigned sx = (unsigned)ny * (unsigned)nz;
+        const unsigned q = line * (unsigned)nz + k0;
+        const unsigned ip = wrapi((int)i + 1, nx), im = wrapi((int)i - 1, nx);
+        const unsigned jp = wrapi((int)j + 1, ny), jm = wrapi((int)j - 1, ny);
+        double c[4], xp[4], xm[4], yp[4], ym[4];
+        load4<R>(phi_in, x, q, c);
+        load4<R>(phi_in, x, q + (ip - i) * sx, xp);
+        load4<R>(phi_in, x, q + (im - i) * sx, xm);
+        load4<R>(phi_in, x, q + (jp - j) * (unsigned)nz, yp);
+        load4<R>(phi_in, x, q + (jm - j) * (unsigned)nz, ym);
+        const unsigned kl = wrapi((int)k0 - 1, nz), kr = wrapi((int)k0 + 4, nz);
+        const unsigned rowb = line * (unsigned)nz;
+        double zl = phi_in[rowb + kl], zr = phi_in[rowb + kr];
+        if (x) {
+            zl += (double)x[rowb + kl];
+            zr += (double)x[rowb + kr];
+        }
+        // per-axis coefficients (uniform constants or gathered)
+        double cpx, cmx, ddx, cpy, cmy, ddy;
+        if (L.c[0].uni) {
+            cpx = cmx = L.c[0].ucp;
+            ddx = L.c[0].udd;
+        } else {
+            cpx = L.c[0].cp[i];
+            cmx = L.c[0].cm[i];
+            ddx = L.c[0].dd[i];
+        }
+        if (L.c[1].uni) {
+            cpy = cmy = L.c[1].ucp;
+            ddy = L.c[1].udd;
+        } else {
+            cpy = L.c[1].cp[j];
+            cmy = L.c[1].cm[j];
+            ddy = L.c[1].dd[j];
+        }
+        double rh[4];
+        if constexpr (std::is_same<RIN, float>::value) {
+            const float4 r4 = *reinterpret_cast<const float4 *>(rho + q);
+            rh[0] = r4.x;
+            rh[1] = r4.y;
+            rh[2] = r4.z;
+            rh[3] = r4.w;
+        } else {
+            const double2 r0 = *reinterpret_cast<const double2 *>(rho + q);
+            const double2 r1 = *reinterpret_cast<const double2 *>(rho + q + 2);
+            rh[0] = r0.x;
+            rh[1] = r0.y;
+            rh[2] = r1.x;
+            rh[3] = r1.y;
+        }
+        double F[4], ex[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const double zp = u < 3 ? c[u + 1] : zr, zm = u > 0 ? c[u - 1] : zl;
+            double cpz, cmz, ddz;
+            if (L.c[2].uni) {
+                cpz = cmz = L.c[2].ucp;
+                ddz = L.c[2].udd;
+            } else {
+                cpz = L.c[2].cp[k0 + u];
+                cmz = L.c[2].cm[k0 + u];
+                ddz = L.c[2].dd[k0 + u];
+            }
+            double a, d;
+            if (L.c[0].uni) a = cpx * (xp[u] + xm[u]);
+            else a = cpx * xp[u] + cmx * xm[u];
+            if (L.c[1].uni) a += cpy * (yp[u] + ym[u]);
+            else a += cpy * yp[u] + cmy * ym[u];
+            if (L.c[2].uni) a += cpz * (zp + zm);
+            else a += cpz * zp + cmz * zm;
+            d = ddx + ddy + ddz;
+            ex[u] = exp(c[u]);
+            F[u] = 2. * a - d * c[u] - ex[u] + rh[u];
+            fmaxv = fmax(fmaxv, fabs(F[u]));
+            rmax = fmax(rmax, fabs(rh[u]));
+        }
+        if (x) store4<double>(phi_out, q, c);
+        const double nb[4] = {-F[0], -F[1], -F[2], -F[3]};
+        store4<R>(b, q, nb);
+        store4<R>(e, q, ex);
+    }
+    fmaxv = block_max(fmaxv, sh);
+    rmax = block_max(rmax, sh);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = fmaxv;
+        part[2 * blockIdx.x + 1] = rmax;
+        if (x && blockIdx.x == 0) ctl[3] = out_id;
+    }
+}
+
+// the newest iterate back into the caller's buffer when it sits in the other one
+__global__ void k_phi_final(double *__restrict__ phi, const double *__restrict__ alt, long n, const int *ctl)
+{
+    if (ctl[3] != 1) return;
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) phi[p] = alt[p];
+}
+
+// ctl: [0] done, [1] iterations, [2] converged, [3] buffer of the newest
+// iterate (0 caller's, 1 alternate); stat: [0] rel resid
 __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_check(const double *__restrict__ part, int nblk, int it, int max_it, double tol, int *ctl,
                int *fail, double *stat)
@@ -237,7 +493,7 @@ __global__ void k_rb(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
                      const R *__restrict__ e, int color, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (q >= L.N / 2) return;
     const int nz2 = L.n[2] / 2, ny = L.n[1];
     int i, j, kk;
@@ -255,7 +511,7 @@ __global__ void k_jacobi(LevelView<R> L, const R *__restrict__ x, R *__restrict_
                          const R *__restrict__ b, const R *__restrict__ e, R w, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= L.N) return;
     const int nz = L.n[2], ny = L.n[1];
     int i, j, k;
@@ -270,7 +526,7 @@ template <typename R>
 __global__ void k_copy(R *__restrict__ dst, const R *__restrict__ src, long n, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p < n) dst[p] = src[p];
 }
 
@@ -280,7 +536,7 @@ __global__ void k_lin_resid(LevelView<R> L, const R *__restrict__ x, const R *__
                             const R *__restrict__ e, R *__restrict__ r, const int *ctl)
 {
     if (ctl && ctl[0]) return;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= L.N) return;
     const int nz = L.n[2], ny = L.n[1];
     int i, j, k;
@@ -297,7 +553,7 @@ __global__ void k_restrict(const R *__restrict__ f, int fx, int fy, int fz, R *_
 {
     if (ctl && ctl[0]) return;
     const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= (long)cx * cy * cz) return;
     int i, j, k;
         dec3(p, cy, cz, i, j, k);
@@ -323,7 +579,7 @@ __global__ void k_prolong_add(const R *__restrict__ c, int cx, int cy, int cz, R
 {
     if (ctl && ctl[0]) return;
     const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= (long)fx * fy * fz) return;
     int i, j, k;
         dec3(p, fy, fz, i, j, k);
@@ -785,6 +1041,7 @@ Poisson::~Poisson()
     (void)hipHostFree(h_stat_);
     (void)hipHostFree(h_ctl_);
     (void)hipFree(corr64_);
+    (void)hipFree(alt64_);
     (void)hipFree(d_scal_);
     (void)hipFree(lsx_);
     (void)hipFree(lsr_);
@@ -812,6 +1069,11 @@ LevelView<R> Poisson::view(int l) const
         v.c[d].cm = base + off + L.n[d];
         v.c[d].dd = base + off + 2 * L.n[d];
         off += 3 * (size_t)L.n[d];
+        // equidistant axis: h = L/n at every level, cp = cm = 1/(2h^2), dd = 2/h^2
+        v.c[d].uni = ax_[d].type == 1 ? 1 : 0;
+        const double h = ax_[d].L / (double)L.n[d];
+        v.c[d].ucp = (R)(1. / (2. * h * h));
+        v.c[d].udd = (R)(2. / (h * h));
     }
     return v;
 }
@@ -924,7 +1186,8 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
 {
     Level &L0 = lv_[0];
     const long N = L0.N;
-    const unsigned nbr = nblocks(N, RED_BLOCK * NR_PTS);
+    const bool z4 = (L0.n[2] % 4) == 0;
+    const unsigned nbr = z4 ? nblocks(N / 4, RED_BLOCK) : nblocks(N, RED_BLOCK * NR_PTS);
     LevelView<double> v64 = view<double>(0);
     R *x0, *b0, *e0;
     if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
@@ -933,13 +1196,37 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
     const bool graph = graph_it_ > 0;
     const int max_it = graph ? graph_it_ : max_it_;
+    if (!alt64_) KF_HIP(hipMalloc(&alt64_, sizeof(double) * N));
+    // iteration it >= 1 applies the correction of iteration it-1: odd
+    // iterations write the alternate buffer, even ones the caller's
     auto check = [&](int it) {
-        if (rho64)
-            hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
-                               (const double *)phi, rho64, b0, e0, d_red_, ctl);
-        else
-            hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
-                               (const double *)phi, rho32, b0, e0, d_red_, ctl);
+        if (z4) {
+            const double *src = (it == 0 || (it & 1)) ? phi : alt64_;
+            double *dst = (it & 1) ? alt64_ : phi;
+            const R *xc = it == 0 ? nullptr : (const R *)x0;
+            if (rho64)
+                hipLaunchKernelGGL((k_newton_z4<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
+                                   it & 1, rho64, b0, e0, d_red_, d_ctl_);
+            else
+                hipLaunchKernelGGL((k_newton_z4<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
+                                   it & 1, rho32, b0, e0, d_red_, d_ctl_);
+        } else if (it == 0) {
+            if (rho64)
+                hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
+                                   (const double *)phi, rho64, b0, e0, d_red_, ctl);
+            else
+                hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
+                                   (const double *)phi, rho32, b0, e0, d_red_, ctl);
+        } else {
+            const double *src = (it & 1) ? phi : alt64_;
+            double *dst = (it & 1) ? alt64_ : phi;
+            if (rho64)
+                hipLaunchKernelGGL((k_newton_upd_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
+                                   (const R *)x0, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
+            else
+                hipLaunchKernelGGL((k_newton_upd_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
+                                   (const R *)x0, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
+        }
         hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr,
                            it, max_it, tol_, d_ctl_, d_fail_, d_stat_);
         if (graph) return;
@@ -954,7 +1241,6 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
             hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0, ctl);
             for (int c = 0; c < inner_; ++c) vcycle<R>(0, s);
         }
-        hipLaunchKernelGGL(k_update<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const R *)x0, N, ctl);
     };
     check(0);
     for (int it = 0; it < max_it; ++it) {
@@ -964,6 +1250,7 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
         KF_HIP(hipEventSynchronize(ev_[it & 1]));
         if (h_ctl_[4 * (it & 1)]) break;
     }
+    hipLaunchKernelGGL(k_phi_final, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const double *)alt64_, N, ctl);
 }
 
 template <typename R>

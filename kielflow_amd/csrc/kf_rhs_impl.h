@@ -287,22 +287,22 @@ inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
                      hipStream_t s)
 {
     GridView<R> gv = g.view<R>();
-    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
     launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
                           StorePlain<R>{p.scratch, ax}, s, prof::FFT_X, p.vx);
-    launch_strided<R, +1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
-    launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzc}, st, s);
+    launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
+                          StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
+    launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzs}, st, s);
 }
 
 template <typename R, typename RS>
 inline void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, hipStream_t s)
 {
-    StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), ay = addr_y(p.nx, p.ny, p.nzc);
+    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
     launch_zr2c<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadConv<R, RS>{src, p.nz},
-                   RowStoreC<R>{p.scratch, p.nzc}, s);
-    launch_strided<R, -1>(p.ny, (long)p.nx * p.nzc, p.tw_y, LoadPlain<R>{p.scratch, ay},
-                          StorePlain<R>{p.scratch, ay}, s, prof::FFT_Y, p.vy);
+                   RowStoreC<R>{p.scratch, p.nzs}, s);
+    launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
+                          StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
     launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X, p.vx);
 }
 
