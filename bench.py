@@ -91,16 +91,15 @@ def run_precision(kf, ctx, g, axes, prec, nd_d, mask_d, Fn0, steps, warmup, prof
     # state after warmup + timed steps (compared across precisions)
     out["state"] = [f.get() for f in F]
     out["phi"] = phi.get()
-    if prof_family is not None:
-        # the same steps again with the dominant kernel family bracketed by
-        # HIP events on the library's stream
-        kf.api.profile_enable(prof_family)
+    for fam in (prof_family or ()):
+        # the same steps again with one kernel family bracketed by HIP events
+        # on the library's stream
+        kf.api.profile_enable(fam)
         rk.step(F, steps)
         ctx.sync()
         n, ms = kf.api.profile_read()
         kf.api.profile_enable(-1)
-        out["prof_launches"] = n
-        out["prof_ms"] = ms
+        out.setdefault("prof", {})[fam] = (n, ms)
     for o in (rk, rhs, ps):
         o.close()
     return out
@@ -305,6 +304,34 @@ def long_run(kf, n, steps, every=100):
     print(json.dumps(out), flush=True)
 
 
+def hbm_traffic(kernel, functor, grid=None):
+    """HBM bytes per dispatch of a kernel from the committed counter summary
+    (profiles/hbm_traffic_256_mixed.txt: FETCH_SIZE + WRITE_SIZE per
+    dispatch, calibrated on the library's copy kernel), launch-weighted over
+    the matching lines; (None, None) when absent."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_256_mixed.txt")
+    try:
+        lines = open(path).read().splitlines()
+    except OSError:
+        return None, None
+    tot, cnt = 0.0, 0
+    for ln in lines:
+        if kernel not in ln or (functor and functor not in ln):
+            continue
+        parts = ln.split()
+        try:
+            g, nlaunch, total = parts[-5], int(parts[-4]), float(parts[-1])
+        except (ValueError, IndexError):
+            continue
+        if grid is not None and g != grid:
+            continue
+        tot += total * nlaunch
+        cnt += nlaunch
+    if not cnt:
+        return None, None
+    return round(tot / cnt * 1e6), "profiles/hbm_traffic_256_mixed.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
@@ -340,23 +367,37 @@ def main():
     nd_d, mask_d = ctx.to_device(nd), ctx.to_device(mask)
     del d_ln
 
-    fam = kf.api.PROF_FFT_Y
-    mixed = run_precision(kf, ctx, g, axes, "mixed", nd_d, mask_d, Fn0, args.steps, args.warmup, fam)
+    fams = (kf.api.PROF_FFT_Y, kf.api.PROF_RB0)
+    mixed = run_precision(kf, ctx, g, axes, "mixed", nd_d, mask_d, Fn0, args.steps, args.warmup, fams)
     fp64 = None
     if not args.no_fp64:
         fp64 = run_precision(kf, ctx, g, axes, "fp64", nd_d, mask_d, Fn0, args.steps, args.warmup)
 
-    # roofline of the dominant kernel family: strided y pass, FP32, reads and
-    # writes the complex field once: 2 * Nx*Ny*(Nz/2+1) * 8 bytes per launch
+    # roofline of the FFT y pass (strided, FP32, length 256): it reads and
+    # writes the complex field once, 2 * Nx*Ny*(Nz/2+1) * 8 bytes per launch
     nc = n * n * (n // 2 + 1)
     alg_bytes = 2 * nc * 8
-    avg_ms = mixed["prof_ms"] / max(mixed["prof_launches"], 1)
+    ny_l, ny_ms = mixed["prof"][kf.api.PROF_FFT_Y]
+    avg_ms = ny_ms / max(ny_l, 1)
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic, tsrc = hbm_traffic("k_strided<float, 256", "LoadMasked")
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "kernel": "k_strided (FFT y pass, FP32, length 256)",
-            "avg_launch_us": round(avg_ms * 1e3, 2), "launches": mixed["prof_launches"],
-            "alg_bytes_per_launch": alg_bytes}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_strided (FFT y pass, FP32, length 256, both directions)",
+            "avg_launch_us": round(avg_ms * 1e3, 2), "launches": ny_l, "alg_bytes_per_launch": alg_bytes,
+            "traffic_source": tsrc}
+    # the finest-level red-black half sweep of the Poisson multigrid: per
+    # launch it needs the other colour's x and this colour's b, e and x out,
+    # 4 * (N/2) * 4 bytes (FP32)
+    rb_l, rb_ms = mixed["prof"][kf.api.PROF_RB0]
+    rb_avg = rb_ms / max(rb_l, 1)
+    rb_bytes = 8 * n ** 3
+    rb_traffic, rb_src = hbm_traffic("k_rb<float>", None, grid=str(n ** 3 // 2))
+    roof_rb = {"bound": "hbm", "achieved": round(rb_bytes / (rb_avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": round(rb_bytes / (rb_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+               "traffic": rb_traffic, "kernel": "k_rb (red-black half sweep, finest level, FP32)",
+               "avg_launch_us": round(rb_avg * 1e3, 2), "launches": rb_l, "alg_bytes_per_launch": rb_bytes,
+               "traffic_source": rb_src}
 
     extra = {"poisson_failures_mixed": mixed["poisson_failures"]}
     if fp64 is not None:
@@ -392,6 +433,7 @@ def main():
                    "n": n, "L": L_BOX, "dt": 1e-3, "M": 1.5, "Q": -0.3, "tau": 0.1, "theta": 100.0,
                    "penalization": True, "spectral_viscosity": False, "poisson_tol": 1e-10},
         "roofline": roof,
+        "roofline_poisson_sweep": roof_rb,
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -274,7 +274,7 @@ int kf_adjoint_spectral_to_real(kf_adjoint a, const double *dJdF, double *dJdf);
  * Diagnostic for roofline figures (no counterpart in the reference, which
  * has no profiler hooks, logger.hpp:13-77 aside).  family: 0 FFT x passes,
  * 1 FFT y passes, 2 FFT z R2C passes, 3 FFT z C2R passes, 4 whole Poisson
- * solves; -1 disables.  Each launch of the enabled family is bracketed by
+ * solves, 5 red-black half sweeps on the finest multigrid level; -1 disables.  Each launch of the enabled family is bracketed by
  * HIP events on its stream; read returns the count and summed milliseconds
  * since the last enable. */
 int kf_profile_enable(int family);

@@ -561,7 +561,7 @@ class Batch:
         return c.value
 
 
-PROF_FFT_X, PROF_FFT_Y, PROF_FFT_ZR2C, PROF_FFT_ZC2R, PROF_POISSON = 0, 1, 2, 3, 4
+PROF_FFT_X, PROF_FFT_Y, PROF_FFT_ZR2C, PROF_FFT_ZC2R, PROF_POISSON, PROF_RB0 = 0, 1, 2, 3, 4, 5
 
 
 def profile_enable(family: int):

@@ -47,23 +47,22 @@ static C2<R> *twiddles(int n, int count, hipStream_t s)
     return d;
 }
 
-// Column-block autotuner: time both variants of the x and y passes on the
-// plan's scratch field (median of 3 after a warm-up launch) and keep the
-// faster.  KF_FFT_AUTOTUNE=0 keeps variant 0.  The variants compute the same
-// numbers, so the choice only changes speed.
+// Autotuner: time the column-block variants of the x and y passes, and the
+// y pass with and without the padded scratch row, on the plan's scratch
+// field (median of 3 after a warm-up launch), and keep the fastest.
+// KF_FFT_AUTOTUNE=0 keeps variant 0 and the padded rows.  Every choice
+// computes the same numbers; only the speed changes.
 template <typename R>
 static void autotune(Plan<R> &p, hipStream_t s)
 {
     p.vx = p.vy = 0;
     const char *env = std::getenv("KF_FFT_AUTOTUNE");
     if (env && env[0] == '0') return;
-    if (cb_variant<R>(p.nx, 1) == cb_variant<R>(p.nx, 0) && cb_variant<R>(p.ny, 1) == cb_variant<R>(p.ny, 0))
-        return;
     KF_HIP(hipMemsetAsync(p.scratch, 0, sizeof(C2<R>) * (size_t)p.nx * p.ny * p.nzs, s));
     hipEvent_t e0, e1;
     KF_HIP(hipEventCreate(&e0));
     KF_HIP(hipEventCreate(&e1));
-    const StridedAddr ay = saddr_y(p), ax = saddr_x(p);
+    StridedAddr ay = saddr_y(p), ax = saddr_x(p);
     auto time_pass = [&](int pass, int var) {
         float ms[3];
         for (int r = -1; r < 3; ++r) {
@@ -81,8 +80,28 @@ static void autotune(Plan<R> &p, hipStream_t s)
         std::sort(ms, ms + 3);
         return ms[1];
     };
+    // y pass: column-block variant and scratch row padding together
+    const int nzs_pad = p.nzs;
+    float best = 1e30f;
+    int best_nzs = nzs_pad, best_vy = 0;
+    for (int nzs : {nzs_pad, p.nzc}) {
+        p.nzs = nzs;
+        ay = saddr_y(p);
+        for (int v = 0; v < 2; ++v) {
+            if (v == 1 && cb_variant<R>(p.ny, 1) == cb_variant<R>(p.ny, 0)) continue;
+            const float t = time_pass(1, v);
+            if (t < best) {
+                best = t;
+                best_nzs = nzs;
+                best_vy = v;
+            }
+        }
+        if (nzs_pad == p.nzc) break;
+    }
+    p.nzs = best_nzs;
+    p.vy = best_vy;
+    ax = saddr_x(p);
     if (cb_variant<R>(p.nx, 1) != cb_variant<R>(p.nx, 0)) p.vx = time_pass(0, 1) < time_pass(0, 0) ? 1 : 0;
-    if (cb_variant<R>(p.ny, 1) != cb_variant<R>(p.ny, 0)) p.vy = time_pass(1, 1) < time_pass(1, 0) ? 1 : 0;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
 }

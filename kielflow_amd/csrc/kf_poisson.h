@@ -19,6 +19,8 @@
 //    246-418) built on that iteration.
 #pragma once
 
+#include <type_traits>
+
 #include <vector>
 
 #include "kf_common.h"
@@ -159,6 +161,13 @@ class Poisson {
     int max_it_ = 100;
     int nu1_ = 2, nu2_ = 2, nuc_ = 24, inner_ = 1;
     int graph_it_ = 0;
+    // the finest level's FP32 x, b, e are stored colour by colour
+    bool split0_ = false;
+    template <typename R>
+    bool split_level(int l) const
+    {
+        return l == 0 && split0_ && std::is_same<R, float>::value;
+    }
     std::vector<Level> lv_;
     // device control block: [0] done flag, [1] cycles, [2] fail counter
     int *d_ctl_ = nullptr;

@@ -326,7 +326,38 @@ __device__ __forceinline__ void store4(R *__restrict__ p, unsigned q, const doub
     }
 }
 
-template <typename RIN, typename R>
+// index of (i, j, k) in a colour-split field (layout described further down)
+struct SplitIdx {
+    unsigned ny, nzh, half;
+    __device__ __forceinline__ unsigned operator()(unsigned i, unsigned j, unsigned k) const
+    {
+        return ((i + j + k) & 1u) * half + (i * ny + j) * nzh + (k >> 1);
+    }
+};
+
+// the four values at z = k0..k0+3 (k0 % 4 == 0) of a colour-split field
+// (FP32): colour (i+j) & 1 holds k0 and k0+2, the other colour k0+1 and k0+3
+__device__ __forceinline__ void split4_load(const float *__restrict__ x, unsigned half, unsigned line,
+                                           unsigned nzh, unsigned par0, unsigned k0, double v[4])
+{
+    const unsigned o = line * nzh + (k0 >> 1);
+    const float2 a = *reinterpret_cast<const float2 *>(x + par0 * half + o);
+    const float2 c = *reinterpret_cast<const float2 *>(x + (1u - par0) * half + o);
+    v[0] += (double)a.x;
+    v[1] += (double)c.x;
+    v[2] += (double)a.y;
+    v[3] += (double)c.y;
+}
+
+__device__ __forceinline__ void split4_store(float *__restrict__ x, unsigned half, unsigned line, unsigned nzh,
+                                            unsigned par0, unsigned k0, const double v[4])
+{
+    const unsigned o = line * nzh + (k0 >> 1);
+    *reinterpret_cast<float2 *>(x + par0 * half + o) = make_float2((float)v[0], (float)v[2]);
+    *reinterpret_cast<float2 *>(x + (1u - par0) * half + o) = make_float2((float)v[1], (float)v[3]);
+}
+
+template <typename RIN, typename R, bool SPLIT = false>
 __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_z4(LevelView<double> L, const double *__restrict__ phi_in, const R *__restrict__ x,
             double *__restrict__ phi_out, int out_id, const RIN *__restrict__ rho, R *__restrict__ b,
@@ -347,17 +378,36 @@ k_newton_z4(LevelView<double> L, const double *__restrict__ phi_in, const R *__r
         const unsigned ip = wrapi((int)i + 1, nx), im = wrapi((int)i - 1, nx);
         const unsigned jp = wrapi((int)j + 1, ny), jm = wrapi((int)j - 1, ny);
         double c[4], xp[4], xm[4], yp[4], ym[4];
-        load4<R>(phi_in, x, q, c);
-        load4<R>(phi_in, x, q + (ip - i) * sx, xp);
-        load4<R>(phi_in, x, q + (im - i) * sx, xm);
-        load4<R>(phi_in, x, q + (jp - j) * (unsigned)nz, yp);
-        load4<R>(phi_in, x, q + (jm - j) * (unsigned)nz, ym);
         const unsigned kl = wrapi((int)k0 - 1, nz), kr = wrapi((int)k0 + 4, nz);
         const unsigned rowb = line * (unsigned)nz;
         double zl = phi_in[rowb + kl], zr = phi_in[rowb + kr];
-        if (x) {
-            zl += (double)x[rowb + kl];
-            zr += (double)x[rowb + kr];
+        const unsigned nzh = (unsigned)nz >> 1, half = (unsigned)(L.N >> 1);
+        if constexpr (SPLIT && std::is_same<R, float>::value) {
+            load4<R>(phi_in, nullptr, q, c);
+            load4<R>(phi_in, nullptr, q + (ip - i) * sx, xp);
+            load4<R>(phi_in, nullptr, q + (im - i) * sx, xm);
+            load4<R>(phi_in, nullptr, q + (jp - j) * (unsigned)nz, yp);
+            load4<R>(phi_in, nullptr, q + (jm - j) * (unsigned)nz, ym);
+            if (x) {
+                split4_load(x, half, line, nzh, (i + j) & 1u, k0, c);
+                split4_load(x, half, ip * (unsigned)ny + j, nzh, (ip + j) & 1u, k0, xp);
+                split4_load(x, half, im * (unsigned)ny + j, nzh, (im + j) & 1u, k0, xm);
+                split4_load(x, half, i * (unsigned)ny + jp, nzh, (i + jp) & 1u, k0, yp);
+                split4_load(x, half, i * (unsigned)ny + jm, nzh, (i + jm) & 1u, k0, ym);
+                const SplitIdx S{(unsigned)ny, nzh, half};
+                zl += (double)x[S(i, j, kl)];
+                zr += (double)x[S(i, j, kr)];
+            }
+        } else {
+            load4<R>(phi_in, x, q, c);
+            load4<R>(phi_in, x, q + (ip - i) * sx, xp);
+            load4<R>(phi_in, x, q + (im - i) * sx, xm);
+            load4<R>(phi_in, x, q + (jp - j) * (unsigned)nz, yp);
+            load4<R>(phi_in, x, q + (jm - j) * (unsigned)nz, ym);
+            if (x) {
+                zl += (double)x[rowb + kl];
+                zr += (double)x[rowb + kr];
+            }
         }
         // per-axis coefficients (uniform constants or gathered)
         double cpx, cmx, ddx, cpy, cmy, ddy;
@@ -420,8 +470,13 @@ k_newton_z4(LevelView<double> L, const double *__restrict__ phi_in, const R *__r
         }
         if (x) store4<double>(phi_out, q, c);
         const double nb[4] = {-F[0], -F[1], -F[2], -F[3]};
-        store4<R>(b, q, nb);
-        store4<R>(e, q, ex);
+        if constexpr (SPLIT && std::is_same<R, float>::value) {
+            split4_store(b, half, line, nzh, (i + j) & 1u, k0, nb);
+            split4_store(e, half, line, nzh, (i + j) & 1u, k0, ex);
+        } else {
+            store4<R>(b, q, nb);
+            store4<R>(e, q, ex);
+        }
     }
     fmaxv = block_max(fmaxv, sh);
     rmax = block_max(rmax, sh);
@@ -594,6 +649,173 @@ __global__ void k_prolong_add(const R *__restrict__ c, int cx, int cy, int cz, R
                  wj1 * (wk0 * CV(i1, j1, k0) + wk1 * CV(i1, j1, k1)));
 #undef CV
     f[p] += v;
+}
+
+// ======================================================================== colour-split finest level
+// In mixed precision, when the finest level is not solved by the coarse
+// kernel and its extents are even (Nz a multiple of 4), its FP32 x, b and e
+// are stored colour by colour: colour c = (i+j+k) & 1 at
+// c*N/2 + (i*ny + j)*nz/2 + k/2.  A red-black half sweep then reads the other
+// colour's x and this colour's b and e, and writes this colour's x, as whole
+// cache lines (half the bytes of the interleaved layout).  The residual r
+// stays in the natural layout for the restriction.
+
+template <typename R>
+__device__ __forceinline__ void stencil_split(const LevelView<R> &L, const R *__restrict__ x, const SplitIdx &S,
+                                              int i, int j, int k, R &A, R &D)
+{
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const unsigned ip = wrapi(i + 1, nx), im = wrapi(i - 1, nx);
+    const unsigned jp = wrapi(j + 1, ny), jm = wrapi(j - 1, ny);
+    const unsigned kp = wrapi(k + 1, nz), km = wrapi(k - 1, nz);
+    const R xip = x[S(ip, j, k)], xim = x[S(im, j, k)];
+    const R xjp = x[S(i, jp, k)], xjm = x[S(i, jm, k)];
+    const R xkp = x[S(i, j, kp)], xkm = x[S(i, j, km)];
+    R a, d;
+    if (L.c[0].uni) {
+        a = L.c[0].ucp * (xip + xim);
+        d = L.c[0].udd;
+    } else {
+        a = L.c[0].cp[i] * xip + L.c[0].cm[i] * xim;
+        d = L.c[0].dd[i];
+    }
+    if (L.c[1].uni) {
+        a += L.c[1].ucp * (xjp + xjm);
+        d += L.c[1].udd;
+    } else {
+        a += L.c[1].cp[j] * xjp + L.c[1].cm[j] * xjm;
+        d += L.c[1].dd[j];
+    }
+    if (L.c[2].uni) {
+        a += L.c[2].ucp * (xkp + xkm);
+        d += L.c[2].udd;
+    } else {
+        a += L.c[2].cp[k] * xkp + L.c[2].cm[k] * xkm;
+        d += L.c[2].dd[k];
+    }
+    A = (R)2 * a;
+    D = d;
+}
+
+// half sweep of colour `color` on the split layout: thread q -> (i, j, kk)
+template <typename R>
+__global__ void k_rb_split(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b, const R *__restrict__ e,
+                           int color, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const unsigned half = (unsigned)(L.N >> 1);
+    const unsigned q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= half) return;
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const unsigned nzh = (unsigned)nz >> 1;
+    const unsigned line = q / nzh, kk = q - line * nzh;
+    const unsigned i = line / (unsigned)ny, j = line - i * (unsigned)ny;
+    const unsigned par = (i + j + (unsigned)color) & 1u;
+    const unsigned k = 2u * kk + par;
+    const R *__restrict__ o = x + (1u - (unsigned)color) * half;
+    const unsigned ip = wrapi((int)i + 1, nx), im = wrapi((int)i - 1, nx);
+    const unsigned jp = wrapi((int)j + 1, ny), jm = wrapi((int)j - 1, ny);
+    unsigned kp = kk + par, km = kk + par + nzh - 1u;
+    if (kp >= nzh) kp -= nzh;
+    if (km >= nzh) km -= nzh;
+    const unsigned base = line * nzh;
+    const R xip = o[(ip * (unsigned)ny + j) * nzh + kk], xim = o[(im * (unsigned)ny + j) * nzh + kk];
+    const R xjp = o[(i * (unsigned)ny + jp) * nzh + kk], xjm = o[(i * (unsigned)ny + jm) * nzh + kk];
+    const R xkp = o[base + kp], xkm = o[base + km];
+    R a, d;
+    if (L.c[0].uni) {
+        a = L.c[0].ucp * (xip + xim);
+        d = L.c[0].udd;
+    } else {
+        a = L.c[0].cp[i] * xip + L.c[0].cm[i] * xim;
+        d = L.c[0].dd[i];
+    }
+    if (L.c[1].uni) {
+        a += L.c[1].ucp * (xjp + xjm);
+        d += L.c[1].udd;
+    } else {
+        a += L.c[1].cp[j] * xjp + L.c[1].cm[j] * xjm;
+        d += L.c[1].dd[j];
+    }
+    if (L.c[2].uni) {
+        a += L.c[2].ucp * (xkp + xkm);
+        d += L.c[2].udd;
+    } else {
+        a += L.c[2].cp[k] * xkp + L.c[2].cm[k] * xkm;
+        d += L.c[2].dd[k];
+    }
+    const unsigned c = (unsigned)color * half + base + kk;
+    x[c] = ((R)2 * a - b[c]) / (d + e[c]);
+}
+
+// r (natural) = b - (L x - e x), x, b, e split
+template <typename R>
+__global__ void k_lin_resid_split(LevelView<R> L, const R *__restrict__ x, const R *__restrict__ b,
+                                  const R *__restrict__ e, R *__restrict__ r, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p >= L.N) return;
+    const int nz = L.n[2], ny = L.n[1];
+    int i, j, k;
+    dec3(p, ny, nz, i, j, k);
+    const SplitIdx S{(unsigned)ny, (unsigned)nz >> 1, (unsigned)(L.N >> 1)};
+    R A, D;
+    stencil_split<R>(L, x, S, i, j, k, A, D);
+    const unsigned c = S(i, j, k);
+    r[p] = b[c] - (A - D * x[c] - e[c] * x[c]);
+}
+
+// coarse correction added to the split finest level
+template <typename R>
+__global__ void k_prolong_add_split(const R *__restrict__ c, int cx, int cy, int cz, R *__restrict__ f,
+                                    const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p >= (long)fx * fy * fz) return;
+    int i, j, k;
+    dec3(p, fy, fz, i, j, k);
+    const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
+    const int i1 = wrapi(i0 + 1, cx), j1 = wrapi(j0 + 1, cy), k1 = wrapi(k0 + 1, cz);
+    const R wi1 = (i & 1) ? (R)0.5 : (R)0, wj1 = (j & 1) ? (R)0.5 : (R)0, wk1 = (k & 1) ? (R)0.5 : (R)0;
+    const R wi0 = (R)1 - wi1, wj0 = (R)1 - wj1, wk0 = (R)1 - wk1;
+#define CV(a, b, d) c[((long)(a)*cy + (b)) * cz + (d)]
+    R v = wi0 * (wj0 * (wk0 * CV(i0, j0, k0) + wk1 * CV(i0, j0, k1)) +
+                 wj1 * (wk0 * CV(i0, j1, k0) + wk1 * CV(i0, j1, k1))) +
+          wi1 * (wj0 * (wk0 * CV(i1, j0, k0) + wk1 * CV(i1, j0, k1)) +
+                 wj1 * (wk0 * CV(i1, j1, k0) + wk1 * CV(i1, j1, k1)));
+#undef CV
+    const SplitIdx S{(unsigned)fy, (unsigned)fz >> 1, (unsigned)((long)fx * fy * fz >> 1)};
+    f[S(i, j, k)] += v;
+}
+
+// full weighting of a split fine field (the finest level's e) to the natural coarse level
+template <typename R>
+__global__ void k_restrict_split(const R *__restrict__ f, int fx, int fy, int fz, R *__restrict__ c,
+                                 const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
+    const long p = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (p >= (long)cx * cy * cz) return;
+    int i, j, k;
+    dec3(p, cy, cz, i, j, k);
+    const SplitIdx S{(unsigned)fy, (unsigned)fz >> 1, (unsigned)((long)fx * fy * fz >> 1)};
+    const R w[3] = {(R)0.25, (R)0.5, (R)0.25};
+    R acc = 0;
+    for (int a = -1; a <= 1; ++a) {
+        const int fi = wrapi(2 * i + a, fx);
+        for (int bb = -1; bb <= 1; ++bb) {
+            const int fj = wrapi(2 * j + bb, fy);
+            for (int cc = -1; cc <= 1; ++cc) {
+                const int fk = wrapi(2 * k + cc, fz);
+                acc += w[a + 1] * w[bb + 1] * w[cc + 1] * f[S(fi, fj, fk)];
+            }
+        }
+    }
+    c[p] = acc;
 }
 
 // ======================================================================== coarse levels in one workgroup
@@ -1012,6 +1234,9 @@ Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed
     KF_HIP(hipEventCreateWithFlags(&ev_[0], hipEventDisableTiming));
     KF_HIP(hipEventCreateWithFlags(&ev_[1], hipEventDisableTiming));
     krylov_ = (ax[0].type != 1) || (ax[1].type != 1) || (ax[2].type != 1);
+    // colour-split finest level (FP32 arrays of the mixed solver)
+    split0_ = mixed_ && !krylov_ && lv_.size() > 1 && first_coarse() > 0 && lv_[0].n[0] % 2 == 0 &&
+              lv_[0].n[1] % 2 == 0 && lv_[0].n[2] % 4 == 0;
     {
         const size_t es = mixed_ ? sizeof(float) : sizeof(double);
         for (auto &v : kv_) KF_HIP(hipMalloc(&v, es * lv_[0].N));
@@ -1087,11 +1312,18 @@ void Poisson::smooth(int l, int sweeps, hipStream_t s)
     if constexpr (std::is_same<R, double>::value) { x = L.x64; b = L.b64; e = L.e64; r = L.r64; }
     else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; }
     const bool rb = (L.n[0] % 2 == 0) && (L.n[1] % 2 == 0) && (L.n[2] % 2 == 0);
+    const bool sp = split_level<R>(l);
     for (int it = 0; it < sweeps; ++it) {
         if (rb) {
-            for (int c = 0; c < 2; ++c)
-                hipLaunchKernelGGL(k_rb<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x, b, e, c,
-                                   (const int *)d_ctl_);
+            for (int c = 0; c < 2; ++c) {
+                prof::Scope ps_(l == 0 ? prof::RB0 : 1000, s);
+                if (sp)
+                    hipLaunchKernelGGL(k_rb_split<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x,
+                                       (const R *)b, (const R *)e, c, (const int *)d_ctl_);
+                else
+                    hipLaunchKernelGGL(k_rb<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x, b, e, c,
+                                       (const int *)d_ctl_);
+            }
         } else {
             hipLaunchKernelGGL(k_jacobi<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x, r,
                                (const R *)b, (const R *)e, (R)0.8, (const int *)d_ctl_);
@@ -1140,8 +1372,12 @@ void Poisson::restrict_e(hipStream_t s, const int *ctl)
         R *eb;
         if constexpr (std::is_same<R, double>::value) { ea = A.e64; eb = B.e64; }
         else { ea = A.e32; eb = B.e32; }
-        hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1], A.n[2],
-                           eb, ctl);
+        if (split_level<R>(l))
+            hipLaunchKernelGGL(k_restrict_split<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1],
+                               A.n[2], eb, ctl);
+        else
+            hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(B.N, 256)), dim3(256), 0, s, ea, A.n[0], A.n[1],
+                               A.n[2], eb, ctl);
     }
     if (lc < (int)lv_.size() - 1)
         hipLaunchKernelGGL(k_coarse_restrict_e<R>, dim3(1), dim3(COARSE_BLOCK), 0, s, coarse_set<R>(lc), ctl);
@@ -1166,15 +1402,24 @@ void Poisson::vcycle(int l, hipStream_t s)
     else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; cx = C.x32; cb = C.b32; }
     smooth<R>(l, nu1_, s);
     LevelView<R> v = view<R>(l);
-    hipLaunchKernelGGL(k_lin_resid<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x,
-                       (const R *)b, (const R *)e, r, (const int *)d_ctl_);
+    const bool sp = split_level<R>(l);
+    if (sp)
+        hipLaunchKernelGGL(k_lin_resid_split<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x,
+                           (const R *)b, (const R *)e, r, (const int *)d_ctl_);
+    else
+        hipLaunchKernelGGL(k_lin_resid<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x,
+                           (const R *)b, (const R *)e, r, (const int *)d_ctl_);
     hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(C.N, 256)), dim3(256), 0, s, (const R *)r, L.n[0],
                        L.n[1], L.n[2], cb, (const int *)d_ctl_);
     hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(C.N, 256)), dim3(256), 0, s, cx, C.N, (R)0,
                        (const int *)d_ctl_);
     vcycle<R>(l + 1, s);
-    hipLaunchKernelGGL(k_prolong_add<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, (const R *)cx,
-                       C.n[0], C.n[1], C.n[2], x, (const int *)d_ctl_);
+    if (sp)
+        hipLaunchKernelGGL(k_prolong_add_split<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, (const R *)cx,
+                           C.n[0], C.n[1], C.n[2], x, (const int *)d_ctl_);
+    else
+        hipLaunchKernelGGL(k_prolong_add<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, (const R *)cx,
+                           C.n[0], C.n[1], C.n[2], x, (const int *)d_ctl_);
     smooth<R>(l, nu2_, s);
 }
 
@@ -1204,12 +1449,20 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
             const double *src = (it == 0 || (it & 1)) ? phi : alt64_;
             double *dst = (it & 1) ? alt64_ : phi;
             const R *xc = it == 0 ? nullptr : (const R *)x0;
-            if (rho64)
+            if (split_level<R>(0)) {
+                if (rho64)
+                    hipLaunchKernelGGL((k_newton_z4<double, R, true>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
+                                       xc, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
+                else
+                    hipLaunchKernelGGL((k_newton_z4<float, R, true>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
+                                       xc, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
+            } else if (rho64) {
                 hipLaunchKernelGGL((k_newton_z4<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
                                    it & 1, rho64, b0, e0, d_red_, d_ctl_);
-            else
+            } else {
                 hipLaunchKernelGGL((k_newton_z4<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
                                    it & 1, rho32, b0, e0, d_red_, d_ctl_);
+            }
         } else if (it == 0) {
             if (rho64)
                 hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,

@@ -15,7 +15,8 @@ enum Family {
     FFT_ZR2C = 2,   // z pass, real rows -> half spectrum
     FFT_ZC2R = 3,   // z pass, half spectrum -> real rows
     POISSON = 4,    // all Poisson kernels
-    NFAM = 5
+    RB0 = 5,        // red-black half sweeps on the finest multigrid level
+    NFAM = 6
 };
 
 extern int g_active;  // enabled family or -1
