@@ -259,13 +259,18 @@ int kf_adjoint_accumulators(kf_adjoint a, double *M_bar, double *nd_bar);
  * unit-charge grain shape) and dJ/dFn0 (spectral, device) */
 int kf_adjoint_run(kf_adjoint a, double *const y[4], int nsteps, int k, const double *dV, const double *G,
                    double *J, double *dJdM, double *dJdQ, double *dJdFn0);
-/* HBM budget (bytes) for the stage intermediates of one checkpoint segment
- * (default 64 GiB): within it each segment is recomputed once and kept;
- * beyond it every step is recomputed from the segment's stored states */
+/* HBM budget (bytes) for kept stage intermediates (default 64 GiB).  When
+ * the whole run's intermediates fit (and half the free HBM holds them), the
+ * forward sweep keeps them and nothing is recomputed; else when one
+ * checkpoint segment's fit, each segment is recomputed once and kept; else
+ * every step is recomputed from the segment's stored states */
 int kf_adjoint_set_capture_budget(kf_adjoint a, size_t bytes);
-/* of the last kf_adjoint_run: whether segments were kept (1) or recomputed
- * per step (0), and the iterations of its transposed Poisson solves */
-int kf_adjoint_run_info(kf_adjoint a, int *kept_segments, long *linear_iterations);
+/* bytes of the kept intermediates of one RK4 step (four stages) */
+int kf_adjoint_step_capture_bytes(kf_adjoint a, size_t *bytes);
+/* of the last kf_adjoint_run: how it got its intermediates (KF_ADJ_*), and
+ * the iterations of its transposed Poisson solves */
+enum { KF_ADJ_RECOMPUTE = 0, KF_ADJ_SEGMENTS = 1, KF_ADJ_TRAJECTORY = 2 };
+int kf_adjoint_run_info(kf_adjoint a, int *mode, long *linear_iterations);
 /* real-space cotangent of a spectral one through Fn = OP_FFT(ln n):
  * dJ/d(ln n)(x) = (R2C^T dJ/dFn)(x) (device fields; dJ/dn0 = that / n0) */
 int kf_adjoint_spectral_to_real(kf_adjoint a, const double *dJdF, double *dJdf);

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "libkielflow_amd.so")
 
 KF_OK, KF_ERR_ARG, KF_ERR_HIP, KF_ERR_NOT_CONVERGED, KF_ERR_NAN = 0, 1, 2, 3, 4
 KF_FP64, KF_MIXED, KF_FP32 = 0, 1, 2
+# how an adjoint run got its stage intermediates
+KF_ADJ_RECOMPUTE, KF_ADJ_SEGMENTS, KF_ADJ_TRAJECTORY = 0, 1, 2
 EQST, HYST, SIST = 1, 2, 3
 POISSON_NEWTON_MG, POISSON_NLJ, POISSON_CASCADE = 0, 1, 2
 
@@ -118,6 +120,7 @@ SIGNATURES = {
     "kf_batch_synchronize": (C.c_int, [_vp]),
     "kf_batch_poisson_failures": (C.c_int, [_vp, _ip]),
     "kf_adjoint_set_capture_budget": (C.c_int, [_vp, C.c_size_t]),
+    "kf_adjoint_step_capture_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
     "kf_adjoint_run_info": (C.c_int, [_vp, _ip, C.POINTER(C.c_long)]),
     "kf_profile_enable": (C.c_int, [C.c_int]),
     "kf_profile_read": (C.c_int, [_ip, _dp]),
@@ -489,11 +492,17 @@ class Adjoint:
     def set_capture_budget(self, nbytes):
         check(self.lib.kf_adjoint_set_capture_budget(self.h, int(nbytes)))
 
+    def step_capture_bytes(self):
+        """Bytes of the kept intermediates of one RK4 step."""
+        b = C.c_size_t(0)
+        check(self.lib.kf_adjoint_step_capture_bytes(self.h, C.byref(b)))
+        return b.value
+
     def run_info(self):
-        """(segments kept in HBM?, transposed Poisson iterations) of the last run."""
+        """(KF_ADJ_* mode, transposed Poisson iterations) of the last run."""
         k, it = C.c_int(0), C.c_long(0)
         check(self.lib.kf_adjoint_run_info(self.h, C.byref(k), C.byref(it)))
-        return bool(k.value), it.value
+        return k.value, it.value
 
     def spectral_to_real(self, dF, out):
         """out = R2C^T dF: real-space cotangent of the field whose spectrum is F."""

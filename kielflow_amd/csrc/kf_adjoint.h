@@ -50,11 +50,17 @@ class Adjoint {
     double *nd_bar() { return ndb_; }
     void set_tolerance(double t) { ltol_ = t; }
 
-    // capture budget (bytes) of the segment intermediates kept in HBM
+    // capture budget (bytes) of the stage intermediates kept in HBM
     void set_capture_budget(size_t b) { capture_budget_ = b; }
-    // whether the last run kept its segments' intermediates (no per-step
-    // recomputation) and the transposed Poisson iterations it took
-    bool last_run_kept_segments() const { return used_pool_; }
+    // bytes of the intermediates of one RK4 step (four stages)
+    size_t step_capture_bytes() const;
+    // how the last run got its intermediates: RUN_RECOMPUTE (every step
+    // recomputed from the segment's stored states), RUN_SEGMENTS (each
+    // segment recomputed once and kept) or RUN_TRAJECTORY (the whole
+    // trajectory kept by the forward sweep, nothing recomputed); and the
+    // iterations of its transposed Poisson solves
+    enum { RUN_RECOMPUTE = 0, RUN_SEGMENTS = 1, RUN_TRAJECTORY = 2 };
+    int last_run_mode() const { return run_mode_; }
     long last_run_linear_iterations() const { return lin_its_; }
 
   private:
@@ -69,9 +75,10 @@ class Adjoint {
     double ltol_ = 1e-12;
     Capture cap_[4];
     Rk4<double> *rk_ = nullptr;      // forward steps of runs and recomputations
-    std::vector<Capture> pool_;      // per-stage intermediates of a segment
+    std::vector<Capture> pool_;      // per-stage intermediates of kept steps
+    std::vector<void *> slabs_;      // the pool's allocations
     size_t capture_budget_ = (size_t)64 << 30;
-    bool used_pool_ = false;
+    int run_mode_ = RUN_RECOMPUTE;
     double *lam_[4] = {};            // transposed Poisson solutions per stage (warm start)
     bool lam_valid_ = false;
     double *lamh_[4][3] = {};        // per stage: the last three solutions, newest last

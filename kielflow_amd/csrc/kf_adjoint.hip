@@ -234,15 +234,6 @@ static T *dalloc(long n)
     return p;
 }
 
-static void free_capture(Capture &c)
-{
-    for (auto &u : c.U) (void)hipFree(u);
-    (void)hipFree(c.L);
-    (void)hipFree(c.phi);
-    for (auto &gg : c.G) (void)hipFree(gg);
-    for (auto &d : c.dF0) (void)hipFree(d);
-}
-
 Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rhs.plan()), s_(rhs.stream()), dt_(dt)
 {
     const long nr = g_.nr, nc = g_.nc;
@@ -280,22 +271,25 @@ Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rh
     part_ = dalloc<double>(AMAXB + 8);
 }
 
+static void free_capture(Capture &c)
+{
+    for (auto &u : c.U) (void)hipFree(u);
+    (void)hipFree(c.L);
+    (void)hipFree(c.phi);
+    for (auto &gg : c.G) (void)hipFree(gg);
+    for (auto &d : c.dF0) (void)hipFree(d);
+}
+
 Adjoint::~Adjoint()
 {
     (void)hipStreamSynchronize(s_);
-    for (auto &c : cap_) {
-        for (auto &u : c.U) (void)hipFree(u);
-        (void)hipFree(c.L);
-        (void)hipFree(c.phi);
-        for (auto &gg : c.G) (void)hipFree(gg);
-        for (auto &d : c.dF0) (void)hipFree(d);
-    }
+    for (auto &c : cap_) free_capture(c);
     for (auto &st : kb_)
         for (auto &f : st) (void)hipFree(f);
     for (auto &l : lam_) (void)hipFree(l);
     for (auto &st : lamh_)
         for (auto &l : st) (void)hipFree(l);
-    for (auto &c : pool_) free_capture(c);
+    for (void *p : slabs_) (void)hipFree(p);
     delete rk_;
     for (int a = 0; a < 4; ++a) {
         (void)hipFree(Fb_[a]);
@@ -611,29 +605,40 @@ void Adjoint::objective_adjoint(const double2 *F[4], const double *dV, double2 *
 
 void Adjoint::spectral_to_real_T(const double2 *Y, double *out) { c2r_w(g_, p_, Y, 0, 0, 1.0, out, s_); }
 
-// intermediates of one RK4 step (four stages) in bytes
-static size_t step_capture_bytes(const Grid &g)
+// one stage's intermediates are carved from a slab in pieces rounded up to
+// 256 bytes, so that every field starts on an aligned address
+static size_t piece(size_t bytes) { return (bytes + 255) & ~(size_t)255; }
+
+static size_t stage_capture_bytes(const Grid &g)
 {
-    return 4 * (17 * sizeof(double) * (size_t)g.nr + 4 * sizeof(double2) * (size_t)g.nc);
+    return 17 * piece(sizeof(double) * (size_t)g.nr) + 4 * piece(sizeof(double2) * (size_t)g.nc);
 }
 
+size_t Adjoint::step_capture_bytes() const { return 4 * stage_capture_bytes(g_); }
+
+// grow the pool to the intermediates of `steps` steps, within the capture
+// budget and half of the free HBM; one allocation per growth
 bool Adjoint::ensure_pool(int steps)
 {
-    if ((size_t)steps * step_capture_bytes(g_) > capture_budget_) return false;
+    const size_t per = step_capture_bytes();
+    if ((size_t)steps * per > capture_budget_) return false;
     if ((int)pool_.size() >= 4 * steps) return true;
+    const int add = steps - (int)pool_.size() / 4;
+    const size_t need = (size_t)add * per;
     size_t fr = 0, tot = 0;
     KF_HIP(hipMemGetInfo(&fr, &tot));
-    const size_t have = pool_.size() / 4 * step_capture_bytes(g_);
-    const size_t need = (size_t)steps * step_capture_bytes(g_) - have;
     if (need > fr / 2) return false;
-    const long nr = g_.nr, nc = g_.nc;
-    while ((int)pool_.size() < 4 * steps) {
+    char *slab = nullptr;
+    KF_HIP(hipMalloc(&slab, need));
+    slabs_.push_back(slab);
+    const size_t br = piece(sizeof(double) * (size_t)g_.nr), bc = piece(sizeof(double2) * (size_t)g_.nc);
+    for (int q = 0; q < 4 * add; ++q) {
         Capture c{};
-        for (auto &u : c.U) u = dalloc<double>(nr);
-        c.L = dalloc<double>(nr);
-        c.phi = dalloc<double>(nr);
-        for (auto &gg : c.G) gg = dalloc<double>(nr);
-        for (auto &d : c.dF0) d = dalloc<double2>(nc);
+        for (auto &u : c.U) u = (double *)slab, slab += br;
+        c.L = (double *)slab, slab += br;
+        c.phi = (double *)slab, slab += br;
+        for (auto &gg : c.G) gg = (double *)slab, slab += br;
+        for (auto &d : c.dF0) d = (double2 *)slab, slab += bc;
         pool_.push_back(c);
     }
     return true;
@@ -659,17 +664,23 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
         own.v.push_back(p);
         return p;
     };
-    std::vector<double2 *> ck((size_t)nck * 4);
-    std::vector<double *> ckphi(nck);
-    for (auto &p : ck) p = (double2 *)alloc(sizeof(double2) * nc);
-    for (auto &p : ckphi) p = (double *)alloc(sizeof(double) * nr);
     double2 *yb[4];
     for (auto &p : yb) p = (double2 *)alloc(sizeof(double2) * nc);
-    // a segment's stage intermediates are kept in HBM when they fit, so each
-    // segment is recomputed once; otherwise every step is recomputed from the
-    // segment's stored states
-    const bool keep = ensure_pool(std::min(k, nsteps));
-    std::vector<double2 *> seg;
+    // Where HBM allows, the forward sweep keeps every stage's intermediates
+    // and nothing is recomputed.  Otherwise states and potentials are stored
+    // every k steps and each segment is recomputed once with its
+    // intermediates kept; when even one segment does not fit, every step is
+    // recomputed from the segment's stored states.
+    const bool whole = ensure_pool(nsteps);
+    const bool keep = whole || ensure_pool(std::min(k, nsteps));
+    std::vector<double2 *> ck, seg;
+    std::vector<double *> ckphi;
+    if (!whole) {
+        ck.resize((size_t)nck * 4);
+        ckphi.resize(nck);
+        for (auto &p : ck) p = (double2 *)alloc(sizeof(double2) * nc);
+        for (auto &p : ckphi) p = (double *)alloc(sizeof(double) * nr);
+    }
     if (!keep) {
         seg.resize((size_t)k * 4);
         for (auto &p : seg) p = (double2 *)alloc(sizeof(double2) * nc);
@@ -677,7 +688,9 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
     rk_->set_dt(dt_);
     rhs_.reset_history();
     for (int n = 0; n < nsteps; ++n) {
-        if (n % k == 0) {
+        if (whole) {
+            rhs_.set_capture_stages(&pool_[(size_t)n * 4]);
+        } else if (n % k == 0) {
             const int c = n / k;
             for (int a = 0; a < 4; ++a)
                 KF_HIP(hipMemcpyAsync(ck[(size_t)c * 4 + a], y[a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
@@ -685,6 +698,7 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
         }
         rk_->step(y);
     }
+    rhs_.set_capture_stages(nullptr);
     const double2 *yc[4] = {y[0], y[1], y[2], y[3]};
     const double J = objective(yc, dV);
     reset_accumulators();
@@ -692,7 +706,10 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
     for (int &c : lam_count_) c = 0;
     lin_its_ = 0;
     objective_adjoint(yc, dV, yb);
-    for (int c = nck - 1; c >= 0; --c) {
+    if (whole) {
+        for (int n = nsteps - 1; n >= 0; --n) reverse_step(&pool_[(size_t)n * 4], yb);
+    }
+    for (int c = whole ? -1 : nck - 1; c >= 0; --c) {
         const int n0 = c * k, n1 = std::min(nsteps, n0 + k);
         KF_HIP(hipMemcpyAsync(rhs_.phi(), ckphi[c], sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
         rhs_.reset_history();
@@ -737,7 +754,7 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
     }
     if (dFn0) KF_HIP(hipMemcpyAsync(dFn0, yb[3], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
     KF_HIP(hipStreamSynchronize(s_));
-    used_pool_ = keep;
+    run_mode_ = whole ? RUN_TRAJECTORY : (keep ? RUN_SEGMENTS : RUN_RECOMPUTE);
     return J;
 }
 

@@ -865,12 +865,21 @@ int kf_adjoint_set_capture_budget(kf_adjoint a, size_t bytes)
     });
 }
 
-int kf_adjoint_run_info(kf_adjoint a, int *kept_segments, long *linear_iterations)
+int kf_adjoint_run_info(kf_adjoint a, int *mode, long *linear_iterations)
 {
     return guard([&] {
         KF_REQUIRE(a, "null adjoint");
-        if (kept_segments) *kept_segments = a->a->last_run_kept_segments() ? 1 : 0;
+        if (mode) *mode = a->a->last_run_mode();
         if (linear_iterations) *linear_iterations = a->a->last_run_linear_iterations();
+        return KF_OK;
+    });
+}
+
+int kf_adjoint_step_capture_bytes(kf_adjoint a, size_t *bytes)
+{
+    return guard([&] {
+        KF_REQUIRE(a && bytes, "null argument");
+        *bytes = a->a->step_capture_bytes();
         return KF_OK;
     });
 }

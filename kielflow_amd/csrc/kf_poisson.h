@@ -135,10 +135,13 @@ class Poisson {
     };
     template <typename R>
     LevelView<R> view(int l) const;
+    // zero: x of level l starts at 0 (nothing needs to clear it first)
     template <typename R>
-    void vcycle(int l, hipStream_t s);
+    void vcycle(int l, hipStream_t s, bool zero);
     template <typename R>
-    void smooth(int l, int sweeps, hipStream_t s);
+    void smooth(int l, int sweeps, hipStream_t s, bool zero = false);
+    template <typename R>
+    void zero_level(int l, hipStream_t s);
     template <typename R>
     void newton(double *phi, const double *rho64, const float *rho32, hipStream_t s);
     // BiCGStab on J x = b (level-0 b holds the right side on entry), right

@@ -542,6 +542,16 @@ __global__ void k_update(double *__restrict__ phi, const R *__restrict__ x, long
     if (p < n) phi[p] += (double)x[p];
 }
 
+// the diagonal D of the stencil at (i, j, k) (the sum of the axes' dd)
+template <typename R>
+__device__ __forceinline__ R diag(const LevelView<R> &L, int i, int j, int k)
+{
+    R d = L.c[0].uni ? L.c[0].udd : L.c[0].dd[i];
+    d += L.c[1].uni ? L.c[1].udd : L.c[1].dd[j];
+    d += L.c[2].uni ? L.c[2].udd : L.c[2].dd[k];
+    return d;
+}
+
 // red-black Gauss-Seidel half sweep of  L x - e x = b  (all dims even)
 template <typename R>
 __global__ void k_rb(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
@@ -558,6 +568,26 @@ __global__ void k_rb(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
     stencil<R>(L, x, i, j, k, A, D);
     const long p = ((long)i * ny + j) * L.n[2] + k;
     x[p] = (A - b[p]) / (D + e[p]);
+}
+
+// the first red half sweep from x = 0: red = -b / (D + e) (the sweep's own
+// arithmetic with zero neighbours), black = 0; replaces zeroing x first
+template <typename R>
+__global__ void k_rb_zero(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b, const R *__restrict__ e,
+                          const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= L.N / 2) return;
+    const int nz2 = L.n[2] / 2, ny = L.n[1];
+    int i, j, kk;
+    dec3(q, ny, nz2, i, j, kk);
+    const int par = (i + j) & 1;
+    const int k = 2 * kk + par;
+    const R D = diag<R>(L, i, j, k);
+    const long row = ((long)i * ny + j) * L.n[2];
+    x[row + k] = ((R)0 - b[row + k]) / (D + e[row + k]);
+    x[row + 2 * kk + 1 - par] = (R)0;
 }
 
 // weighted Jacobi sweep (levels with an odd extent): out = x + w((A-b)/(D+e) - x)
@@ -748,6 +778,25 @@ __global__ void k_rb_split(LevelView<R> L, R *__restrict__ x, const R *__restric
     x[c] = ((R)2 * a - b[c]) / (d + e[c]);
 }
 
+// k_rb_zero on the split layout: colour 0 = -b / (D + e), colour 1 = 0
+template <typename R>
+__global__ void k_rb_split_zero(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
+                                const R *__restrict__ e, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const unsigned half = (unsigned)(L.N >> 1);
+    const unsigned q = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= half) return;
+    const int ny = L.n[1], nz = L.n[2];
+    const unsigned nzh = (unsigned)nz >> 1;
+    const unsigned line = q / nzh, kk = q - line * nzh;
+    const unsigned i = line / (unsigned)ny, j = line - i * (unsigned)ny;
+    const unsigned k = 2u * kk + ((i + j) & 1u);
+    const R D = diag<R>(L, (int)i, (int)j, (int)k);
+    x[q] = ((R)0 - b[q]) / (D + e[q]);
+    x[half + q] = (R)0;
+}
+
 // r (natural) = b - (L x - e x), x, b, e split
 template <typename R>
 __global__ void k_lin_resid_split(LevelView<R> L, const R *__restrict__ x, const R *__restrict__ b,
@@ -832,6 +881,7 @@ struct CoarseSet {
     LevelView<R> L[COARSE_MAXL];
     R *x[COARSE_MAXL], *b[COARSE_MAXL], *e[COARSE_MAXL], *r[COARSE_MAXL];
     int nu1, nu2, nuc;
+    int zero0;  // start the first level from x = 0
 };
 
 template <typename R>
@@ -876,6 +926,10 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs,
     if (ctl[0]) return;
     const int tid = threadIdx.x, bs = blockDim.x;
     const int nl = cs.nl;
+    if (cs.zero0) {
+        for (long p = tid; p < cs.L[0].N; p += bs) cs.x[0][p] = (R)0;
+        __syncthreads();
+    }
     for (int l = 0; l < nl - 1; ++l) {
         const LevelView<R> &L = cs.L[l];
         blk_smooth<R>(L, cs.x[l], cs.b[l], cs.e[l], cs.r[l], cs.nu1);
@@ -1304,7 +1358,17 @@ LevelView<R> Poisson::view(int l) const
 }
 
 template <typename R>
-void Poisson::smooth(int l, int sweeps, hipStream_t s)
+void Poisson::zero_level(int l, hipStream_t s)
+{
+    Level &L = lv_[l];
+    R *x;
+    if constexpr (std::is_same<R, double>::value) x = L.x64;
+    else x = L.x32;
+    hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, x, L.N, (R)0, (const int *)d_ctl_);
+}
+
+template <typename R>
+void Poisson::smooth(int l, int sweeps, hipStream_t s, bool zero)
 {
     Level &L = lv_[l];
     LevelView<R> v = view<R>(l);
@@ -1313,16 +1377,29 @@ void Poisson::smooth(int l, int sweeps, hipStream_t s)
     else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; }
     const bool rb = (L.n[0] % 2 == 0) && (L.n[1] % 2 == 0) && (L.n[2] % 2 == 0);
     const bool sp = split_level<R>(l);
+    if (zero && (!rb || sweeps < 1)) {
+        zero_level<R>(l, s);
+        zero = false;
+    }
     for (int it = 0; it < sweeps; ++it) {
         if (rb) {
             for (int c = 0; c < 2; ++c) {
                 prof::Scope ps_(l == 0 ? prof::RB0 : 1000, s);
-                if (sp)
-                    hipLaunchKernelGGL(k_rb_split<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x,
-                                       (const R *)b, (const R *)e, c, (const int *)d_ctl_);
-                else
-                    hipLaunchKernelGGL(k_rb<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x, b, e, c,
-                                       (const int *)d_ctl_);
+                const unsigned nb = nblocks(L.N / 2, 256);
+                const int *ctl = d_ctl_;
+                if (zero && it == 0 && c == 0) {
+                    if (sp)
+                        hipLaunchKernelGGL(k_rb_split_zero<R>, dim3(nb), dim3(256), 0, s, v, x, (const R *)b,
+                                           (const R *)e, ctl);
+                    else
+                        hipLaunchKernelGGL(k_rb_zero<R>, dim3(nb), dim3(256), 0, s, v, x, (const R *)b,
+                                           (const R *)e, ctl);
+                } else if (sp) {
+                    hipLaunchKernelGGL(k_rb_split<R>, dim3(nb), dim3(256), 0, s, v, x, (const R *)b, (const R *)e,
+                                       c, ctl);
+                } else {
+                    hipLaunchKernelGGL(k_rb<R>, dim3(nb), dim3(256), 0, s, v, x, b, e, c, ctl);
+                }
             }
         } else {
             hipLaunchKernelGGL(k_jacobi<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, v, (const R *)x, r,
@@ -1384,23 +1461,24 @@ void Poisson::restrict_e(hipStream_t s, const int *ctl)
 }
 
 template <typename R>
-void Poisson::vcycle(int l, hipStream_t s)
+void Poisson::vcycle(int l, hipStream_t s, bool zero)
 {
     const int nl = (int)lv_.size();
     if (l == first_coarse() && l < nl - 1) {
-        hipLaunchKernelGGL(k_coarse_vcycle<R>, dim3(1), dim3(COARSE_BLOCK), 0, s, coarse_set<R>(l),
-                           (const int *)d_ctl_);
+        CoarseSet<R> cs = coarse_set<R>(l);
+        cs.zero0 = zero ? 1 : 0;
+        hipLaunchKernelGGL(k_coarse_vcycle<R>, dim3(1), dim3(COARSE_BLOCK), 0, s, cs, (const int *)d_ctl_);
         return;
     }
     if (l == nl - 1) {
-        smooth<R>(l, nuc_, s);
+        smooth<R>(l, nuc_, s, zero);
         return;
     }
     Level &L = lv_[l], &C = lv_[l + 1];
     R *x, *b, *e, *r, *cx, *cb;
     if constexpr (std::is_same<R, double>::value) { x = L.x64; b = L.b64; e = L.e64; r = L.r64; cx = C.x64; cb = C.b64; }
     else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; cx = C.x32; cb = C.b32; }
-    smooth<R>(l, nu1_, s);
+    smooth<R>(l, nu1_, s, zero);
     LevelView<R> v = view<R>(l);
     const bool sp = split_level<R>(l);
     if (sp)
@@ -1411,9 +1489,7 @@ void Poisson::vcycle(int l, hipStream_t s)
                            (const R *)b, (const R *)e, r, (const int *)d_ctl_);
     hipLaunchKernelGGL(k_restrict<R>, dim3(nblocks(C.N, 256)), dim3(256), 0, s, (const R *)r, L.n[0],
                        L.n[1], L.n[2], cb, (const int *)d_ctl_);
-    hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(C.N, 256)), dim3(256), 0, s, cx, C.N, (R)0,
-                       (const int *)d_ctl_);
-    vcycle<R>(l + 1, s);
+    vcycle<R>(l + 1, s, true);
     if (sp)
         hipLaunchKernelGGL(k_prolong_add_split<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, (const R *)cx,
                            C.n[0], C.n[1], C.n[2], x, (const int *)d_ctl_);
@@ -1491,8 +1567,7 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
         if (krylov_) {
             bicgstab<R>(kiters_, s);
         } else {
-            hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(N, 256)), dim3(256), 0, s, x0, N, (R)0, ctl);
-            for (int c = 0; c < inner_; ++c) vcycle<R>(0, s);
+            for (int c = 0; c < inner_; ++c) vcycle<R>(0, s, c == 0);
         }
     };
     check(0);
@@ -1543,8 +1618,7 @@ void Poisson::bicgstab(int iters, hipStream_t s)
     (void)bytes;
     auto precond = [&](const R *in, R *out) {
         hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, b0, in, N, ctl);
-        hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, x0, N, (R)0, ctl);
-        vcycle<R>(0, s);
+        vcycle<R>(0, s, true);
         hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, out, (const R *)x0, N, ctl);
     };
     for (int k = 0; k < iters; ++k) {
@@ -1842,8 +1916,7 @@ int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, h
         if (krylov_) {
             bicgstab<double>(kiters_, s);
         } else {
-            hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, L0.x64, N, 0., (const int *)d_ctl_);
-            vcycle<double>(0, s);
+            vcycle<double>(0, s, true);
         }
         hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, lsx_, (const double *)L0.x64, N);
     }

@@ -151,8 +151,9 @@ def _tangent_dJ(og, F0, nd, mask, steps, dV, dM=0.0, dnd=None, dFn=None, dt=1e-3
 
 
 def test_checkpointed_run_gradients(ctx):
-    """dJ/dM, dJ/dQ and dJ/dFn0 of a short run: kept and recomputed segments
-    agree, and both match central differences and the CPU tangent run."""
+    """dJ/dM, dJ/dQ and dJ/dFn0 of a short run: the kept trajectory, kept
+    segments and per-step recomputation agree, and match central
+    differences and the CPU tangent run."""
     import kielflow_amd as kf
     print("adjoint run", flush=True)
     kind, n, steps, k = "hyst", 16, 6, 4
@@ -185,12 +186,16 @@ def test_checkpointed_run_gradients(ctx):
         return J, dM, dQ, dFn0.get()
 
     J, dM, dQ, dFn0 = run_adj()
-    assert adj.run_info()[0]
-    adj.set_capture_budget(0)
-    J2, dM2, dQ2, dFn02 = run_adj()
-    assert not adj.run_info()[0]
-    assert abs(dM2 - dM) <= 1e-9 * abs(dM) and abs(dQ2 - dQ) <= 1e-9 * abs(dQ)
-    assert np.abs(dFn02 - dFn0).max() <= 1e-9 * np.abs(dFn0).max()
+    assert adj.run_info()[0] == kf.api.KF_ADJ_TRAJECTORY
+    # room for one segment only, then for nothing
+    for budget, mode in ((k * adj.step_capture_bytes(), kf.api.KF_ADJ_SEGMENTS), (0, kf.api.KF_ADJ_RECOMPUTE)):
+        adj.set_capture_budget(budget)
+        J2, dM2, dQ2, dFn02 = run_adj()
+        assert adj.run_info()[0] == mode
+        print(f"mode {mode}: dM {dM2:.12e}/{dM:.12e} dQ {dQ2:.12e}/{dQ:.12e}", flush=True)
+        assert abs(J2 - J) <= 1e-9 * abs(J)
+        assert abs(dM2 - dM) <= 1e-9 * abs(dM) and abs(dQ2 - dQ) <= 1e-9 * abs(dQ)
+        assert np.abs(dFn02 - dFn0).max() <= 1e-9 * np.abs(dFn0).max()
     J0 = run_J()
     assert abs(J - J0) <= 1e-8 * abs(J0)
     hM, hQ = 1e-3, 1e-4
