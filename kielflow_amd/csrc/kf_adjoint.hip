@@ -110,7 +110,7 @@ __global__ void k_specT(GridView<double> g, const double2 *in, double2 *out, int
     int i, j, k;
     dec3c(idx, g.ny, g.nzc, i, j, k);
     double2 v = in[idx];
-    const double kv = -g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)] * coef;
+    const double kv = -k_along(g, dir, i, j, k) * coef;
     double2 w = mk<double2>(-kv * v.y, kv * v.x);
     if (d23 && !keep23(g, i, j, k)) w = mk<double2>(0., 0.);
     if (acc) {

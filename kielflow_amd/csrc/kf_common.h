@@ -84,6 +84,17 @@ struct GridView {
     double d23_thr;   // 2/3-rule: sqrt(s) < 2/3  <=>  s < d23_thr  (s = sum of q)
 };
 
+// wavenumber along axis dir at (i, j, kk): all three table values are
+// loaded and one is selected, so that no run-time offset into the view is
+// formed (that would force a copy of a functor holding the view into
+// scratch memory)
+template <typename R>
+__device__ __forceinline__ R k_along(const GridView<R> &g, int dir, int i, int j, int kk)
+{
+    const R a = g.k[0][i], b = g.k[1][j], c = g.k[2][kk];
+    return dir == 0 ? a : (dir == 1 ? b : c);
+}
+
 // D23 membership from per-axis squared ratios, evaluated as the reference
 // does: ((qx + qy) + qz) then compared through the exact threshold that
 // sqrt(s) < 2./3. induces (OP_dealiasing.hpp:28-29, rhs_standard.cpp:22).

@@ -32,7 +32,7 @@ __global__ void k_eff_spec(GridView<double> g, const double2 *in, double2 *out, 
     double2 v = in[idx];
     if (d23 && !keep23(g, i, j, k)) v = mk<double2>(0., 0.);
     if (filt) v = cscale(v, phi_filter(g, i, j, k));
-    if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+    if (dir >= 0) v = ik(k_along(g, dir, i, j, k), v);
     v = cscale(v, c);
     if (acc) {
         const double2 o = out[idx];

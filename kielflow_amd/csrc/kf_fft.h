@@ -58,8 +58,37 @@ struct StorePlain {
 };
 
 // ------------------------------------------------------------------ kernels
-// Strided complex pass: CB transforms of length N per workgroup.
-// Layout in LDS: [t][c] (ST = CB, SC = 1).
+// Adaptors between the engine's src.ld(t) / dst.st(t, v) and the pass
+// kernels' functors: element t of transform q, or zero / no store for a
+// transform past the end.  They refer to the kernel's functor argument
+// rather than copying it: a copy whose arrays are indexed at run time would
+// live in scratch memory.
+template <typename C, class LD>
+struct SrcQ {
+    const LD &f;
+    long q;
+    bool live;
+    __device__ __forceinline__ C ld(int t) const
+    {
+        using R = decltype(C().x);
+        return live ? f(q, t) : mk<C>((R)0, (R)0);
+    }
+};
+template <typename C, class ST_>
+struct DstQ {
+    const ST_ &f;
+    long q;
+    bool live;
+    __device__ __forceinline__ void st(int t, C v) const
+    {
+        if (live) f(q, t, v);
+    }
+};
+
+// Strided complex pass: CB transforms of length N per workgroup; thread
+// (c = tid % CB, t0 = tid / CB) works on transform q0 + c, so every global
+// access of a wave covers CB consecutive complex values of each of 64/CB
+// rows.  LDS: column layout, two buffers when they fit 32 KiB.
 template <typename R, int N, int CB, int DIR, class LD, class ST_>
 __global__ void __launch_bounds__(tpt_for(N) * CB)
 k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
@@ -67,24 +96,17 @@ k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
     using C = C2<R>;
     constexpr int TPT = tpt_for(N);
     constexpr int WG = TPT * CB;
-    __shared__ C lds[N * CB];
+    constexpr bool PING = 2 * N * CB * (int)sizeof(C) <= 32768;
+    __shared__ C lds[(PING ? 2 : 1) * N * CB];
     __shared__ C tw[N];
     const int tid = threadIdx.x;
     const long q0 = (long)xcd_block(blockIdx.x, gridDim.x) * CB;
     for (int m = tid; m < N; m += WG) tw[m] = tw_g[m];
-    // each thread owns one column c and rows t0, t0+TPT, ... (the column's
-    // address arithmetic is loop invariant)
     const int c = tid % CB, t0 = tid / CB;
     const long q = q0 + c;
     const bool live = q < Q;
-#pragma unroll
-    for (int t = t0; t < N; t += TPT) lds[t * CB + c] = live ? ld(q, t) : mk<C>((R)0, (R)0);
-    __syncthreads();
-    fft_lds<C, N, TPT, 1, CB, DIR>(lds, tw, c, t0);
-    if (live) {
-#pragma unroll
-        for (int t = t0; t < N; t += TPT) st(q, t, lds[t * CB + c]);
-    }
+    const ColBuf<C, N, CB> A{lds, c}, B{lds + (PING ? N * CB : 0), c};
+    fft_io<C, N, TPT, DIR, PING, false>(tw, t0, SrcQ<C, LD>{ld, q, live}, DstQ<C, ST_>{st, q, live}, A, B);
 }
 
 // Real rows -> half-spectrum rows (forward, unnormalised).
@@ -115,36 +137,36 @@ k_zr2c(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long ro
        ST_ st)
 {
     using C = C2<R>;
+    using Buf = RowBuf<C, M>;
     constexpr int TPT = tpt_for(M);
     constexpr int WG = TPT * RB;
-    constexpr int SC = M + 1;
-    __shared__ C lds[RB * SC];
+    __shared__ C lds[RB * Buf::SC];
     __shared__ C tw[M];
     __shared__ C twN[M + 1];  // exp(-2 pi i k / (2M)), k = 0..M
     const int tid = threadIdx.x;
     const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
     for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
     for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
-    for (int e = tid; e < RB * M; e += WG) {
-        const int c = e / M, n = e % M;
-        const long row = r0 + c;
-        lds[c * SC + n] = (row < rows) ? ld(row, n) : mk<C>((R)0, (R)0);
-    }
+    // row c's Nz/2-point transform by threads tt = 0..TPT-1, result in LDS
+    const int c = tid / TPT, tt = tid % TPT;
+    const long row = r0 + c;
+    const Buf A{lds, c};
+    fft_io<C, M, TPT, -1, false, true>(tw, tt, SrcQ<C, LD>{ld, row, row < rows}, A, A, A);
     __syncthreads();
-    fft_lds<C, M, TPT, SC, 1, -1>(lds, tw, tid / TPT, tid % TPT);
     // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k])
     for (int e = tid; e < RB * (M + 1); e += WG) {
-        const int c = e / (M + 1), k = e % (M + 1);
-        const long row = r0 + c;
-        if (row < rows) {
-            C zk = lds[c * SC + (k == M ? 0 : k)];
-            C zm = cconj(lds[c * SC + (k == 0 ? 0 : M - k)]);
+        const int cc = e / (M + 1), k = e % (M + 1);
+        const long rr = r0 + cc;
+        if (rr < rows) {
+            const Buf Z{lds, cc};
+            C zk = Z.ld(k == M ? 0 : k);
+            C zm = cconj(Z.ld(k == 0 ? 0 : M - k));
             C ev = mk<C>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y + zm.y));
             C od = mk<C>((R)0.5 * (zk.x - zm.x), (R)0.5 * (zk.y - zm.y));
             // -i * W^k * od
             C wod = cmul(twN[k], od);
             C X = mk<C>(ev.x + wod.y, ev.y - wod.x);
-            st(row, k, X);
+            st(rr, k, X);
         }
     }
 }
@@ -174,47 +196,50 @@ struct RowStoreReal {
     }
 };
 
+// C2R merge at load: Z[k] = (X[k] + conj X[M-k]) + i conj(W^k) (X[k] - conj X[M-k])
+// for k = 0..M-1, with Im X_0 and Im X_M dropped
+template <typename C, int M, class LD>
+struct MergeLoad {
+    const LD &f;
+    long row;
+    bool live;
+    const C *__restrict__ twN;
+    __device__ __forceinline__ C ld(int k) const
+    {
+        using R = decltype(C().x);
+        if (!live) return mk<C>((R)0, (R)0);
+        C xk = f(row, k), xm = f(row, M - k);
+        if (k == 0) {
+            xk.y = (R)0;
+            xm.y = (R)0;
+        }
+        xm = cconj(xm);
+        const C ev = cadd(xk, xm);
+        const C od = cmul(cconj(twN[k]), csub(xk, xm));
+        return mk<C>(ev.x - od.y, ev.y + od.x);
+    }
+};
+
 template <typename R, int M, int RB, class LD, class ST_>
 __global__ void __launch_bounds__(tpt_for(M) * RB)
 k_zc2r(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, LD ld,
        ST_ st)
 {
     using C = C2<R>;
+    using Buf = RowBuf<C, M>;
     constexpr int TPT = tpt_for(M);
     constexpr int WG = TPT * RB;
-    constexpr int SC = M + 1;
-    __shared__ C xs[RB * SC];
-    __shared__ C lds[RB * SC];
+    __shared__ C lds[RB * Buf::SC];
     __shared__ C tw[M];
-    __shared__ C twN[M + 1];
     const int tid = threadIdx.x;
     const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
     for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
-    for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
-    for (int e = tid; e < RB * (M + 1); e += WG) {
-        const int c = e / (M + 1), k = e % (M + 1);
-        const long row = r0 + c;
-        C v = (row < rows) ? ld(row, k) : mk<C>((R)0, (R)0);
-        if (k == 0 || k == M) v.y = (R)0;
-        xs[c * SC + k] = v;
-    }
-    __syncthreads();
-    // merge: Z[k] = (X[k] + conj X[M-k]) + i conj(W^k) (X[k] - conj X[M-k])
-    for (int e = tid; e < RB * M; e += WG) {
-        const int c = e / M, k = e % M;
-        C xk = xs[c * SC + k];
-        C xm = cconj(xs[c * SC + (M - k)]);
-        C ev = cadd(xk, xm);
-        C od = cmul(cconj(twN[k]), csub(xk, xm));
-        lds[c * SC + k] = mk<C>(ev.x - od.y, ev.y + od.x);
-    }
-    __syncthreads();
-    fft_lds<C, M, TPT, SC, 1, +1>(lds, tw, tid / TPT, tid % TPT);
-    for (int e = tid; e < RB * M; e += WG) {
-        const int c = e / M, n = e % M;
-        const long row = r0 + c;
-        if (row < rows) st(row, n, lds[c * SC + n]);
-    }
+    const int c = tid / TPT, tt = tid % TPT;
+    const long row = r0 + c;
+    const bool live = row < rows;
+    const Buf A{lds, c};
+    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, LD>{ld, row, live, twN_g},
+                                        DstQ<C, ST_>{st, row, live}, A, A);
 }
 
 // ------------------------------------------------------------------ plan
@@ -316,16 +341,6 @@ void launch_zr2c(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST
     }
 }
 
-// rows per workgroup for the C2R z pass, which stages two row buffers
-template <typename R>
-__host__ __device__ constexpr int rb_c2r_for(int m)
-{
-    int lim = 24576 / ((m + 1) * (int)sizeof(C2<R>));
-    int v = cb_for<R>(m);
-    while (v > lim && v > 1) v /= 2;
-    return v;
-}
-
 template <typename R, class LD, class ST_>
 void launch_zc2r(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST_ st,
                  hipStream_t s)
@@ -334,7 +349,7 @@ void launch_zc2r(int m, long rows, const C2<R> *twm, const C2<R> *twn, LD ld, ST
     switch (m) {
 #define KF_CASE(NN)                                                                       \
     case NN: {                                                                            \
-        constexpr int RB = rb_c2r_for<R>(NN);                                                 \
+        constexpr int RB = cb_for<R>(NN);                                                 \
         dim3 grid((unsigned)((rows + RB - 1) / RB));                                      \
         hipLaunchKernelGGL((k_zc2r<R, NN, RB, LD, ST_>), grid, dim3(tpt_for(NN) * RB), 0, s, \
                            twm, twn, rows, ld, st);                                       \

@@ -1,15 +1,14 @@
 // kf_fft_engine.h -- LDS-resident batched 1D FFT for gfx950.
 //
-// A workgroup holds CB transforms of length N in LDS.  TPT threads cooperate
-// on one transform; each owns E = N/TPT elements per pass.  Passes are
+// A workgroup runs several transforms of length N; TPT threads cooperate on
+// one transform and each owns about N/TPT elements per pass.  Passes are
 // Stockham autosort steps (natural order in, natural order out) with radices
-// 8, 4, 2, 3 chosen at compile time.  Twiddles come from a per-length table
-// w[m] = exp(-2 pi i m / N) held in LDS (loaded from a host-computed FP64
-// table, so FP64 transforms carry correctly rounded twiddles).
-//
-// Element (transform c, position t) lives at lds[c*SC + t*ST]; SC/ST are the
-// layout strides the caller chose (row layout for contiguous z transforms,
-// column layout for strided x/y transforms).
+// 8, 4, 2, 3 chosen at compile time.  The first pass takes its inputs
+// straight from global memory into registers and the last pass stores
+// straight from registers, so only the passes in between go through LDS.
+// Twiddles come from a per-length table w[m] = exp(-2 pi i m / N) held in
+// LDS (loaded from a host-computed FP64 table, so FP64 transforms carry
+// correctly rounded twiddles).
 #pragma once
 
 #include "kf_common.h"
@@ -104,13 +103,18 @@ __device__ __forceinline__ void dft(C *v)
     else static_assert(R_ == 2, "unsupported radix");
 }
 
-// ---------------------------------------------------------------- one pass
+// ---------------------------------------------------------------- passes
 // Stockham step of radix R_ at stride NS (product of earlier radices).
 //   butterfly j (0 <= j < N/R_):  inputs  x[j + q*N/R_]
 //   twiddle  w^(q * (j mod NS) * N/(NS*R_))
 //   outputs  y[(j/NS)*NS*R_ + (j mod NS) + q*NS]
-template <typename C, int N, int R_, int NS, int TPT, int SC, int ST, int DIR>
-__device__ __forceinline__ void pass(C *lds, const C *tw, int c, int tt)
+// Inputs come from src.ld(t), outputs go to dst.st(t, v): global memory (the
+// load/store functors of the kernel) for the first and the last pass, an LDS
+// buffer in between.  All of a thread's loads are issued before its
+// arithmetic.  SYNC_MID: src and dst are the same LDS buffer, so every thread
+// must finish reading before any writes.
+template <typename C, int N, int R_, int NS, int TPT, int DIR, bool SYNC_MID, class SRC, class DST>
+__device__ __forceinline__ void pass_io(const C *tw, int tt, const SRC &src, const DST &dst)
 {
     constexpr int NB = N / R_;
     constexpr int MB = (NB + TPT - 1) / TPT;
@@ -120,7 +124,13 @@ __device__ __forceinline__ void pass(C *lds, const C *tw, int c, int tt)
         const int j = tt + m * TPT;
         if (NB % TPT == 0 || j < NB) {
 #pragma unroll
-            for (int q = 0; q < R_; ++q) v[m][q] = lds[c * SC + (j + q * NB) * ST];
+            for (int q = 0; q < R_; ++q) v[m][q] = src.ld(j + q * NB);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MB; ++m) {
+        const int j = tt + m * TPT;
+        if (NB % TPT == 0 || j < NB) {
             if constexpr (NS > 1) {
                 const int kk = j % NS;
 #pragma unroll
@@ -133,36 +143,128 @@ __device__ __forceinline__ void pass(C *lds, const C *tw, int c, int tt)
             dft<R_, DIR>(v[m]);
         }
     }
-    __syncthreads();
+    if constexpr (SYNC_MID) __syncthreads();
 #pragma unroll
     for (int m = 0; m < MB; ++m) {
         const int j = tt + m * TPT;
         if (NB % TPT == 0 || j < NB) {
             const int base = (j / NS) * NS * R_ + (j % NS);
 #pragma unroll
-            for (int q = 0; q < R_; ++q) lds[c * SC + (base + q * NS) * ST] = v[m][q];
+            for (int q = 0; q < R_; ++q) dst.st(base + q * NS, v[m][q]);
         }
     }
-    __syncthreads();
 }
 
-template <typename C, int N, int REM, int NS, int TPT, int SC, int ST, int DIR>
-__device__ __forceinline__ void passes(C *lds, const C *tw, int c, int tt)
+// The passes of a length-N transform.  The first reads src, the last writes
+// dst; in between, data moves through LDS buffers `in`/`out` (PING: two
+// buffers used alternately, one barrier per pass; else one buffer, two
+// barriers per pass).  LAST_LDS: dst is the single LDS buffer itself.
+// The first pass uses no twiddles, so a table the caller fills in LDS is
+// read only after the first barrier.  Returns with every LDS write of the
+// chain issued but not synchronised.
+template <typename C, int N, int REM, int NS, int TPT, int DIR, bool PING, bool LAST_LDS, class SRC, class DST,
+          class BUF>
+__device__ __forceinline__ void chain_io(const C *tw, int tt, const SRC &src, const DST &dst, const BUF &in,
+                                         const BUF &out)
 {
-    if constexpr (REM > 1) {
-        constexpr int R_ = next_radix(REM);
-        pass<C, N, R_, NS, TPT, SC, ST, DIR>(lds, tw, c, tt);
-        passes<C, N, REM / R_, NS * R_, TPT, SC, ST, DIR>(lds, tw, c, tt);
+    static_assert(!(PING && LAST_LDS), "an LDS destination needs the single-buffer chain");
+    constexpr int R_ = next_radix(REM);
+    constexpr bool first = NS == 1, last = REM == R_;
+    if constexpr (first && last) {
+        pass_io<C, N, R_, NS, TPT, DIR, false>(tw, tt, src, dst);
+    } else if constexpr (first) {
+        pass_io<C, N, R_, NS, TPT, DIR, false>(tw, tt, src, out);
+        __syncthreads();
+        chain_io<C, N, REM / R_, NS * R_, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, out, PING ? in : out);
+    } else if constexpr (last) {
+        pass_io<C, N, R_, NS, TPT, DIR, LAST_LDS>(tw, tt, in, dst);
+    } else {
+        pass_io<C, N, R_, NS, TPT, DIR, !PING>(tw, tt, in, out);
+        __syncthreads();
+        chain_io<C, N, REM / R_, NS * R_, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, out, PING ? in : out);
     }
 }
 
-// Full length-N transform of transform c by thread tt (0..TPT-1) of its group.
-// Caller must __syncthreads() after filling LDS; this returns synchronised.
-template <typename C, int N, int TPT, int SC, int ST, int DIR>
-__device__ __forceinline__ void fft_lds(C *lds, const C *tw, int c, int tt)
+// Transform of length N by TPT threads (this one is tt): see chain_io.
+// A and B are the LDS buffers (the same buffer unless PING).
+template <typename C, int N, int TPT, int DIR, bool PING, bool LAST_LDS, class SRC, class DST, class BUF>
+__device__ __forceinline__ void fft_io(const C *tw, int tt, const SRC &src, const DST &dst, const BUF &A,
+                                       const BUF &B)
 {
-    passes<C, N, N, 1, TPT, SC, ST, DIR>(lds, tw, c, tt);
+    chain_io<C, N, N, 1, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, B, A);
 }
+
+// ---------------------------------------------------------------- LDS layouts
+// Bank-conflict-avoiding permutations of the positions t of one transform:
+// t ^ ((t >> A) & MASK) (a bijection since MASK < 2^A), chosen per length
+// and layout with a model of the gfx950 LDS banks over every pass's reads
+// and writes.
+__host__ __device__ constexpr int swz(int t, int a, int mask) { return t ^ ((t >> a) & mask); }
+
+// column layout (strided passes): element t of column c at t*CB + c;
+// RBYTES = CB * sizeof(C), the bytes of one LDS row
+__host__ __device__ constexpr int col_swz_a(int n, int rbytes)
+{
+    return (n >= 128 && rbytes == 128) || (n == 256 && rbytes == 64) ? 3 : 1;
+}
+__host__ __device__ constexpr int col_swz_mask(int n, int rbytes)
+{
+    return (n >= 128 && rbytes == 128) ? 1 : (n == 256 && rbytes == 64) ? 3 : 0;
+}
+
+template <typename C, int N, int CB>
+struct ColBuf {
+    C *p;
+    int c;
+    static constexpr int A = col_swz_a(N, CB * (int)sizeof(C)), MASK = col_swz_mask(N, CB * (int)sizeof(C));
+    __device__ __forceinline__ C ld(int t) const { return p[swz(t, A, MASK) * CB + c]; }
+    __device__ __forceinline__ void st(int t, C v) const { p[swz(t, A, MASK) * CB + c] = v; }
+};
+
+// row layout (z passes): element t of row c at c*SC + swz(t); {pad, A, MASK}
+// per half-length M and complex size
+struct RowLay {
+    int pad, a, mask;
+};
+__host__ __device__ constexpr RowLay row_layout(int m, int csize)
+{
+    if (csize == 8) {
+        switch (m) {
+        case 12: return {2, 2, 1};
+        case 16: return {2, 3, 1};
+        case 24: return {2, 3, 1};
+        case 32: return {4, 3, 3};
+        case 48: return {6, 5, 1};
+        case 64: return {8, 3, 7};
+        case 96: return {8, 3, 7};
+        case 128: return {8, 4, 7};
+        case 192: return {8, 4, 7};
+        default: return {1, 1, 0};
+        }
+    }
+    switch (m) {
+    case 12: return {2, 2, 1};
+    case 16: return {2, 3, 1};
+    case 24: return {3, 4, 1};
+    case 32: return {4, 3, 3};
+    case 48: return {6, 4, 1};
+    case 64: return {8, 3, 7};
+    case 96: return {4, 3, 7};
+    case 128: return {0, 3, 7};
+    case 192: return {8, 3, 7};
+    default: return {1, 1, 0};
+    }
+}
+
+template <typename C, int M>
+struct RowBuf {
+    C *p;
+    int c;
+    static constexpr RowLay L = row_layout(M, (int)sizeof(C));
+    static constexpr int SC = M + L.pad;
+    __device__ __forceinline__ C ld(int t) const { return p[c * SC + swz(t, L.a, L.mask)]; }
+    __device__ __forceinline__ void st(int t, C v) const { p[c * SC + swz(t, L.a, L.mask)] = v; }
+};
 
 }  // namespace fft
 }  // namespace kf

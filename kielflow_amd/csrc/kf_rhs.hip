@@ -38,7 +38,7 @@ __global__ void k_spec(GridView<R> g, const C2<R> *__restrict__ in, C2<R> *out, 
         const R f = phi_filter(g, i, j, k);
         v = mk<C2<R>>(v.x * f, v.y * f);
     }
-    if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+    if (dir >= 0) v = ik(k_along(g, dir, i, j, k), v);
     if (acc) {
         C2<R> o = out[idx];
         v = mk<C2<R>>(o.x + v.x, o.y + v.y);

@@ -87,7 +87,7 @@ struct XLoadSpec {
         decode_x(g, q, t, i, j, k, idx);
         C2<R> v = src[idx];
         if (d23 && !keep23(g, i, j, k)) return mk<C2<R>>((R)0, (R)0);
-        if (dir >= 0) v = ik(g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+        if (dir >= 0) v = ik(k_along(g, dir, i, j, k), v);
         if (w) v = cscale(v, wplane(g, k));
         if (filt) v = cscale(v, phi_filter(g, i, j, k));
         return v;
@@ -184,6 +184,21 @@ struct XStore {
     // XK_ADJ: o0 (+)= coef * [D23] * [-i k_dir] * W'/N * X   (adjoint of C2R)
     int d23, dir, acc;
     R coef, invN;
+    // linear terms of velocity component a (rhs_standard.cpp:119-140,
+    // 219-247, 279-310) from its spectrum u; adds i k_a [D23] u to ln
+    __device__ __forceinline__ C2<R> lin_u(C2<R> u, R fac, C2<R> F1, bool kp, R kx, R ka, C2<R> &ln) const
+    {
+        using C = C2<R>;
+        C l = mk<C>(fac * u.x, fac * u.y);
+        const C dF1 = ik(ka, F1);
+        l = mk<C>(l.x + dF1.x, l.y + dF1.y);
+        const C ud = kp ? u : mk<C>((R)0, (R)0);
+        const C dx = ik(kx, ud);
+        l = mk<C>(l.x + M * dx.x, l.y + M * dx.y);
+        const C da = ik(ka, ud);
+        ln = mk<C>(ln.x + da.x, ln.y + da.y);
+        return l;
+    }
     __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const
     {
         using C = C2<R>;
@@ -196,7 +211,7 @@ struct XStore {
             if (!d23 || keep23(g, i, j, k)) {
                 const R f = coef * invN / wplane(g, k);
                 v = cscale(X, f);
-                if (dir >= 0) v = ik(-g.k[dir][dir == 0 ? i : (dir == 1 ? j : k)], v);
+                if (dir >= 0) v = ik(-k_along(g, dir, i, j, k), v);
             }
             if (acc) {
                 C o = o0[idx];
@@ -235,10 +250,6 @@ struct XStore {
             const bool kp = keep23(g, i, j, k);
             const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
             const R kx = g.k[0][i], ky = g.k[1][j], kz = g.k[2][k];
-            const C *FU[3] = {a0, a1, a2};
-            C *LO[3] = {o0, o1, o2};
-            const R ka[3] = {kx, ky, kz};
-            C divu = mk<C>((R)0, (R)0);
             C dnx;
             {
                 const C fd = kp ? fn : mk<C>((R)0, (R)0);
@@ -246,23 +257,10 @@ struct XStore {
             }
             C ln = sv_on ? mk<C>(sv * fn.x, sv * fn.y) : mk<C>((R)0, (R)0);
             ln = mk<C>(ln.x + M * dnx.x, ln.y + M * dnx.y);
-#pragma unroll
-            for (int a = 0; a < 3; ++a) {
-                const C u = FU[a][idx];
-                const R fac = sv_on ? (tau + sv) : tau;
-                C l = mk<C>(fac * u.x, fac * u.y);
-                const C dF1 = ik(ka[a], F1);
-                l = mk<C>(l.x + dF1.x, l.y + dF1.y);
-                const C ud = kp ? u : mk<C>((R)0, (R)0);
-                const C dx = ik(kx, ud);
-                l = mk<C>(l.x + M * dx.x, l.y + M * dx.y);
-                LO[a][idx] = l;
-                const C da = ik(ka[a], ud);
-                divu = (a == 0) ? da : mk<C>(divu.x + da.x, divu.y + da.y);
-                if (a == 0) ln = mk<C>(ln.x + da.x, ln.y + da.y);
-                else ln = mk<C>(ln.x + da.x, ln.y + da.y);
-            }
-            (void)divu;
+            const R fac = sv_on ? (tau + sv) : tau;
+            o0[idx] = lin_u(a0[idx], fac, F1, kp, kx, kx, ln);
+            o1[idx] = lin_u(a1[idx], fac, F1, kp, kx, ky, ln);
+            o2[idx] = lin_u(a2[idx], fac, F1, kp, kx, kz, ln);
             o3[idx] = ln;
             break;
         }
