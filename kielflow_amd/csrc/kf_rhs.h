@@ -84,7 +84,8 @@ class Rhs {
   private:
     void eval_fused(C2<R> *F[4]);
     void eval_general(C2<R> *F[4]);
-    void poisson_step(C2<R> *Fn);
+    // rho from Fn (or from its staged x/y passes), then the Poisson solve
+    void poisson_step(const C2<R> *Fn, const C2<R> *staged = nullptr);
     void output(C2<R> *F[4]);
     void deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accumulate);
 
@@ -105,6 +106,7 @@ class Rhs {
     C2<R> *lin_[4] = {nullptr, nullptr, nullptr, nullptr};
     C2<R> *F1_ = nullptr;
     C2<R> *dF_[12] = {};
+    C2<R> *H_[4][3] = {};       // staged x/y passes of the state fields (equidistant grids)
     int stage_ = -1;
     Capture *cap_ = nullptr;
     Capture *cap_stages_ = nullptr;

@@ -1,10 +1,13 @@
 // kf_rhs.hip -- RHS (rhs_standard.cpp:89-566) and RK4 (RKO4.cpp:50-248).
 //
-// The equidistant-grid RHS fuses every spectral multiplier into the first
-// pass of an inverse transform and every real-space product, exp, and
-// penalization blend into the last pass of the transform that produces it;
-// 25 transforms per evaluation (the reference's 33 minus 8 that repeat an
-// inverse already computed or undo a forward transform).  Stretched axes
+// The equidistant-grid RHS fuses every spectral multiplier into a pass of an
+// inverse transform and every real-space product, exp, and penalization
+// blend into the last pass of the transform that produces it; 25 transforms
+// per evaluation (the reference's 33 minus 8 that repeat an inverse already
+// computed or undo a forward transform).  The 16 inverse transforms of the
+// state fields and their derivatives share x and y passes: per field, two x
+// passes (D23 F and i k_x D23 F) and three y passes serve all four z passes.
+// Stretched axes
 // take the general path, which materialises the stretched derivatives
 // d_b X = FFT(dS_b * iFFT(D23 i k_b X)) as the reference does
 // (OP_partial_derivative.cpp:104-136).
@@ -154,6 +157,9 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     for (int a = 0; a < 4; ++a) adv_[a] = ralloc<R>(nr);
     for (int a = 0; a < 4; ++a) lin_[a] = ralloc<C2<R>>(nc);
     if (!std::is_same<R, double>::value) phiR_ = ralloc<R>(nr);
+    if (!g.any_stretched())
+        for (auto &f : H_)
+            for (auto &h : f) h = ralloc<C2<R>>((long)plan_.nx * plan_.ny * plan_.nzs);
     KF_HIP(hipMemsetAsync(nd_, 0, sizeof(R) * nr, s));
     KF_HIP(hipMemsetAsync(mask_, 0, sizeof(R) * nr, s));
     KF_HIP(hipStreamSynchronize(s));
@@ -175,6 +181,8 @@ Rhs<R>::~Rhs()
     }
     (void)hipFree(F1_);
     for (auto &d : dF_) (void)hipFree(d);
+    for (auto &f : H_)
+        for (auto &h : f) (void)hipFree(h);
     (void)hipFree(phiR_);
     for (auto &st : ph_)
         for (auto &p : st) (void)hipFree(p);
@@ -232,13 +240,14 @@ void Rhs<R>::set_stage(int s)
 
 // rhs_standard.cpp:100-113: rho = exp(iFFT(D23 Fn)) + n_d, then Poisson
 template <typename R>
-void Rhs<R>::poisson_step(C2<R> *Fn)
+void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
 {
     ZStore<R> z = zs<R>(ZK_RHO, plan_);
     z.o0 = L_;
     z.o1 = rho_;
     z.a0 = nd_;
-    c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
+    if (staged) z_staged<R>(g_, plan_, staged, false, z, s_);
+    else c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
     if (!phi_) throw Error("RHS potential field not set");
     const long nr = g_.nr;
     if (stage_ >= 0 && hc_[stage_] >= 2) {
@@ -296,7 +305,10 @@ void Rhs<R>::output(C2<R> *F[4])
 template <typename R>
 void Rhs<R>::eval_fused(C2<R> *F[4])
 {
-    poisson_step(F[3]);
+    // x and y passes of every state field, shared by its value and its
+    // three derivatives
+    for (int a = 0; a < 4; ++a) stage_xy<R>(g_, plan_, F[a], H_[a][0], H_[a][1], H_[a][2], s_);
+    poisson_step(nullptr, H_[3][0]);
     // FFT(Phi) -> filter -> + Fn/theta -> all linear terms (4 outputs)
     XStore<R> x = xs<R>(XK_LIN, g_);
     x.o0 = lin_[0];
@@ -316,16 +328,18 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
     for (int b = 0; b < 3; ++b) {
         ZStore<R> z = zs<R>(ZK_WRITE, plan_);
         z.o0 = U_[b];
-        c2r_spec<R>(g_, plan_, F[b], 1, -1, z, s_);
+        z_staged<R>(g_, plan_, H_[b][0], false, z, s_);
     }
-    // advection / continuity products (rhs_standard.cpp:323-343, 403-437)
+    // advection / continuity products (rhs_standard.cpp:323-343, 403-437):
+    // adv_a = sum_b U_b dF_a/db
     for (int a = 0; a < 4; ++a)
         for (int b = 0; b < 3; ++b) {
             ZStore<R> z = zs<R>(ZK_PROD, plan_);
             z.o0 = adv_[a];
             z.a0 = U_[b];
             z.flag = (b > 0);
-            c2r_spec<R>(g_, plan_, F[a], 1, b, z, s_);
+            const C2<R> *h = b == 0 ? H_[a][1] : (b == 1 ? H_[a][2] : H_[a][0]);
+            z_staged<R>(g_, plan_, h, b == 2, z, s_);
         }
     output(F);
 }

@@ -293,6 +293,65 @@ inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
     launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzs}, st, s);
 }
 
+// y pass load of the scratch layout with i k_y (the y index is t) applied
+template <typename R>
+struct LoadMaskedK {
+    const C2<R> *p;
+    StridedAddr a;
+    unsigned nvalid;
+    const R *k;
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const
+    {
+        if ((unsigned)q % (unsigned)a.qdiv >= nvalid) return mk<C2<R>>((R)0, (R)0);
+        return ik(k[t], p[a(q, t)]);
+    }
+};
+
+// z pass load of half-spectrum rows of the scratch layout with i k_z applied
+template <typename R>
+struct RowLoadCK {
+    const C2<R> *p;
+    int nzs;
+    const R *k;
+    __device__ __forceinline__ C2<R> operator()(long row, int kk) const { return ik(k[kk], p[row * nzs + kk]); }
+};
+
+// The x and y passes of the inverse transforms of a state field F and its
+// x and y derivatives, in the scratch layout:
+//   H  = FFT_y FFT_x (D23 F)
+//   Hx = FFT_y FFT_x (i k_x D23 F)
+//   Hy = FFT_y (i k_y FFT_x (D23 F))      (i k_y commutes with FFT_x)
+// A z pass of H gives F in real space, of H with i k_z at its load dF/dz,
+// of Hx dF/dx and of Hy dF/dy (OP_partial_derivative.cpp:104-116 on an
+// equidistant grid, dealiased first as at rhs_standard.cpp:253-256).
+template <typename R>
+inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C2<R> *Hy, hipStream_t s)
+{
+    GridView<R> gv = g.view<R>();
+    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
+    const long qx = (long)p.ny * p.nzc, qy = scratch_cols_y(p);
+    const unsigned nv = (unsigned)p.nzc;
+    launch_strided<R, +1>(p.nx, qx, p.tw_x, XLoadSpec<R>{F, gv, 1, -1}, StorePlain<R>{H, ax}, s, prof::FFT_X, p.vx);
+    launch_strided<R, +1>(p.nx, qx, p.tw_x, XLoadSpec<R>{F, gv, 1, 0}, StorePlain<R>{Hx, ax}, s, prof::FFT_X, p.vx);
+    launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMaskedK<R>{H, ay, nv, gv.k[1]}, StoreMasked<R>{Hy, ay, nv}, s,
+                          prof::FFT_Y, p.vy);
+    launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv}, s, prof::FFT_Y,
+                          p.vy);
+    launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{Hx, ay, nv}, StoreMasked<R>{Hx, ay, nv}, s,
+                          prof::FFT_Y, p.vy);
+}
+
+// z pass of a staged field into a ZStore; with i k_z when dz
+template <typename R>
+inline void z_staged(Grid &g, Plan<R> &p, const C2<R> *H, bool dz, const ZStore<R> &st, hipStream_t s)
+{
+    const long rows = (long)p.nx * p.ny;
+    if (dz)
+        launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadCK<R>{H, p.nzs, g.view<R>().k[2]}, st, s);
+    else
+        launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{H, p.nzs}, st, s);
+}
+
 template <typename R, typename RS>
 inline void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, hipStream_t s)
 {
