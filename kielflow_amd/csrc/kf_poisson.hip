@@ -920,55 +920,85 @@ __device__ void blk_smooth(const LevelView<R> &L, R *x, const R *b, const R *e, 
     }
 }
 
-template <typename R>
+// points of all coarse levels together: every level has 1/8 of the points
+// of the one above it, so the sum stays below 8/7 of the first
+constexpr long COARSE_LDS_PTS = COARSE_N + COARSE_N / 7 + 8;
+
+// The coarse V-cycle.  IN_LDS: x and b of every coarse level and the
+// residual live in LDS (the first level's b is read from, and its x written
+// back to, global memory), so the ~100 barrier-separated phases of the
+// cycle wait on LDS rather than on L2; e stays in global memory (read-only).
+template <typename R, bool IN_LDS>
 __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs, const int *ctl)
 {
+    __shared__ R sx[IN_LDS ? COARSE_LDS_PTS : 1], sb[IN_LDS ? COARSE_LDS_PTS : 1], sr[IN_LDS ? COARSE_N : 1];
+    __shared__ long soff[COARSE_MAXL + 1];
     if (ctl[0]) return;
     const int tid = threadIdx.x, bs = blockDim.x;
     const int nl = cs.nl;
-    if (cs.zero0) {
+    if (IN_LDS && tid == 0) {
+        long o = 0;
+        for (int l = 0; l < nl; ++l) {
+            soff[l] = o;
+            o += cs.L[l].N;
+        }
+    }
+#define CX(l) (IN_LDS ? sx + soff[l] : cs.x[l])
+#define CB(l) (IN_LDS ? sb + soff[l] : cs.b[l])
+#define CR(l) (IN_LDS ? sr : cs.r[l])
+    if (IN_LDS) {
+        for (long p = tid; p < cs.L[0].N; p += bs) {
+            sb[p] = cs.b[0][p];
+            sx[p] = cs.zero0 ? (R)0 : cs.x[0][p];
+        }
+        __syncthreads();
+    } else if (cs.zero0) {
         for (long p = tid; p < cs.L[0].N; p += bs) cs.x[0][p] = (R)0;
         __syncthreads();
     }
     for (int l = 0; l < nl - 1; ++l) {
         const LevelView<R> &L = cs.L[l];
-        blk_smooth<R>(L, cs.x[l], cs.b[l], cs.e[l], cs.r[l], cs.nu1);
+        R *x = CX(l), *b = CB(l), *r = CR(l);
+        const R *e = cs.e[l];
+        blk_smooth<R>(L, x, b, e, r, cs.nu1);
         const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
         for (long p = tid; p < L.N; p += bs) {
             int i, j, k;
-        dec3(p, ny, nz, i, j, k);
+            dec3(p, ny, nz, i, j, k);
             R A, D;
-            stencil<R>(L, cs.x[l], i, j, k, A, D);
-            cs.r[l][p] = cs.b[l][p] - (A - D * cs.x[l][p] - cs.e[l][p] * cs.x[l][p]);
+            stencil<R>(L, x, i, j, k, A, D);
+            r[p] = b[p] - (A - D * x[p] - e[p] * x[p]);
         }
         __syncthreads();
         const LevelView<R> &C = cs.L[l + 1];
+        R *xc = CX(l + 1), *bc = CB(l + 1);
         const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
         const R w[3] = {(R)0.25, (R)0.5, (R)0.25};
         for (long p = tid; p < C.N; p += bs) {
             int i, j, k;
-        dec3(p, cy, cz, i, j, k);
+            dec3(p, cy, cz, i, j, k);
             R acc = 0;
             for (int a = -1; a <= 1; ++a)
                 for (int bb = -1; bb <= 1; ++bb)
                     for (int cc = -1; cc <= 1; ++cc) {
                         const int fi = wrapi(2 * i + a, nx), fj = wrapi(2 * j + bb, ny), fk = wrapi(2 * k + cc, nz);
-                        acc += w[a + 1] * w[bb + 1] * w[cc + 1] * cs.r[l][((long)fi * ny + fj) * nz + fk];
+                        acc += w[a + 1] * w[bb + 1] * w[cc + 1] * r[((long)fi * ny + fj) * nz + fk];
                     }
-            cs.b[l + 1][p] = acc;
-            cs.x[l + 1][p] = (R)0;
+            bc[p] = acc;
+            xc[p] = (R)0;
         }
         __syncthreads();
     }
-    blk_smooth<R>(cs.L[nl - 1], cs.x[nl - 1], cs.b[nl - 1], cs.e[nl - 1], cs.r[nl - 1], cs.nuc);
+    blk_smooth<R>(cs.L[nl - 1], CX(nl - 1), CB(nl - 1), cs.e[nl - 1], CR(nl - 1), cs.nuc);
     for (int l = nl - 2; l >= 0; --l) {
         const LevelView<R> &L = cs.L[l], &C = cs.L[l + 1];
         const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
         const int fy = L.n[1], fz = L.n[2];
-        const R *c = cs.x[l + 1];
+        const R *c = CX(l + 1);
+        R *x = CX(l);
         for (long p = tid; p < L.N; p += bs) {
             int i, j, k;
-        dec3(p, fy, fz, i, j, k);
+            dec3(p, fy, fz, i, j, k);
             const int i0 = i >> 1, j0 = j >> 1, k0 = k >> 1;
             const int i1 = wrapi(i0 + 1, cx), j1 = wrapi(j0 + 1, cy), k1 = wrapi(k0 + 1, cz);
             const R wi1 = (i & 1) ? (R)0.5 : (R)0, wj1 = (j & 1) ? (R)0.5 : (R)0, wk1 = (k & 1) ? (R)0.5 : (R)0;
@@ -979,11 +1009,17 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs,
                   wi1 * (wj0 * (wk0 * CV(i1, j0, k0) + wk1 * CV(i1, j0, k1)) +
                          wj1 * (wk0 * CV(i1, j1, k0) + wk1 * CV(i1, j1, k1)));
 #undef CV
-            cs.x[l][p] += v;
+            x[p] += v;
         }
         __syncthreads();
-        blk_smooth<R>(L, cs.x[l], cs.b[l], cs.e[l], cs.r[l], cs.nu2);
+        blk_smooth<R>(L, x, CB(l), cs.e[l], CR(l), cs.nu2);
     }
+    if (IN_LDS) {
+        for (long p = tid; p < cs.L[0].N; p += bs) cs.x[0][p] = sx[p];
+    }
+#undef CX
+#undef CB
+#undef CR
 }
 
 // restriction of the coefficient e through the coarse levels, one workgroup
@@ -1467,7 +1503,12 @@ void Poisson::vcycle(int l, hipStream_t s, bool zero)
     if (l == first_coarse() && l < nl - 1) {
         CoarseSet<R> cs = coarse_set<R>(l);
         cs.zero0 = zero ? 1 : 0;
-        hipLaunchKernelGGL(k_coarse_vcycle<R>, dim3(1), dim3(COARSE_BLOCK), 0, s, cs, (const int *)d_ctl_);
+        long pts = 0;
+        for (int m = 0; m < cs.nl; ++m) pts += cs.L[m].N;
+        if (pts <= COARSE_LDS_PTS)
+            hipLaunchKernelGGL((k_coarse_vcycle<R, true>), dim3(1), dim3(COARSE_BLOCK), 0, s, cs, (const int *)d_ctl_);
+        else
+            hipLaunchKernelGGL((k_coarse_vcycle<R, false>), dim3(1), dim3(COARSE_BLOCK), 0, s, cs, (const int *)d_ctl_);
         return;
     }
     if (l == nl - 1) {
