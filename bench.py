@@ -198,7 +198,7 @@ def adjoint_cost(kf, ctx, n=64, steps=200, k=20, m=3.0):
             "intermediates": kept, "transposed_poisson_iterations": lin_it}
 
 
-def batch_sweep(kf, ctx, nbox=64, steps=100, warm=3, seed=0, nstreams=4, poisson_iterations=6, check_boxes=4):
+def batch_sweep(kf, ctx, nbox=64, steps=100, warm=4, seed=0, nstreams=4, poisson_iterations=6, check_boxes=4):
     """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
     M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
     precision, one captured graph per box and step over `nstreams` streams.
@@ -371,37 +371,42 @@ def main():
     nd_d, mask_d = ctx.to_device(nd), ctx.to_device(mask)
     del d_ln
 
-    fams = (kf.api.PROF_FFT_Y, kf.api.PROF_RB0)
+    fams = (kf.api.PROF_FFT_ZC2R, kf.api.PROF_FFT_Y, kf.api.PROF_RB0)
     mixed = run_precision(kf, ctx, g, axes, "mixed", nd_d, mask_d, Fn0, args.steps, args.warmup, fams)
     fp64 = None
     if not args.no_fp64:
         fp64 = run_precision(kf, ctx, g, axes, "fp64", nd_d, mask_d, Fn0, args.steps, args.warmup)
 
-    # roofline of the FFT y pass (strided, FP32, length 256): it reads and
-    # writes the complex field once, 2 * Nx*Ny*(Nz/2+1) * 8 bytes per launch
+    def roofline(fam, alg_bytes, kernel, traffic_key):
+        launches, ms = mixed["prof"][fam]
+        avg_ms = ms / max(launches, 1)
+        achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+        traffic, tsrc = hbm_traffic(*traffic_key)
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": kernel,
+                "avg_launch_us": round(avg_ms * 1e3, 2), "launches": launches,
+                "alg_bytes_per_launch": int(alg_bytes), "traffic_source": tsrc}
+
     nc = n * n * (n // 2 + 1)
-    alg_bytes = 2 * nc * 8
-    ny_l, ny_ms = mixed["prof"][kf.api.PROF_FFT_Y]
-    avg_ms = ny_ms / max(ny_l, 1)
-    achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
-    traffic, tsrc = hbm_traffic("k_strided<float, 256", "LoadMasked")
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_strided (FFT y pass, FP32, length 256, both directions)",
-            "avg_launch_us": round(avg_ms * 1e3, 2), "launches": ny_l, "alg_bytes_per_launch": alg_bytes,
-            "traffic_source": tsrc}
-    # the finest-level red-black half sweep of the Poisson multigrid: per
-    # launch it needs the other colour's x and this colour's b, e and x out,
+    C, Rb = 8 * nc, 4 * n ** 3  # bytes of one FP32 complex / real field
+    # The dominant kernel: the z pass of the inverse transforms (half
+    # spectrum -> real rows, FP32).  One RHS evaluation runs 20 of them; with
+    # their fused loads and stores they move 20 C + 54 R bytes:
+    #   rho: C + nd + ln n + rho = C + 3R;  velocities: 3 (C + R);
+    #   products sum_b U_b dF_a/db, 4 fields: (C + 2R) + 2 (C + 3R) = 3C + 8R each;
+    #   penalization blend, 4 fields: C + 4R each (U or ln n, adv, mask in; adv out)
+    roof = roofline(kf.api.PROF_FFT_ZC2R, (20 * C + 54 * Rb) / 20,
+                    "k_zc2r (FFT z pass, half spectrum -> real rows with fused products, FP32, Nz = 256)",
+                    ("k_zc2r<float, 128", None))
+    # the FFT y pass (strided, FP32, length 256): reads and writes the
+    # complex field once, 2 C per launch
+    roof_y = roofline(kf.api.PROF_FFT_Y, 2 * C, "k_strided (FFT y pass, FP32, length 256, both directions)",
+                      ("k_strided<float, 256", "LoadMasked"))
+    # the finest-level red-black half sweep of the Poisson multigrid: the
+    # other colour's x, this colour's b and e in, this colour's x out,
     # 4 * (N/2) * 4 bytes (FP32)
-    rb_l, rb_ms = mixed["prof"][kf.api.PROF_RB0]
-    rb_avg = rb_ms / max(rb_l, 1)
-    rb_bytes = 8 * n ** 3
-    rb_traffic, rb_src = hbm_traffic("k_rb<float>", None, grid=str(n ** 3 // 2))
-    roof_rb = {"bound": "hbm", "achieved": round(rb_bytes / (rb_avg * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
-               "unit": "GB/s", "frac": round(rb_bytes / (rb_avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-               "traffic": rb_traffic, "kernel": "k_rb (red-black half sweep, finest level, FP32)",
-               "avg_launch_us": round(rb_avg * 1e3, 2), "launches": rb_l, "alg_bytes_per_launch": rb_bytes,
-               "traffic_source": rb_src}
+    roof_rb = roofline(kf.api.PROF_RB0, 8 * n ** 3, "k_rb_split (red-black half sweep, finest level, FP32)",
+                       ("k_rb_split<float>", None, str(n ** 3 // 2)))
 
     extra = {"poisson_failures_mixed": mixed["poisson_failures"]}
     if fp64 is not None:
@@ -437,7 +442,7 @@ def main():
                    "n": n, "L": L_BOX, "dt": 1e-3, "M": 1.5, "Q": -0.3, "tau": 0.1, "theta": 100.0,
                    "penalization": True, "spectral_viscosity": False, "poisson_tol": 1e-10},
         "roofline": roof,
-        "roofline_poisson_sweep": roof_rb,
+        "roofline_fft_y": roof_y, "roofline_poisson_sweep": roof_rb,
         "cpu_baseline": cpu,
     }
     line.update(extra)

@@ -330,7 +330,7 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
     z.o0 = rt_;
     z.prof = g_.prof<double>(b);
     z.pdir = b;
-    XLoadSpec<double> ld{Y, g_.view<double>(), 0, -1};
+    XLoadSpec<double> ld = xload<double>(Y, g_.view<double>(), 0, -1);
     ld.w = 1;
     {
         GridView<double> gv = g_.view<double>();
@@ -356,7 +356,7 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
 static void c2r_w(Grid &g, Plan<double> &p, const double2 *src, int d23, int filt, double scale, double *out,
                   hipStream_t s)
 {
-    XLoadSpec<double> ld{src, g.view<double>(), d23, -1};
+    XLoadSpec<double> ld = xload<double>(src, g.view<double>(), d23, -1);
     ld.w = 1;
     ld.filt = filt;
     ZStore<double> z = zs<double>(ZK_WRITE, p);

@@ -14,6 +14,9 @@
 #include "kf_rhs.h"
 #include "kf_rhs_impl.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 #include <cmath>
 #include <vector>
 
@@ -147,6 +150,8 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     : g_(g), ps_(p), pa_(pa), sw_(sw), s_(s)
 {
     plan_init<R>(plan_, g.nx, g.ny, g.nz, s);
+    // KF_EXTRAP_ORDER=1..3 caps the order of the potential extrapolation
+    if (const char *env = std::getenv("KF_EXTRAP_ORDER")) extrap_order_ = std::max(1, std::min(3, std::atoi(env)));
     const long nr = g.nr, nc = g.nc;
     nd_ = ralloc<R>(nr);
     mask_ = ralloc<R>(nr);
@@ -156,7 +161,17 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     rtmp_ = ralloc<R>(nr);
     for (int a = 0; a < 4; ++a) adv_[a] = ralloc<R>(nr);
     for (int a = 0; a < 4; ++a) lin_[a] = ralloc<C2<R>>(nc);
-    if (!std::is_same<R, double>::value) phiR_ = ralloc<R>(nr);
+    if (!std::is_same<R, double>::value) {
+        phiR_ = ralloc<R>(nr);
+        // KF_RHO64=0 forms rho in R (FP32) in the mixed RHS
+        const char *env = std::getenv("KF_RHO64");
+        rho64_on_ = !(env && std::atoi(env) == 0);
+        if (rho64_on_) {
+            rho64_ = ralloc<double>(nr);
+            nd64_ = ralloc<double>(nr);
+            KF_HIP(hipMemsetAsync(nd64_, 0, sizeof(double) * nr, s));
+        }
+    }
     if (!g.any_stretched())
         for (auto &f : H_)
             for (auto &h : f) h = ralloc<C2<R>>((long)plan_.nx * plan_.ny * plan_.nzs);
@@ -174,6 +189,8 @@ Rhs<R>::~Rhs()
     for (int b = 0; b < 3; ++b) (void)hipFree(U_[b]);
     (void)hipFree(L_);
     (void)hipFree(rho_);
+    (void)hipFree(rho64_);
+    (void)hipFree(nd64_);
     (void)hipFree(rtmp_);
     for (int a = 0; a < 4; ++a) {
         (void)hipFree(adv_[a]);
@@ -194,6 +211,7 @@ void Rhs<R>::set_fields(double *phi, const double *nd, const double *mask)
     if (phi) phi_ = phi;
     const long nr = g_.nr;
     if (nd) hipLaunchKernelGGL((k_convert<double, R>), dim3(nbk(nr)), dim3(256), 0, s_, nd, nd_, nr);
+    if (nd && nd64_) KF_HIP(hipMemcpyAsync(nd64_, nd, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
     if (mask) hipLaunchKernelGGL((k_convert<double, R>), dim3(nbk(nr)), dim3(256), 0, s_, mask, mask_, nr);
     KF_HIP(hipGetLastError());
 }
@@ -207,25 +225,30 @@ int Rhs<R>::poisson_failures()
     return f;
 }
 
-// phi = polynomial extrapolation in time of the last converged potentials
-// h2 (newest), h1, h0 (h0 unused when order 1)
-__global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h2, const double *__restrict__ h1,
-                         const double *__restrict__ h0, int order, long n)
+// phi = polynomial extrapolation in time (order 1..3) of the last converged
+// potentials h3 (newest), h2, h1, h0
+__global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h3, const double *__restrict__ h2,
+                         const double *__restrict__ h1, const double *__restrict__ h0, int order, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    phi[i] = order == 2 ? 3. * (h2[i] - h1[i]) + h0[i] : 2. * h2[i] - h1[i];
+    double v;
+    if (order >= 3) v = 4. * (h3[i] + h1[i]) - 6. * h2[i] - h0[i];
+    else if (order == 2) v = 3. * (h3[i] - h2[i]) + h1[i];
+    else v = 2. * h3[i] - h2[i];
+    phi[i] = v;
 }
 
-// h0 <- h1 <- h2 <- phi
-__global__ void k_shift3(double *__restrict__ h0, double *__restrict__ h1, double *__restrict__ h2,
-                         const double *__restrict__ phi, long n)
+// h0 <- h1 <- h2 <- h3 <- phi
+__global__ void k_shift4(double *__restrict__ h0, double *__restrict__ h1, double *__restrict__ h2,
+                         double *__restrict__ h3, const double *__restrict__ phi, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     h0[i] = h1[i];
     h1[i] = h2[i];
-    h2[i] = phi[i];
+    h2[i] = h3[i];
+    h3[i] = phi[i];
 }
 
 template <typename R>
@@ -242,31 +265,37 @@ void Rhs<R>::set_stage(int s)
 template <typename R>
 void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
 {
-    ZStore<R> z = zs<R>(ZK_RHO, plan_);
+    ZStore<R> z = zs<R>(rho64_on_ ? ZK_RHO64 : ZK_RHO, plan_);
     z.o0 = L_;
     z.o1 = rho_;
     z.a0 = nd_;
+    z.od = rho64_;
+    z.ad = nd64_;
     if (staged) z_staged<R>(g_, plan_, staged, false, z, s_);
     else c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
     if (!phi_) throw Error("RHS potential field not set");
     const long nr = g_.nr;
     if (stage_ >= 0 && hc_[stage_] >= 2) {
         double **h = ph_[stage_];
-        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, (const double *)h[2], (const double *)h[1],
-                           (const double *)h[0], hc_[stage_] >= 3 ? 2 : 1, nr);
+        const int order = std::min(hc_[stage_] - 1, extrap_order_);
+        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, (const double *)h[3], (const double *)h[2],
+                           (const double *)h[1], (const double *)h[0], order, nr);
     }
     if constexpr (std::is_same<R, double>::value) ps_.solve(phi_, rho_, nullptr, s_, false);
+    else if (rho64_on_) ps_.solve(phi_, rho64_, nullptr, s_, false);
     else ps_.solve(phi_, nullptr, rho_, s_, false);
     if (stage_ >= 0) {
         double **h = ph_[stage_];
         if (graph_safe_) {
-            hipLaunchKernelGGL(k_shift3, dim3(nbk(nr)), dim3(256), 0, s_, h[0], h[1], h[2], (const double *)phi_, nr);
+            hipLaunchKernelGGL(k_shift4, dim3(nbk(nr)), dim3(256), 0, s_, h[0], h[1], h[2], h[3], (const double *)phi_,
+                               nr);
         } else {
             double *oldest = h[0];
             h[0] = h[1];
             h[1] = h[2];
-            h[2] = oldest;
-            KF_HIP(hipMemcpyAsync(h[2], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+            h[2] = h[3];
+            h[3] = oldest;
+            KF_HIP(hipMemcpyAsync(h[3], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
         }
         hc_[stage_]++;
     }

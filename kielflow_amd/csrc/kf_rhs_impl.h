@@ -9,7 +9,7 @@ namespace kf {
 
 using namespace fft;
 
-enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN };
+enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN, ZK_RHO64 };
 enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN, XK_ADJ };
 
 // ------------------------------------------------------------------ spectral factors
@@ -80,6 +80,8 @@ struct XLoadSpec {
     int d23, dir;
     int w = 0;     // multiply by W (adjoint of R2C)
     int filt = 0;  // multiply by the Phi filter
+    // the wavenumber table of axis dir (set by xload): one load per element
+    const R *kd = nullptr;
     __device__ __forceinline__ C2<R> operator()(long q, int t) const
     {
         int i, j, k;
@@ -87,12 +89,21 @@ struct XLoadSpec {
         decode_x(g, q, t, i, j, k, idx);
         C2<R> v = src[idx];
         if (d23 && !keep23(g, i, j, k)) return mk<C2<R>>((R)0, (R)0);
-        if (dir >= 0) v = ik(k_along(g, dir, i, j, k), v);
+        if (dir >= 0) v = ik(kd[dir == 0 ? i : (dir == 1 ? j : k)], v);
         if (w) v = cscale(v, wplane(g, k));
         if (filt) v = cscale(v, phi_filter(g, i, j, k));
         return v;
     }
 };
+
+// XLoadSpec of src with [D23] and [i k_dir] (dir = -1: none)
+template <typename R>
+inline XLoadSpec<R> xload(const C2<R> *src, const GridView<R> &gv, int d23, int dir)
+{
+    XLoadSpec<R> l{src, gv, d23, dir};
+    l.kd = dir >= 0 ? gv.k[dir] : nullptr;
+    return l;
+}
 
 // real rows of a product a0*a1 (forward transform of a product)
 template <typename R>
@@ -118,6 +129,8 @@ struct ZStore {
     int flag;
     const R *prof;
     int pdir;
+    double *od;         // ZK_RHO64: rho in FP64
+    const double *ad;   // ZK_RHO64: n_d in FP64
     __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const
     {
         using C = C2<R>;
@@ -144,6 +157,14 @@ struct ZStore {
             const C d = *reinterpret_cast<const C *>(a0 + base);
             *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
             *reinterpret_cast<C *>(o1 + base) = mk<C>(exp(x0) + d.x, exp(x1) + d.y);
+            break;
+        }
+        case ZK_RHO64: {
+            // ln n in R; rho = exp(ln n) + n_d formed in FP64, so that the
+            // FP64 Newton residual sees no rounding of rho ~ 1 to R
+            const double2 d = *reinterpret_cast<const double2 *>(ad + base);
+            *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
+            *reinterpret_cast<double2 *>(od + base) = make_double2(exp((double)x0) + d.x, exp((double)x1) + d.y);
             break;
         }
         case ZK_PROD: {
@@ -286,7 +307,7 @@ inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
 {
     GridView<R> gv = g.view<R>();
     const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
-    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, XLoadSpec<R>{src, gv, d23, dir},
+    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, xload<R>(src, gv, d23, dir),
                           StorePlain<R>{p.scratch, ax}, s, prof::FFT_X, p.vx);
     launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
                           StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
@@ -331,8 +352,8 @@ inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C
     const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
     const long qx = (long)p.ny * p.nzc, qy = scratch_cols_y(p);
     const unsigned nv = (unsigned)p.nzc;
-    launch_strided<R, +1>(p.nx, qx, p.tw_x, XLoadSpec<R>{F, gv, 1, -1}, StorePlain<R>{H, ax}, s, prof::FFT_X, p.vx);
-    launch_strided<R, +1>(p.nx, qx, p.tw_x, XLoadSpec<R>{F, gv, 1, 0}, StorePlain<R>{Hx, ax}, s, prof::FFT_X, p.vx);
+    launch_strided<R, +1>(p.nx, qx, p.tw_x, xload<R>(F, gv, 1, -1), StorePlain<R>{H, ax}, s, prof::FFT_X, p.vx);
+    launch_strided<R, +1>(p.nx, qx, p.tw_x, xload<R>(F, gv, 1, 0), StorePlain<R>{Hx, ax}, s, prof::FFT_X, p.vx);
     launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMaskedK<R>{H, ay, nv, gv.k[1]}, StoreMasked<R>{Hy, ay, nv}, s,
                           prof::FFT_Y, p.vy);
     launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv}, s, prof::FFT_Y,
