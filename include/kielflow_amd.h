@@ -225,7 +225,7 @@ int kf_batch_set_fields(kf_batch b, int box, const double *nd, const double *mas
 /* record the graphs; Poisson solves inside run exactly poisson_iterations
  * Newton iterations (unconverged solves are counted as failures).  The
  * order of the potential warm start (extrapolation in time) is recorded
- * too: capture after at least 4 kf_batch_steps, when it is steady (cubic) */
+ * too: capture after at least 5 kf_batch_steps, when it is steady (quartic) */
 int kf_batch_capture(kf_batch b, int poisson_iterations);
 int kf_batch_steps(kf_batch b, int nsteps);
 int kf_batch_synchronize(kf_batch b);

@@ -50,9 +50,9 @@ class Rhs {
     void eval(double t, C2<R> *F[4]);
     void set_force_general(bool v) { force_general_ = v; }
     // RK stage (0..3) of the next evaluations, or -1.  Within an RK run the
-    // Poisson solve of stage s starts from the cubic extrapolation in time of
-    // the last four converged potentials of stage s (linear, quadratic while
-    // only two, three exist).
+    // Poisson solve of stage s starts from the quartic extrapolation in time
+    // of the last five converged potentials of stage s (of lower order while
+    // fewer exist).
     void set_stage(int s);
     // evaluations store their intermediates into *c (general path) while set
     void set_capture(Capture *c) { cap_ = c; }
@@ -118,9 +118,9 @@ class Rhs {
         if (cap_) return cap_;
         return (cap_stages_ && stage_ >= 0) ? &cap_stages_[stage_] : nullptr;
     }
-    double *ph_[4][4] = {};  // per RK stage: the last four converged potentials, newest last
+    double *ph_[4][6] = {};  // per RK stage: the last six converged potentials, newest last
     int hc_[4] = {0, 0, 0, 0};
-    int extrap_order_ = 3;   // highest order of the extrapolation in time
+    int extrap_order_ = 4;   // highest order of the extrapolation in time
 };
 
 // RK4 with an FP64 state and accumulator; stages evaluated in precision R.

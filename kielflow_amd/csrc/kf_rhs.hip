@@ -150,8 +150,8 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     : g_(g), ps_(p), pa_(pa), sw_(sw), s_(s)
 {
     plan_init<R>(plan_, g.nx, g.ny, g.nz, s);
-    // KF_EXTRAP_ORDER=1..3 caps the order of the potential extrapolation
-    if (const char *env = std::getenv("KF_EXTRAP_ORDER")) extrap_order_ = std::max(1, std::min(3, std::atoi(env)));
+    // KF_EXTRAP_ORDER=1..5 caps the order of the potential extrapolation
+    if (const char *env = std::getenv("KF_EXTRAP_ORDER")) extrap_order_ = std::max(1, std::min(5, std::atoi(env)));
     const long nr = g.nr, nc = g.nc;
     nd_ = ralloc<R>(nr);
     mask_ = ralloc<R>(nr);
@@ -225,30 +225,38 @@ int Rhs<R>::poisson_failures()
     return f;
 }
 
-// phi = polynomial extrapolation in time (order 1..3) of the last converged
-// potentials h3 (newest), h2, h1, h0
-__global__ void k_extrap(double *__restrict__ phi, const double *__restrict__ h3, const double *__restrict__ h2,
-                         const double *__restrict__ h1, const double *__restrict__ h0, int order, long n)
+// phi = polynomial extrapolation in time (order 1..5) of the last converged
+// potentials h[5] (newest), h[4], ..., h[0]: the value at the next equal
+// time step of the polynomial through the newest order+1 of them
+constexpr int HIST = 6;
+struct Hist {
+    const double *h[HIST];
+};
+__global__ void k_extrap(double *__restrict__ phi, Hist H, int order, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const double *const *h = H.h;
     double v;
-    if (order >= 3) v = 4. * (h3[i] + h1[i]) - 6. * h2[i] - h0[i];
-    else if (order == 2) v = 3. * (h3[i] - h2[i]) + h1[i];
-    else v = 2. * h3[i] - h2[i];
+    if (order >= 5) v = 6. * (h[5][i] + h[1][i]) - 15. * (h[4][i] + h[2][i]) + 20. * h[3][i] - h[0][i];
+    else if (order == 4) v = 5. * (h[5][i] - h[2][i]) - 10. * (h[4][i] - h[3][i]) + h[1][i];
+    else if (order == 3) v = 4. * (h[5][i] + h[3][i]) - 6. * h[4][i] - h[2][i];
+    else if (order == 2) v = 3. * (h[5][i] - h[4][i]) + h[3][i];
+    else v = 2. * h[5][i] - h[4][i];
     phi[i] = v;
 }
 
-// h0 <- h1 <- h2 <- h3 <- phi
-__global__ void k_shift4(double *__restrict__ h0, double *__restrict__ h1, double *__restrict__ h2,
-                         double *__restrict__ h3, const double *__restrict__ phi, long n)
+// h[0] <- h[1] <- ... <- h[HIST-1] <- phi
+struct HistW {
+    double *h[HIST];
+};
+__global__ void k_shift_hist(HistW H, const double *__restrict__ phi, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    h0[i] = h1[i];
-    h1[i] = h2[i];
-    h2[i] = h3[i];
-    h3[i] = phi[i];
+#pragma unroll
+    for (int m = 0; m < HIST - 1; ++m) H.h[m][i] = H.h[m + 1][i];
+    H.h[HIST - 1][i] = phi[i];
 }
 
 template <typename R>
@@ -278,8 +286,9 @@ void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
     if (stage_ >= 0 && hc_[stage_] >= 2) {
         double **h = ph_[stage_];
         const int order = std::min(hc_[stage_] - 1, extrap_order_);
-        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, (const double *)h[3], (const double *)h[2],
-                           (const double *)h[1], (const double *)h[0], order, nr);
+        Hist H{};
+        for (int m = 0; m < HIST; ++m) H.h[m] = h[m];
+        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, H, order, nr);
     }
     if constexpr (std::is_same<R, double>::value) ps_.solve(phi_, rho_, nullptr, s_, false);
     else if (rho64_on_) ps_.solve(phi_, rho64_, nullptr, s_, false);
@@ -287,15 +296,14 @@ void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
     if (stage_ >= 0) {
         double **h = ph_[stage_];
         if (graph_safe_) {
-            hipLaunchKernelGGL(k_shift4, dim3(nbk(nr)), dim3(256), 0, s_, h[0], h[1], h[2], h[3], (const double *)phi_,
-                               nr);
+            HistW H{};
+            for (int m = 0; m < HIST; ++m) H.h[m] = h[m];
+            hipLaunchKernelGGL(k_shift_hist, dim3(nbk(nr)), dim3(256), 0, s_, H, (const double *)phi_, nr);
         } else {
             double *oldest = h[0];
-            h[0] = h[1];
-            h[1] = h[2];
-            h[2] = h[3];
-            h[3] = oldest;
-            KF_HIP(hipMemcpyAsync(h[3], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+            for (int m = 0; m < HIST - 1; ++m) h[m] = h[m + 1];
+            h[HIST - 1] = oldest;
+            KF_HIP(hipMemcpyAsync(h[HIST - 1], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
         }
         hc_[stage_]++;
     }

@@ -53,7 +53,7 @@ def test_batch_graph_replays_match_solo_runs(ctx):
         fft.r2c(ctx.to_device(ln_n), F[3])
         fft_keep.append((fft, g))
         solos.append(_solo(kf, ctx, axes, (ln_n, nd, mask), Ms[b], kf.KF_MIXED))
-    warm = 4  # the potential history is full: the graph keeps the steady-state extrapolation
+    warm = 5  # the potential history is full: the graph keeps the steady-state extrapolation
     B.steps(warm)
     for s in solos:
         s["rk"].step(s["F"], warm)
