@@ -389,15 +389,16 @@ def main():
 
     nc = n * n * (n // 2 + 1)
     C, Rb = 8 * nc, 4 * n ** 3  # bytes of one FP32 complex / real field
-    # The dominant kernel: the z pass of the inverse transforms (half
-    # spectrum -> real rows, FP32).  One RHS evaluation runs 20 of them; with
-    # their fused loads and stores they move 20 C + 54 R bytes:
-    #   rho: C + nd + ln n + rho = C + 3R;  velocities: 3 (C + R);
-    #   products sum_b U_b dF_a/db, 4 fields: (C + 2R) + 2 (C + 3R) = 3C + 8R each;
+    # The dominant kernel family: the z passes of the inverse transforms
+    # (half spectrum -> real rows, FP32).  One RHS evaluation runs 12 of them;
+    # with their fused loads and stores they move 20 C + 40 R bytes:
+    #   rho: C + n_d (FP64) in, ln n + rho (FP64) out = C + 5R;
+    #   velocities: 3 (C + R);
+    #   sum_b U_b dF_a/db, one pass per field (3 transforms): 4 (3C + 4R);
     #   penalization blend, 4 fields: C + 4R each (U or ln n, adv, mask in; adv out)
-    roof = roofline(kf.api.PROF_FFT_ZC2R, (20 * C + 54 * Rb) / 20,
-                    "k_zc2r (FFT z pass, half spectrum -> real rows with fused products, FP32, Nz = 256)",
-                    ("k_zc2r<float, 128", None))
+    roof = roofline(kf.api.PROF_FFT_ZC2R, (20 * C + 40 * Rb) / 12,
+                    "FFT z passes (half spectrum -> real rows with fused products, FP32, Nz = 256)",
+                    ("k_z", None))
     # the FFT y pass (strided, FP32, length 256): reads and writes the
     # complex field once, 2 C per launch
     roof_y = roofline(kf.api.PROF_FFT_Y, 2 * C, "k_strided (FFT y pass, FP32, length 256, both directions)",

@@ -79,7 +79,7 @@ struct DstQ {
     const ST_ &f;
     long q;
     bool live;
-    __device__ __forceinline__ void st(int t, C v) const
+    __device__ __forceinline__ void st(int t, C v, int = 0) const
     {
         if (live) f(q, t, v);
     }

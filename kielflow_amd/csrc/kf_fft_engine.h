@@ -108,9 +108,10 @@ __device__ __forceinline__ void dft(C *v)
 //   butterfly j (0 <= j < N/R_):  inputs  x[j + q*N/R_]
 //   twiddle  w^(q * (j mod NS) * N/(NS*R_))
 //   outputs  y[(j/NS)*NS*R_ + (j mod NS) + q*NS]
-// Inputs come from src.ld(t), outputs go to dst.st(t, v): global memory (the
-// load/store functors of the kernel) for the first and the last pass, an LDS
-// buffer in between.  All of a thread's loads are issued before its
+// Inputs come from src.ld(t), outputs go to dst.st(t, v, slot): global
+// memory (the load/store functors of the kernel) for the first and the last
+// pass, an LDS buffer in between; slot (< out_slots) numbers a thread's
+// outputs at compile time, for destinations that keep them in registers.  All of a thread's loads are issued before its
 // arithmetic.  SYNC_MID: src and dst are the same LDS buffer, so every thread
 // must finish reading before any writes.
 template <typename C, int N, int R_, int NS, int TPT, int DIR, bool SYNC_MID, class SRC, class DST>
@@ -150,7 +151,7 @@ __device__ __forceinline__ void pass_io(const C *tw, int tt, const SRC &src, con
         if (NB % TPT == 0 || j < NB) {
             const int base = (j / NS) * NS * R_ + (j % NS);
 #pragma unroll
-            for (int q = 0; q < R_; ++q) dst.st(base + q * NS, v[m][q]);
+            for (int q = 0; q < R_; ++q) dst.st(base + q * NS, v[m][q], m * R_ + q);
         }
     }
 }
@@ -194,6 +195,18 @@ __device__ __forceinline__ void fft_io(const C *tw, int tt, const SRC &src, cons
     chain_io<C, N, N, 1, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, B, A);
 }
 
+// outputs per thread of the last pass of a length-n transform by tpt threads
+__host__ __device__ constexpr int out_slots(int n, int tpt)
+{
+    int rem = n, r = 1;
+    while (rem > 1) {
+        r = next_radix(rem);
+        if (rem == r) break;
+        rem /= r;
+    }
+    return ((n / r + tpt - 1) / tpt) * r;
+}
+
 // ---------------------------------------------------------------- LDS layouts
 // Bank-conflict-avoiding permutations of the positions t of one transform:
 // t ^ ((t >> A) & MASK) (a bijection since MASK < 2^A), chosen per length
@@ -218,7 +231,7 @@ struct ColBuf {
     int c;
     static constexpr int A = col_swz_a(N, CB * (int)sizeof(C)), MASK = col_swz_mask(N, CB * (int)sizeof(C));
     __device__ __forceinline__ C ld(int t) const { return p[swz(t, A, MASK) * CB + c]; }
-    __device__ __forceinline__ void st(int t, C v) const { p[swz(t, A, MASK) * CB + c] = v; }
+    __device__ __forceinline__ void st(int t, C v, int = 0) const { p[swz(t, A, MASK) * CB + c] = v; }
 };
 
 // row layout (z passes): element t of row c at c*SC + swz(t); {pad, A, MASK}
@@ -263,7 +276,7 @@ struct RowBuf {
     static constexpr RowLay L = row_layout(M, (int)sizeof(C));
     static constexpr int SC = M + L.pad;
     __device__ __forceinline__ C ld(int t) const { return p[c * SC + swz(t, L.a, L.mask)]; }
-    __device__ __forceinline__ void st(int t, C v) const { p[c * SC + swz(t, L.a, L.mask)] = v; }
+    __device__ __forceinline__ void st(int t, C v, int = 0) const { p[c * SC + swz(t, L.a, L.mask)] = v; }
 };
 
 }  // namespace fft

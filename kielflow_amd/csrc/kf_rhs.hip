@@ -368,16 +368,9 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
         z_staged<R>(g_, plan_, H_[b][0], false, z, s_);
     }
     // advection / continuity products (rhs_standard.cpp:323-343, 403-437):
-    // adv_a = sum_b U_b dF_a/db
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 3; ++b) {
-            ZStore<R> z = zs<R>(ZK_PROD, plan_);
-            z.o0 = adv_[a];
-            z.a0 = U_[b];
-            z.flag = (b > 0);
-            const C2<R> *h = b == 0 ? H_[a][1] : (b == 1 ? H_[a][2] : H_[a][0]);
-            z_staged<R>(g_, plan_, h, b == 2, z, s_);
-        }
+    // adv_a = sum_b U_b dF_a/db, one z pass per field
+    const R *U[3] = {U_[0], U_[1], U_[2]};
+    for (int a = 0; a < 4; ++a) zprod3<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, adv_[a], s_);
     output(F);
 }
 

@@ -362,6 +362,97 @@ inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C
                           prof::FFT_Y, p.vy);
 }
 
+// adv = sum_b U_b dF/db for one field (rhs_standard.cpp:323-343, 403-437)
+// in one z pass: the C2R transforms of the staged Hx (dF/dx), Hy (dF/dy) and
+// H with i k_z (dF/dz) run one after another in the workgroup, each product
+// is accumulated in registers in the order b = 0, 1, 2, and adv is written
+// once.  Same arithmetic as three accumulating passes (ZK_PROD), a third of
+// the U reads and adv traffic.
+template <typename R>
+struct ProdAcc {
+    C2<R> *acc;   // the thread's accumulator slots
+    int *pos;     // their output positions (pairs n of the real row)
+    const R *u;   // U_b
+    long row;
+    int nz;
+    R scale;
+    bool first, live;
+    __device__ __forceinline__ void st(int t, C2<R> v, int slot) const
+    {
+        if (!live) return;
+        const R x0 = v.x * scale, x1 = v.y * scale;
+        const C2<R> uu = reinterpret_cast<const C2<R> *>(u + row * nz)[t];
+        const C2<R> prev = first ? mk<C2<R>>((R)0, (R)0) : acc[slot];
+        acc[slot] = mk<C2<R>>(prev.x + uu.x * x0, prev.y + uu.y * x1);
+        pos[slot] = t;
+    }
+};
+
+template <typename R, int M, int RB>
+__global__ void __launch_bounds__(tpt_for(M) * RB)
+k_zprod3(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, int nz, int nzs,
+         const C2<R> *Hx, const C2<R> *Hy, const C2<R> *H, const R *kz, const R *U0, const R *U1, const R *U2,
+         R *adv, R scale)
+{
+    using C = C2<R>;
+    using Buf = RowBuf<C, M>;
+    constexpr int TPT = tpt_for(M);
+    constexpr int WG = TPT * RB;
+    constexpr int E = out_slots(M, TPT);
+    __shared__ C lds[RB * Buf::SC];
+    __shared__ C tw[M];
+    const int tid = threadIdx.x;
+    const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
+    for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
+    const int c = tid / TPT, tt = tid % TPT;
+    const long row = r0 + c;
+    const bool live = row < rows;
+    const Buf A{lds, c};
+    C acc[E];
+    int pos[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) pos[e] = -1;
+    const RowLoadC<R> lx{Hx, nzs}, ly{Hy, nzs};
+    const RowLoadCK<R> lz{H, nzs, kz};
+    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{lx, row, live, twN_g},
+                                        ProdAcc<R>{acc, pos, U0, row, nz, scale, true, live}, A, A);
+    __syncthreads();
+    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ly, row, live, twN_g},
+                                        ProdAcc<R>{acc, pos, U1, row, nz, scale, false, live}, A, A);
+    __syncthreads();
+    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadCK<R>>{lz, row, live, twN_g},
+                                        ProdAcc<R>{acc, pos, U2, row, nz, scale, false, live}, A, A);
+    if (live) {
+        C *o = reinterpret_cast<C *>(adv + row * nz);
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+            if (pos[e] >= 0) o[pos[e]] = acc[e];
+    }
+}
+
+template <typename R>
+inline void zprod3(Grid &g, Plan<R> &p, const C2<R> *Hx, const C2<R> *Hy, const C2<R> *H, const R *const U[3],
+                   R *adv, hipStream_t s)
+{
+    prof::Scope ps_(prof::FFT_ZC2R, s);
+    const long rows = (long)p.nx * p.ny;
+    const R *kz = g.view<R>().k[2];
+    const R scale = (R)(1.0 / (double)p.nr);
+    switch (p.nz / 2) {
+#define KF_CASE(NN)                                                                                            \
+    case NN: {                                                                                                 \
+        constexpr int RB = cb_for<R>(NN);                                                                      \
+        hipLaunchKernelGGL((k_zprod3<R, NN, RB>), dim3((unsigned)((rows + RB - 1) / RB)), dim3(tpt_for(NN) * RB), \
+                           0, s, p.tw_m, p.tw_n, rows, p.nz, p.nzs, Hx, Hy, H, kz, U[0], U[1], U[2], adv, scale); \
+        break;                                                                                                 \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported z length " + std::to_string(p.nz));
+    }
+}
+
 // z pass of a staged field into a ZStore; with i k_z when dz
 template <typename R>
 inline void z_staged(Grid &g, Plan<R> &p, const C2<R> *H, bool dz, const ZStore<R> &st, hipStream_t s)
