@@ -319,8 +319,9 @@ def hbm_traffic(kernel, functor, grid=None):
     except OSError:
         return None, None
     tot, cnt = 0.0, 0
+    kernels = (kernel,) if isinstance(kernel, str) else tuple(kernel)
     for ln in lines:
-        if kernel not in ln or (functor and functor not in ln):
+        if not any(k in ln for k in kernels) or (functor and functor not in ln):
             continue
         parts = ln.split()
         try:
@@ -398,7 +399,7 @@ def main():
     #   penalization blend, 4 fields: C + 4R each (U or ln n, adv, mask in; adv out)
     roof = roofline(kf.api.PROF_FFT_ZC2R, (20 * C + 40 * Rb) / 12,
                     "FFT z passes (half spectrum -> real rows with fused products, FP32, Nz = 256)",
-                    ("k_z", None))
+                    (("k_zc2r<float, 128", "k_zprod3<float, 128"), None))
     # the FFT y pass (strided, FP32, length 256): reads and writes the
     # complex field once, 2 C per launch
     roof_y = roofline(kf.api.PROF_FFT_Y, 2 * C, "k_strided (FFT y pass, FP32, length 256, both directions)",
