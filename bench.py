@@ -198,7 +198,7 @@ def adjoint_cost(kf, ctx, n=64, steps=200, k=20, m=3.0):
             "intermediates": kept, "transposed_poisson_iterations": lin_it}
 
 
-def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=6, check_boxes=4):
+def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=4, check_boxes=4):
     """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
     M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
     precision, one captured graph per box and step over `nstreams` streams.

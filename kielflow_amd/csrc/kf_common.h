@@ -82,6 +82,10 @@ struct GridView {
     const double *q[3];
     const double *qzf;
     double d23_thr;   // 2/3-rule: sqrt(s) < 2/3  <=>  s < d23_thr  (s = sum of q)
+    // per column (j, k): the largest |i'| (i' the signed x index) whose mode
+    // passes the 2/3 rule, or -1; the kept x indices of a column are exactly
+    // |i'| <= imax since q_x grows with |i'|
+    const int *imax;
 };
 
 // wavenumber along axis dir at (i, j, kk): all three table values are

@@ -16,6 +16,7 @@ struct Grid {
     float *k32[3] = {nullptr, nullptr, nullptr};
     double *q[3] = {nullptr, nullptr, nullptr};
     double *qzf = nullptr;
+    int *imax = nullptr;  // GridView::imax
     // dealiased 1/S'(xi) per stretched axis (OP_partial_derivative.cpp:311-331);
     // nullptr on equidistant axes
     double *prof64[3] = {nullptr, nullptr, nullptr};
@@ -50,6 +51,7 @@ struct Grid {
         }
         v.qzf = qzf;
         v.d23_thr = d23_thr;
+        v.imax = imax;
         return v;
     }
     template <typename R>

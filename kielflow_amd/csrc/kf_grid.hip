@@ -1,6 +1,7 @@
 // kf_grid.hip -- host construction of the per-axis device tables.
 #include "kf_grid.h"
 
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -95,6 +96,34 @@ Grid::Grid(const AxisDesc axes[3], hipStream_t s)
         }
     }
     {
+        // 2/3-rule membership per column as a bound on |i'|, from the same
+        // FP64 expression the kernels evaluated per element, ((q_x + q_y) + q_z)
+        // < d23_thr; checked to be exactly a bound (q_x grows with |i'|)
+        std::vector<double> qh[3];
+        for (int d = 0; d < 3; ++d) {
+            const int len = (d == 2) ? nzc : ax[d].n;
+            const double kmax = axis_k_val_at(ax[d], ax[d].n / 2);
+            qh[d].resize(len);
+            for (int i = 0; i < len; ++i) qh[d][i] = pow(axis_k_val_at(ax[d], i) / kmax, 2.);
+        }
+        std::vector<int> im((size_t)ny * nzc);
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nzc; ++k) {
+                int m = -1;
+                for (int i = 0; i < nx; ++i) {
+                    const int ip = i <= nx / 2 ? i : nx - i;
+                    if ((qh[0][i] + qh[1][j]) + qh[2][k] < d23_thr) m = std::max(m, ip);
+                }
+                for (int i = 0; i < nx; ++i) {
+                    const int ip = i <= nx / 2 ? i : nx - i;
+                    const bool keep = (qh[0][i] + qh[1][j]) + qh[2][k] < d23_thr;
+                    if (keep != (ip <= m)) throw Error("2/3-rule set of a column is not a bound on |i'|");
+                }
+                im[(size_t)j * nzc + k] = m;
+            }
+        imax = upload(im, s);
+    }
+    {
         std::vector<double> b(nx);
         const double L = ax[0].L;
         const double pi = acos(-1);
@@ -118,6 +147,7 @@ Grid::~Grid()
         (void)hipFree(prof32[d]);
     }
     (void)hipFree(qzf);
+    (void)hipFree(imax);
     (void)hipFree(bar64);
     (void)hipFree(bar32);
 }

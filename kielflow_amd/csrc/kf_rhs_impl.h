@@ -13,10 +13,13 @@ enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN, ZK_RHO64 };
 enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN, XK_ADJ };
 
 // ------------------------------------------------------------------ spectral factors
+// 2/3-rule membership of mode (i, j, k): the exact per-element test
+// d23_keep(q_x, q_y, q_z) reduced to a bound per column (GridView::imax)
 template <typename R>
 __device__ __forceinline__ bool keep23(const GridView<R> &g, int i, int j, int k)
 {
-    return d23_keep(g.q[0][i], g.q[1][j], g.q[2][k], g.d23_thr);
+    const int ip = i <= (g.nx >> 1) ? i : g.nx - i;
+    return ip <= g.imax[j * g.nzc + k];
 }
 
 // spectral viscosity 0.5 (1 - exp(-8 sqrt(knorm)^8)) (rhs_standard.cpp:68-71)
