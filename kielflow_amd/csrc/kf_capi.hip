@@ -686,10 +686,8 @@ int kf_rk4_steps(kf_rk4 k, int n, double *FUx, double *FUy, double *FUz, double 
     return guard([&] {
         KF_REQUIRE(k && FUx && FUy && FUz && Fn, "null argument");
         double2 *y[4] = {(double2 *)FUx, (double2 *)FUy, (double2 *)FUz, (double2 *)Fn};
-        for (int i = 0; i < n; ++i) {
-            if (k->k64) k->k64->step(y);
-            else k->k32->step(y);
-        }
+        if (k->k64) k->k64->steps(y, n);
+        else k->k32->steps(y, n);
         KF_HIP(hipGetLastError());
         return KF_OK;
     });

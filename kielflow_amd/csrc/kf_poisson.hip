@@ -870,7 +870,8 @@ __global__ void k_restrict_split(const R *__restrict__ f, int fx, int fy, int fz
 // ======================================================================== coarse levels in one workgroup
 // The tail of a V-cycle (every level with at most COARSE_N points) runs in a
 // single workgroup: block-stride loops separated by barriers replace the
-// many tiny launches those levels would otherwise cost.
+// many tiny launches those levels would otherwise cost (their launch latency
+// exceeds their work; at 32^3 one workgroup is already slower than launches).
 constexpr long COARSE_N = 4096;
 constexpr int COARSE_MAXL = 8;
 constexpr int COARSE_BLOCK = 1024;
@@ -920,42 +921,45 @@ __device__ void blk_smooth(const LevelView<R> &L, R *x, const R *b, const R *e, 
     }
 }
 
-// points of all coarse levels together: every level has 1/8 of the points
-// of the one above it, so the sum stays below 8/7 of the first
-constexpr long COARSE_LDS_PTS = COARSE_N + COARSE_N / 7 + 8;
+// levels with at most COARSE_LDS_N points keep x and b in LDS; together
+// they hold less than 8/7 of the first such level (each has 1/8 of the
+// points of the one above)
+constexpr long COARSE_LDS_N = 4096;
+constexpr long COARSE_LDS_PTS = COARSE_LDS_N + COARSE_LDS_N / 7 + 8;
 
-// The coarse V-cycle.  IN_LDS: x and b of every coarse level and the
-// residual live in LDS (the first level's b is read from, and its x written
-// back to, global memory), so the ~100 barrier-separated phases of the
-// cycle wait on LDS rather than on L2; e stays in global memory (read-only).
+// The coarse V-cycle.  IN_LDS: x and b of the levels with at most
+// COARSE_LDS_N points and their residual live in LDS, so most of the ~100
+// barrier-separated phases of the cycle wait on LDS rather than on L2
+// (larger levels, and e of every level, stay in global memory).
 template <typename R, bool IN_LDS>
 __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs, const int *ctl)
 {
-    __shared__ R sx[IN_LDS ? COARSE_LDS_PTS : 1], sb[IN_LDS ? COARSE_LDS_PTS : 1], sr[IN_LDS ? COARSE_N : 1];
-    __shared__ long soff[COARSE_MAXL + 1];
+    __shared__ R sx[IN_LDS ? COARSE_LDS_PTS : 1], sb[IN_LDS ? COARSE_LDS_PTS : 1], sr[IN_LDS ? COARSE_LDS_N : 1];
+    __shared__ long soff[COARSE_MAXL + 1];  // LDS offset of a level, -1: global memory
     if (ctl[0]) return;
     const int tid = threadIdx.x, bs = blockDim.x;
     const int nl = cs.nl;
-    if (IN_LDS && tid == 0) {
+    if (tid == 0) {
         long o = 0;
         for (int l = 0; l < nl; ++l) {
-            soff[l] = o;
-            o += cs.L[l].N;
+            const bool in = IN_LDS && cs.L[l].N <= COARSE_LDS_N;
+            soff[l] = in ? o : -1;
+            if (in) o += cs.L[l].N;
         }
     }
-#define CX(l) (IN_LDS ? sx + soff[l] : cs.x[l])
-#define CB(l) (IN_LDS ? sb + soff[l] : cs.b[l])
-#define CR(l) (IN_LDS ? sr : cs.r[l])
-    if (IN_LDS) {
+    __syncthreads();
+#define CX(l) (soff[l] >= 0 ? sx + soff[l] : cs.x[l])
+#define CB(l) (soff[l] >= 0 ? sb + soff[l] : cs.b[l])
+#define CR(l) (soff[l] >= 0 ? sr : cs.r[l])
+    if (soff[0] >= 0) {
         for (long p = tid; p < cs.L[0].N; p += bs) {
             sb[p] = cs.b[0][p];
             sx[p] = cs.zero0 ? (R)0 : cs.x[0][p];
         }
-        __syncthreads();
     } else if (cs.zero0) {
         for (long p = tid; p < cs.L[0].N; p += bs) cs.x[0][p] = (R)0;
-        __syncthreads();
     }
+    __syncthreads();
     for (int l = 0; l < nl - 1; ++l) {
         const LevelView<R> &L = cs.L[l];
         R *x = CX(l), *b = CB(l), *r = CR(l);
@@ -1014,7 +1018,7 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs,
         __syncthreads();
         blk_smooth<R>(L, x, CB(l), cs.e[l], CR(l), cs.nu2);
     }
-    if (IN_LDS) {
+    if (soff[0] >= 0) {
         for (long p = tid; p < cs.L[0].N; p += bs) cs.x[0][p] = sx[p];
     }
 #undef CX
@@ -1504,7 +1508,8 @@ void Poisson::vcycle(int l, hipStream_t s, bool zero)
         CoarseSet<R> cs = coarse_set<R>(l);
         cs.zero0 = zero ? 1 : 0;
         long pts = 0;
-        for (int m = 0; m < cs.nl; ++m) pts += cs.L[m].N;
+        for (int m = 0; m < cs.nl; ++m)
+            if (cs.L[m].N <= COARSE_LDS_N) pts += cs.L[m].N;
         if (pts <= COARSE_LDS_PTS)
             hipLaunchKernelGGL((k_coarse_vcycle<R, true>), dim3(1), dim3(COARSE_BLOCK), 0, s, cs, (const int *)d_ctl_);
         else

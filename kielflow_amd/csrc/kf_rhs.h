@@ -131,7 +131,8 @@ class Rk4 {
     ~Rk4();
     Rk4(const Rk4 &) = delete;
     Rk4 &operator=(const Rk4 &) = delete;
-    void step(double2 *y[4]);  // in place
+    void step(double2 *y[4]);          // in place
+    void steps(double2 *y[4], int n);  // n steps in place
     double dt() const { return dt_; }
     void set_dt(double dt) { dt_ = dt; }
     double t() const { return t_; }

@@ -502,22 +502,23 @@ __global__ void k_rk_stage(Quad<R> q, long n, double c, int first)
     q.S[f][i] = mk<C2<R>>((R)(y0.x + c * kx), (R)(y0.y + c * ky));
 }
 
-// y = y0 + (dt/6) (acc + 1*k4)   (RKO4.cpp:225-238)
+// y = y0 + (dt/6) (acc + 1*k4)   (RKO4.cpp:225-238); with next, also the
+// next step's first stage input S = y (what k_rk_init would write)
 template <typename R>
-__global__ void k_rk_final(Quad<R> q, double2 *y0_, double2 *y1_, double2 *y2_, double2 *y3_, long n,
-                           double dt6)
+__global__ void k_rk_final(Quad<R> q, long n, double dt6, int next)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
     if (i >= n) return;
-    double2 *Y[4] = {y0_, y1_, y2_, y3_};
+    double2 *Y = const_cast<double2 *>(f == 0 ? q.y[0] : f == 1 ? q.y[1] : f == 2 ? q.y[2] : q.y[3]);
     const C2<R> kk = q.S[f][i];
     const double2 a = q.acc[f][i];
-    const double2 y0 = Y[f][i];
+    const double2 y0 = Y[i];
     double2 o;
     o.x = y0.x + dt6 * (a.x + 1. * (double)kk.x);
     o.y = y0.y + dt6 * (a.y + 1. * (double)kk.y);
-    Y[f][i] = o;
+    Y[i] = o;
+    if (next) q.S[f][i] = mk<C2<R>>((R)o.x, (R)o.y);
 }
 
 template <typename R>
@@ -543,6 +544,14 @@ Rk4<R>::~Rk4()
 template <typename R>
 void Rk4<R>::step(double2 *y[4])
 {
+    steps(y, 1);
+}
+
+// n steps; within the call the final update of a step also writes the
+// first stage input of the next one
+template <typename R>
+void Rk4<R>::steps(double2 *y[4], int n)
+{
     hipStream_t s = rhs_.stream();
     const long nc = rhs_.grid().nc;
     Quad<R> q;
@@ -552,19 +561,21 @@ void Rk4<R>::step(double2 *y[4])
         q.acc[f] = acc_[f];
     }
     dim3 grid(nbk(nc), 4);
-    hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
     const double c[3] = {0.5 * dt_, 0.5 * dt_, 1.0 * dt_};
-    const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
-    for (int st = 0; st < 4; ++st) {
-        rhs_.set_stage(st);
-        rhs_.eval(tt[st], S_);
-        if (st < 3)
-            hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0);
+    for (int it = 0; it < n; ++it) {
+        if (it == 0) hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
+        const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
+        for (int st = 0; st < 4; ++st) {
+            rhs_.set_stage(st);
+            rhs_.eval(tt[st], S_);
+            if (st < 3)
+                hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0);
+        }
+        rhs_.set_stage(-1);
+        hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, nc, dt_ / 6., it + 1 < n ? 1 : 0);
+        KF_HIP(hipGetLastError());
+        t_ += dt_;
     }
-    rhs_.set_stage(-1);
-    hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, y[0], y[1], y[2], y[3], nc, dt_ / 6.);
-    KF_HIP(hipGetLastError());
-    t_ += dt_;
 }
 
 // ------------------------------------------------------------------ OP_* on FP64 fields
