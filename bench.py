@@ -400,10 +400,11 @@ def main():
     roof = roofline(kf.api.PROF_FFT_ZC2R, (20 * C + 40 * Rb) / 12,
                     "FFT z passes (half spectrum -> real rows with fused products, FP32, Nz = 256)",
                     (("k_zc2r<float, 128", "k_zprod3<float, 128"), None))
-    # the FFT y pass (strided, FP32, length 256): reads and writes the
-    # complex field once, 2 C per launch
-    roof_y = roofline(kf.api.PROF_FFT_Y, 2 * C, "k_strided (FFT y pass, FP32, length 256, both directions)",
-                      ("k_strided<float, 256", "LoadMasked"))
+    # the FFT y passes (strided, FP32, length 256).  Per RHS evaluation: 4
+    # that read a staged field once and write it and its i k_y multiple (3 C),
+    # and 13 that read and write one complex field (2 C): 38 C over 17
+    roof_y = roofline(kf.api.PROF_FFT_Y, 38 * C / 17, "FFT y passes (k_strided, k_strided_ik; FP32, length 256)",
+                      (("k_strided<float, 256", "k_strided_ik<float, 256"), "LoadMasked"))
     # the finest-level red-black half sweep of the Poisson multigrid: the
     # other colour's x, this colour's b and e in, this colour's x out,
     # 4 * (N/2) * 4 bytes (FP32)

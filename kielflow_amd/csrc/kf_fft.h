@@ -68,10 +68,40 @@ struct SrcQ {
     const LD &f;
     long q;
     bool live;
-    __device__ __forceinline__ C ld(int t) const
+    __device__ __forceinline__ C ld(int t, int = 0) const
     {
         using R = decltype(C().x);
         return live ? f(q, t) : mk<C>((R)0, (R)0);
+    }
+};
+
+// source of the first of two transforms of the same column: loads through
+// the kernel's functor and keeps each input in the thread's slot
+template <typename C, class LD>
+struct SrcKeep {
+    const LD &f;
+    long q;
+    bool live;
+    C *keep;
+    __device__ __forceinline__ C ld(int t, int slot) const
+    {
+        using R = decltype(C().x);
+        const C v = live ? f(q, t) : mk<C>((R)0, (R)0);
+        keep[slot] = v;
+        return v;
+    }
+};
+
+// source of the second: the kept inputs times i k[t] (k of the pass axis)
+template <typename C>
+struct SrcKeptIK {
+    const C *keep;
+    const decltype(C().x) *k;
+    __device__ __forceinline__ C ld(int t, int slot) const
+    {
+        const C v = keep[slot];
+        const auto kv = k[t];
+        return mk<C>(-kv * v.y, kv * v.x);
     }
 };
 template <typename C, class ST_>
@@ -107,6 +137,32 @@ k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
     const bool live = q < Q;
     const ColBuf<C, N, CB> A{lds, c}, B{lds + (PING ? N * CB : 0), c};
     fft_io<C, N, TPT, DIR, PING, false>(tw, t0, SrcQ<C, LD>{ld, q, live}, DstQ<C, ST_>{st, q, live}, A, B);
+}
+
+// Two strided transforms of every column of CB, sharing one load: out A =
+// FFT(x), out B = FFT(i k[t] x) with k the wavenumbers of the pass axis (t
+// the position along it).  The inputs of the first pass stay in registers.
+template <typename R, int N, int CB, int DIR, class LD, class STA, class STB>
+__global__ void __launch_bounds__(tpt_for(N) * CB)
+k_strided_ik(const C2<R> *__restrict__ tw_g, long Q, LD ld, STA sta, STB stb, const R *__restrict__ kax)
+{
+    using C = C2<R>;
+    constexpr int TPT = tpt_for(N);
+    constexpr int WG = TPT * CB;
+    constexpr bool PING = 2 * N * CB * (int)sizeof(C) <= 32768;
+    __shared__ C lds[(PING ? 2 : 1) * N * CB];
+    __shared__ C tw[N];
+    const int tid = threadIdx.x;
+    const long q0 = (long)xcd_block(blockIdx.x, gridDim.x) * CB;
+    for (int m = tid; m < N; m += WG) tw[m] = tw_g[m];
+    const int c = tid % CB, t0 = tid / CB;
+    const long q = q0 + c;
+    const bool live = q < Q;
+    const ColBuf<C, N, CB> A{lds, c}, B{lds + (PING ? N * CB : 0), c};
+    C keep[in_slots(N, TPT)];
+    fft_io<C, N, TPT, DIR, PING, false>(tw, t0, SrcKeep<C, LD>{ld, q, live, keep}, DstQ<C, STA>{sta, q, live}, A, B);
+    __syncthreads();
+    fft_io<C, N, TPT, DIR, PING, false>(tw, t0, SrcKeptIK<C>{keep, kax}, DstQ<C, STB>{stb, q, live}, A, B);
 }
 
 // Real rows -> half-spectrum rows (forward, unnormalised).
@@ -204,7 +260,7 @@ struct MergeLoad {
     long row;
     bool live;
     const C *__restrict__ twN;
-    __device__ __forceinline__ C ld(int k) const
+    __device__ __forceinline__ C ld(int k, int = 0) const
     {
         using R = decltype(C().x);
         if (!live) return mk<C>((R)0, (R)0);
@@ -317,6 +373,33 @@ void launch_strided(int n, long Q, const C2<R> *tw, LD ld, ST_ st, hipStream_t s
 #undef KF_CASE
     default:
         throw Error("unsupported transform length " + std::to_string(n));
+    }
+}
+
+template <typename R, int DIR, class LD, class STA, class STB>
+void launch_strided_ik(int n, long Q, const C2<R> *tw, LD ld, STA sta, STB stb, const R *kax, hipStream_t s,
+                       int fam, int var = 0)
+{
+    prof::Scope ps_(fam, s);
+    switch (n) {
+#define KF_CASE(NN)                                                                            \
+    case NN: {                                                                                 \
+        constexpr int CB0 = cb_variant<R>(NN, 0), CB1 = cb_variant<R>(NN, 1);                  \
+        if (var == 1 && CB1 != CB0) {                                                          \
+            dim3 grid((unsigned)((Q + CB1 - 1) / CB1));                                        \
+            hipLaunchKernelGGL((k_strided_ik<R, NN, CB1, DIR, LD, STA, STB>), grid,            \
+                               dim3(tpt_for(NN) * CB1), 0, s, tw, Q, ld, sta, stb, kax);       \
+        } else {                                                                               \
+            dim3 grid((unsigned)((Q + CB0 - 1) / CB0));                                        \
+            hipLaunchKernelGGL((k_strided_ik<R, NN, CB0, DIR, LD, STA, STB>), grid,            \
+                               dim3(tpt_for(NN) * CB0), 0, s, tw, Q, ld, sta, stb, kax);       \
+        }                                                                                      \
+        break;                                                                                 \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported FFT length " + std::to_string(n));
     }
 }
 

@@ -108,10 +108,11 @@ __device__ __forceinline__ void dft(C *v)
 //   butterfly j (0 <= j < N/R_):  inputs  x[j + q*N/R_]
 //   twiddle  w^(q * (j mod NS) * N/(NS*R_))
 //   outputs  y[(j/NS)*NS*R_ + (j mod NS) + q*NS]
-// Inputs come from src.ld(t), outputs go to dst.st(t, v, slot): global
-// memory (the load/store functors of the kernel) for the first and the last
-// pass, an LDS buffer in between; slot (< out_slots) numbers a thread's
-// outputs at compile time, for destinations that keep them in registers.  All of a thread's loads are issued before its
+// Inputs come from src.ld(t, slot), outputs go to dst.st(t, v, slot):
+// global memory (the load/store functors of the kernel) for the first and
+// the last pass, an LDS buffer in between; slot numbers a thread's inputs
+// (< in_slots) and outputs (< out_slots) at compile time, for sources and
+// destinations that keep them in registers.  All of a thread's loads are issued before its
 // arithmetic.  SYNC_MID: src and dst are the same LDS buffer, so every thread
 // must finish reading before any writes.
 template <typename C, int N, int R_, int NS, int TPT, int DIR, bool SYNC_MID, class SRC, class DST>
@@ -125,7 +126,7 @@ __device__ __forceinline__ void pass_io(const C *tw, int tt, const SRC &src, con
         const int j = tt + m * TPT;
         if (NB % TPT == 0 || j < NB) {
 #pragma unroll
-            for (int q = 0; q < R_; ++q) v[m][q] = src.ld(j + q * NB);
+            for (int q = 0; q < R_; ++q) v[m][q] = src.ld(j + q * NB, m * R_ + q);
         }
     }
 #pragma unroll
@@ -195,6 +196,13 @@ __device__ __forceinline__ void fft_io(const C *tw, int tt, const SRC &src, cons
     chain_io<C, N, N, 1, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, B, A);
 }
 
+// inputs per thread of the first pass of a length-n transform by tpt threads
+__host__ __device__ constexpr int in_slots(int n, int tpt)
+{
+    const int r = next_radix(n);
+    return ((n / r + tpt - 1) / tpt) * r;
+}
+
 // outputs per thread of the last pass of a length-n transform by tpt threads
 __host__ __device__ constexpr int out_slots(int n, int tpt)
 {
@@ -230,7 +238,7 @@ struct ColBuf {
     C *p;
     int c;
     static constexpr int A = col_swz_a(N, CB * (int)sizeof(C)), MASK = col_swz_mask(N, CB * (int)sizeof(C));
-    __device__ __forceinline__ C ld(int t) const { return p[swz(t, A, MASK) * CB + c]; }
+    __device__ __forceinline__ C ld(int t, int = 0) const { return p[swz(t, A, MASK) * CB + c]; }
     __device__ __forceinline__ void st(int t, C v, int = 0) const { p[swz(t, A, MASK) * CB + c] = v; }
 };
 
@@ -275,7 +283,7 @@ struct RowBuf {
     int c;
     static constexpr RowLay L = row_layout(M, (int)sizeof(C));
     static constexpr int SC = M + L.pad;
-    __device__ __forceinline__ C ld(int t) const { return p[c * SC + swz(t, L.a, L.mask)]; }
+    __device__ __forceinline__ C ld(int t, int = 0) const { return p[c * SC + swz(t, L.a, L.mask)]; }
     __device__ __forceinline__ void st(int t, C v, int = 0) const { p[c * SC + swz(t, L.a, L.mask)] = v; }
 };
 

@@ -355,12 +355,12 @@ inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C
     const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
     const long qx = (long)p.ny * p.nzc, qy = scratch_cols_y(p);
     const unsigned nv = (unsigned)p.nzc;
-    launch_strided<R, +1>(p.nx, qx, p.tw_x, xload<R>(F, gv, 1, -1), StorePlain<R>{H, ax}, s, prof::FFT_X, p.vx);
-    launch_strided<R, +1>(p.nx, qx, p.tw_x, xload<R>(F, gv, 1, 0), StorePlain<R>{Hx, ax}, s, prof::FFT_X, p.vx);
-    launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMaskedK<R>{H, ay, nv, gv.k[1]}, StoreMasked<R>{Hy, ay, nv}, s,
-                          prof::FFT_Y, p.vy);
-    launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv}, s, prof::FFT_Y,
-                          p.vy);
+    // one load of F: D23 F -> H and i k_x D23 F -> Hx
+    launch_strided_ik<R, +1>(p.nx, qx, p.tw_x, xload<R>(F, gv, 1, -1), StorePlain<R>{H, ax}, StorePlain<R>{Hx, ax},
+                             gv.k[0], s, prof::FFT_X, p.vx);
+    // one load of H: H in place and i k_y H -> Hy
+    launch_strided_ik<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv},
+                             StoreMasked<R>{Hy, ay, nv}, gv.k[1], s, prof::FFT_Y, p.vy);
     launch_strided<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{Hx, ay, nv}, StoreMasked<R>{Hx, ay, nv}, s,
                           prof::FFT_Y, p.vy);
 }
