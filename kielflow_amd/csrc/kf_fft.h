@@ -276,6 +276,63 @@ struct MergeLoad {
     }
 };
 
+// Whether a C2R z pass of half length M can form its merged inputs in
+// registers: the first pass gives each of the row's TPT lanes one butterfly
+// (its inputs t = tt + TPT q), and the row's lanes are a power of two within
+// a wave, so X[M - t] sits in lane (TPT - tt) % TPT, slot R1 - 1 - q
+__host__ __device__ constexpr bool merge_in_regs(int m)
+{
+    const int tpt = tpt_for(m);
+    return in_slots(m, tpt) == m / tpt && m / tpt == next_radix(m) && (tpt & (tpt - 1)) == 0 && tpt <= 64;
+}
+
+// The merged first-pass inputs Z[t] = (X[t] + conj X[M-t]) + i conj(W^t)
+// (X[t] - conj X[M-t]) of the thread's slots, with one load per element:
+// the partner X[M-t] comes by lane shuffle (lane 0 of a row holds its own
+// partners, and loads X[M]).  Im X_0 and Im X_M are dropped.
+template <typename C, int M, class LD>
+__device__ __forceinline__ void merge_regs(const LD &f, long row, bool live, int tt, const C *__restrict__ twN,
+                                           C *z)
+{
+    using R = decltype(C().x);
+    constexpr int TPT = tpt_for(M);
+    constexpr int R1 = M / TPT;
+    C xk[R1];
+#pragma unroll
+    for (int q = 0; q < R1; ++q) xk[q] = live ? f(row, tt + q * TPT) : mk<C>((R)0, (R)0);
+    const C xm = (live && tt == 0) ? f(row, M) : mk<C>((R)0, (R)0);
+    const int lane = (int)(threadIdx.x & 63);
+    const int src = (lane - tt) + ((TPT - tt) & (TPT - 1));
+#pragma unroll
+    for (int q = 0; q < R1; ++q) {
+        C p;
+        p.x = __shfl(xk[R1 - 1 - q].x, src, 64);
+        p.y = __shfl(xk[R1 - 1 - q].y, src, 64);
+        C a = xk[q];
+        if (tt == 0) {
+            if (q == 0) {
+                p = xm;
+                a.y = (R)0;
+                p.y = (R)0;
+            } else {
+                p = xk[R1 - q];
+            }
+        }
+        p = cconj(p);
+        const int t = tt + q * TPT;
+        const C ev = cadd(a, p);
+        const C od = cmul(cconj(twN[t]), csub(a, p));
+        z[q] = mk<C>(ev.x - od.y, ev.y + od.x);
+    }
+}
+
+// first-pass source from registers (slot = the thread's input slot)
+template <typename C>
+struct SrcRegs {
+    const C *z;
+    __device__ __forceinline__ C ld(int, int slot) const { return z[slot]; }
+};
+
 template <typename R, int M, int RB, class LD, class ST_>
 __global__ void __launch_bounds__(tpt_for(M) * RB)
 k_zc2r(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, LD ld,
@@ -294,8 +351,14 @@ k_zc2r(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long ro
     const long row = r0 + c;
     const bool live = row < rows;
     const Buf A{lds, c};
-    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, LD>{ld, row, live, twN_g},
-                                        DstQ<C, ST_>{st, row, live}, A, A);
+    if constexpr (merge_in_regs(M)) {
+        C z[M / TPT];
+        merge_regs<C, M>(ld, row, live, tt, twN_g, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z}, DstQ<C, ST_>{st, row, live}, A, A);
+    } else {
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, LD>{ld, row, live, twN_g},
+                                            DstQ<C, ST_>{st, row, live}, A, A);
+    }
 }
 
 // ------------------------------------------------------------------ plan

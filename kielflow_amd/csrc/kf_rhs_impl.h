@@ -134,12 +134,15 @@ struct ZStore {
     int pdir;
     double *od;         // ZK_RHO64: rho in FP64
     const double *ad;   // ZK_RHO64: n_d in FP64
-    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const
+    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const { apply<-1>(row, n, v); }
+    // K >= 0: the kind fixed at compile time (ZStoreK), so only its code is built
+    template <int K>
+    __device__ __forceinline__ void apply(long row, int n, C2<R> v) const
     {
         using C = C2<R>;
         const R x0 = v.x * scale, x1 = v.y * scale;
         const long base = row * nz + 2 * n;
-        switch (kind) {
+        switch (K >= 0 ? K : kind) {
         case ZK_WRITE:
             *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
             break;
@@ -223,13 +226,16 @@ struct XStore {
         ln = mk<C>(ln.x + da.x, ln.y + da.y);
         return l;
     }
-    __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const
+    __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const { apply<-1>(q, t, X); }
+    // K >= 0: the kind fixed at compile time (XStoreK), so only its code is built
+    template <int K>
+    __device__ __forceinline__ void apply(long q, int t, C2<R> X) const
     {
         using C = C2<R>;
         int i, j, k;
         long idx;
         decode_x(g, q, t, i, j, k, idx);
-        switch (kind) {
+        switch (K >= 0 ? K : kind) {
         case XK_ADJ: {
             C v = mk<C>((R)0, (R)0);
             if (!d23 || keep23(g, i, j, k)) {
@@ -292,6 +298,70 @@ struct XStore {
     }
 };
 
+// Store functors with the kind fixed at compile time: a kernel then holds
+// only that kind's code (the run-time switch builds every kind into every
+// pass kernel)
+template <typename R, int K>
+struct ZStoreK : ZStore<R> {
+    __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const { this->template apply<K>(row, n, v); }
+};
+template <typename R, int K>
+struct XStoreK : XStore<R> {
+    __device__ __forceinline__ void operator()(long q, int t, C2<R> X) const { this->template apply<K>(q, t, X); }
+};
+
+// call f with the store as its compile-time-kind wrapper for the kinds of
+// the fused RHS, else as is
+template <typename R, class F>
+inline void with_zstore(const ZStore<R> &z, F &&f)
+{
+    switch (z.kind) {
+    case ZK_WRITE: f(ZStoreK<R, ZK_WRITE>{z}); break;
+    case ZK_RHO: f(ZStoreK<R, ZK_RHO>{z}); break;
+    case ZK_RHO64: f(ZStoreK<R, ZK_RHO64>{z}); break;
+    case ZK_PEN: f(ZStoreK<R, ZK_PEN>{z}); break;
+    default: f(z);
+    }
+}
+template <typename R, class F>
+inline void with_xstore(const XStore<R> &x, F &&f)
+{
+    switch (x.kind) {
+    case XK_NEG_D23: f(XStoreK<R, XK_NEG_D23>{x}); break;
+    case XK_NEG_ACC: f(XStoreK<R, XK_NEG_ACC>{x}); break;
+    case XK_LIN: f(XStoreK<R, XK_LIN>{x}); break;
+    default: f(x);
+    }
+}
+
+// first load of an inverse transform with at most the 2/3 rule (no
+// derivative, weight or filter): XLoadSpec's common case, built alone
+template <typename R, int D23>
+struct XLoadMask {
+    const C2<R> *src;
+    GridView<R> g;
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const
+    {
+        int i, j, k;
+        long idx;
+        decode_x(g, q, t, i, j, k, idx);
+        const C2<R> v = src[idx];
+        if (D23 && !keep23(g, i, j, k)) return mk<C2<R>>((R)0, (R)0);
+        return v;
+    }
+};
+
+template <typename R, class F>
+inline void with_xload(const XLoadSpec<R> &l, F &&f)
+{
+    if (l.dir < 0 && !l.w && !l.filt) {
+        if (l.d23) f(XLoadMask<R, 1>{l.src, l.g});
+        else f(XLoadMask<R, 0>{l.src, l.g});
+    } else {
+        f(l);
+    }
+}
+
 template <typename R, typename RS>
 struct RowLoadConv {
     const RS *p;
@@ -310,11 +380,15 @@ inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
 {
     GridView<R> gv = g.view<R>();
     const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
-    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, xload<R>(src, gv, d23, dir),
-                          StorePlain<R>{p.scratch, ax}, s, prof::FFT_X, p.vx);
+    with_xload<R>(xload<R>(src, gv, d23, dir), [&](const auto &ld) {
+        launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, ld, StorePlain<R>{p.scratch, ax}, s, prof::FFT_X,
+                              p.vx);
+    });
     launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
                           StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
-    launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzs}, st, s);
+    with_zstore<R>(st, [&](const auto &z) {
+        launch_zc2r<R>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<R>{p.scratch, p.nzs}, z, s);
+    });
 }
 
 // y pass load of the scratch layout with i k_y (the y index is t) applied
@@ -356,7 +430,7 @@ inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C
     const long qx = (long)p.ny * p.nzc, qy = scratch_cols_y(p);
     const unsigned nv = (unsigned)p.nzc;
     // one load of F: D23 F -> H and i k_x D23 F -> Hx
-    launch_strided_ik<R, +1>(p.nx, qx, p.tw_x, xload<R>(F, gv, 1, -1), StorePlain<R>{H, ax}, StorePlain<R>{Hx, ax},
+    launch_strided_ik<R, +1>(p.nx, qx, p.tw_x, XLoadMask<R, 1>{F, gv}, StorePlain<R>{H, ax}, StorePlain<R>{Hx, ax},
                              gv.k[0], s, prof::FFT_X, p.vx);
     // one load of H: H in place and i k_y H -> Hy
     launch_strided_ik<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv},
@@ -417,14 +491,29 @@ k_zprod3(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long 
     for (int e = 0; e < E; ++e) pos[e] = -1;
     const RowLoadC<R> lx{Hx, nzs}, ly{Hy, nzs};
     const RowLoadCK<R> lz{H, nzs, kz};
-    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{lx, row, live, twN_g},
-                                        ProdAcc<R>{acc, pos, U0, row, nz, scale, true, live}, A, A);
-    __syncthreads();
-    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ly, row, live, twN_g},
-                                        ProdAcc<R>{acc, pos, U1, row, nz, scale, false, live}, A, A);
-    __syncthreads();
-    fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadCK<R>>{lz, row, live, twN_g},
-                                        ProdAcc<R>{acc, pos, U2, row, nz, scale, false, live}, A, A);
+    if constexpr (merge_in_regs(M)) {
+        C z[M / TPT];
+        merge_regs<C, M>(lx, row, live, tt, twN_g, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z}, ProdAcc<R>{acc, pos, U0, row, nz, scale, true, live},
+                                            A, A);
+        __syncthreads();
+        merge_regs<C, M>(ly, row, live, tt, twN_g, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z},
+                                            ProdAcc<R>{acc, pos, U1, row, nz, scale, false, live}, A, A);
+        __syncthreads();
+        merge_regs<C, M>(lz, row, live, tt, twN_g, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z},
+                                            ProdAcc<R>{acc, pos, U2, row, nz, scale, false, live}, A, A);
+    } else {
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{lx, row, live, twN_g},
+                                            ProdAcc<R>{acc, pos, U0, row, nz, scale, true, live}, A, A);
+        __syncthreads();
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ly, row, live, twN_g},
+                                            ProdAcc<R>{acc, pos, U1, row, nz, scale, false, live}, A, A);
+        __syncthreads();
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadCK<R>>{lz, row, live, twN_g},
+                                            ProdAcc<R>{acc, pos, U2, row, nz, scale, false, live}, A, A);
+    }
     if (live) {
         C *o = reinterpret_cast<C *>(adv + row * nz);
 #pragma unroll
@@ -461,10 +550,12 @@ template <typename R>
 inline void z_staged(Grid &g, Plan<R> &p, const C2<R> *H, bool dz, const ZStore<R> &st, hipStream_t s)
 {
     const long rows = (long)p.nx * p.ny;
-    if (dz)
-        launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadCK<R>{H, p.nzs, g.view<R>().k[2]}, st, s);
-    else
-        launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{H, p.nzs}, st, s);
+    with_zstore<R>(st, [&](const auto &z) {
+        if (dz)
+            launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadCK<R>{H, p.nzs, g.view<R>().k[2]}, z, s);
+        else
+            launch_zc2r<R>(p.nz / 2, rows, p.tw_m, p.tw_n, RowLoadC<R>{H, p.nzs}, z, s);
+    });
 }
 
 template <typename R, typename RS>
@@ -475,7 +566,9 @@ inline void r2c_store(Grid &g, Plan<R> &p, const RS *src, const XStore<R> &st, h
                    RowStoreC<R>{p.scratch, p.nzs}, s);
     launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
                           StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
-    launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, st, s, prof::FFT_X, p.vx);
+    with_xstore<R>(st, [&](const auto &x) {
+        launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, x, s, prof::FFT_X, p.vx);
+    });
 }
 
 template <typename R>
