@@ -334,9 +334,9 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
     ld.w = 1;
     {
         GridView<double> gv = g_.view<double>();
-        const StridedAddr ax = saddr_x(p_), ay = saddr_y(p_);
-        launch_strided<double, +1>(p_.nx, (long)p_.ny * p_.nzc, p_.tw_x, ld, StorePlain<double>{p_.scratch, ax}, s_,
-                                   prof::FFT_X, p_.vx);
+        const StridedAddr ay = saddr_y(p_);
+        launch_strided<double, +1>(p_.nx, scratch_cols_x(p_), p_.tw_x, pad_cols(ld, p_), scratch_store_x(p_.scratch, p_),
+                                   s_, prof::FFT_X, p_.vx);
         launch_strided<double, +1>(p_.ny, scratch_cols_y(p_), p_.tw_y, LoadMasked<double>{p_.scratch, ay, (unsigned)p_.nzc},
                                StoreMasked<double>{p_.scratch, ay, (unsigned)p_.nzc}, s_, prof::FFT_Y, p_.vy);
         launch_zc2r<double>(p_.nz / 2, (long)p_.nx * p_.ny, p_.tw_m, p_.tw_n, RowLoadC<double>{p_.scratch, p_.nzs}, z,
@@ -362,8 +362,9 @@ static void c2r_w(Grid &g, Plan<double> &p, const double2 *src, int d23, int fil
     ZStore<double> z = zs<double>(ZK_WRITE, p);
     z.o0 = out;
     z.scale = scale;
-    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
-    launch_strided<double, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, ld, StorePlain<double>{p.scratch, ax}, s, prof::FFT_X, p.vx);
+    const StridedAddr ay = saddr_y(p);
+    launch_strided<double, +1>(p.nx, scratch_cols_x(p), p.tw_x, pad_cols(ld, p), scratch_store_x(p.scratch, p), s,
+                               prof::FFT_X, p.vx);
     launch_strided<double, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<double>{p.scratch, ay, (unsigned)p.nzc},
                                StoreMasked<double>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
     launch_zc2r<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<double>{p.scratch, p.nzs}, z, s);

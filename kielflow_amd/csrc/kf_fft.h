@@ -534,8 +534,9 @@ struct StoreMasked {
     }
 };
 
-// scratch-side addressing: y pass over Nx*nzs columns (padding masked),
-// x pass over the Ny*Nzc columns of the unpadded spectral layout
+// scratch-side addressing: y pass over Nx*nzs columns (padding masked);
+// saddr_x: forward x pass over the Ny*Nzc columns of the unpadded spectral
+// layout (its stores, to that layout, stay whole lines)
 template <typename R>
 inline StridedAddr saddr_y(const Plan<R> &p)
 {
@@ -550,6 +551,38 @@ template <typename R>
 inline long scratch_cols_y(const Plan<R> &p)
 {
     return (long)p.nx * p.nzs;
+}
+
+// Inverse x pass from the [Nx][Ny][Nzc] layout into the scratch: it walks
+// the Ny*nzs scratch columns (y, kz), so that a workgroup's CB consecutive
+// columns are whole 128-byte lines of the scratch rows it writes (walking
+// the Ny*Nzc unpadded columns instead puts most of its stores across two
+// lines).  Columns kz >= Nzc are padding: no load, no store.
+template <typename R, class LD>
+struct PadColLoad {
+    LD f;
+    unsigned nzs, nzc;
+    __device__ __forceinline__ C2<R> operator()(long q, int t) const
+    {
+        const unsigned u = (unsigned)q, r = u / nzs, k = u - r * nzs;
+        if (k >= nzc) return mk<C2<R>>((R)0, (R)0);
+        return f((long)(r * nzc + k), t);
+    }
+};
+template <typename R, class LD>
+inline PadColLoad<R, LD> pad_cols(const LD &f, const Plan<R> &p)
+{
+    return PadColLoad<R, LD>{f, (unsigned)p.nzs, (unsigned)p.nzc};
+}
+template <typename R>
+inline StoreMasked<R> scratch_store_x(C2<R> *dst, const Plan<R> &p)
+{
+    return StoreMasked<R>{dst, StridedAddr{p.nzs, p.nzs, (long)p.ny * p.nzs}, (unsigned)p.nzc};
+}
+template <typename R>
+inline long scratch_cols_x(const Plan<R> &p)
+{
+    return (long)p.ny * p.nzs;
 }
 
 // address helpers for the y and x passes on the [Nx][Ny][Nzc] layout

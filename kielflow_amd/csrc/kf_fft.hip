@@ -70,10 +70,16 @@ static void autotune(Plan<R> &p, hipStream_t s)
         float ms[7];
         for (int r = -1; r < 7; ++r) {
             KF_HIP(hipEventRecord(e0, s));
-            if (pass == 0)
+            if (pass == 0) {
+                // the x passes as the transforms run them: inverse from the
+                // unpadded layout into the scratch, forward back out of it
+                // (in place on the zeroed scratch: every value stays zero)
+                const StridedAddr ux = addr_x(p.nx, p.ny, p.nzc);
+                launch_strided<R, +1>(p.nx, scratch_cols_x(p), p.tw_x, pad_cols(LoadPlain<R>{p.scratch, ux}, p),
+                                      scratch_store_x(p.scratch, p), s, 1000, var);
                 launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax},
-                                      StorePlain<R>{p.scratch, ax}, s, 1000, var);
-            else
+                                      StorePlain<R>{p.scratch, ux}, s, 1000, var);
+            } else
                 launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
                                       StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, 1000, var);
             KF_HIP(hipEventRecord(e1, s));
@@ -165,9 +171,9 @@ void r2c(const Plan<R> &p, const R *in, C2<R> *out, hipStream_t s)
 template <typename R>
 void c2r(const Plan<R> &p, const C2<R> *in, R *out, hipStream_t s)
 {
-    const StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), sx = saddr_x(p), sy = saddr_y(p);
-    launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{in, ax},
-                          StorePlain<R>{p.scratch, sx}, s, prof::FFT_X, p.vx);
+    const StridedAddr ax = addr_x(p.nx, p.ny, p.nzc), sy = saddr_y(p);
+    launch_strided<R, +1>(p.nx, scratch_cols_x(p), p.tw_x, pad_cols(LoadPlain<R>{in, ax}, p),
+                          scratch_store_x(p.scratch, p), s, prof::FFT_X, p.vx);
     launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, sy, (unsigned)p.nzc},
                           StoreMasked<R>{p.scratch, sy, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
     const long rows = (long)p.nx * p.ny;

@@ -379,10 +379,10 @@ inline void c2r_spec(Grid &g, Plan<R> &p, const C2<R> *src, int d23, int dir, co
                      hipStream_t s)
 {
     GridView<R> gv = g.view<R>();
-    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
+    const StridedAddr ay = saddr_y(p);
     with_xload<R>(xload<R>(src, gv, d23, dir), [&](const auto &ld) {
-        launch_strided<R, +1>(p.nx, (long)p.ny * p.nzc, p.tw_x, ld, StorePlain<R>{p.scratch, ax}, s, prof::FFT_X,
-                              p.vx);
+        launch_strided<R, +1>(p.nx, scratch_cols_x(p), p.tw_x, pad_cols(ld, p), scratch_store_x(p.scratch, p), s,
+                              prof::FFT_X, p.vx);
     });
     launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
                           StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
@@ -426,12 +426,12 @@ template <typename R>
 inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C2<R> *Hy, hipStream_t s)
 {
     GridView<R> gv = g.view<R>();
-    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
-    const long qx = (long)p.ny * p.nzc, qy = scratch_cols_y(p);
+    const StridedAddr ay = saddr_y(p);
+    const long qx = scratch_cols_x(p), qy = scratch_cols_y(p);
     const unsigned nv = (unsigned)p.nzc;
     // one load of F: D23 F -> H and i k_x D23 F -> Hx
-    launch_strided_ik<R, +1>(p.nx, qx, p.tw_x, XLoadMask<R, 1>{F, gv}, StorePlain<R>{H, ax}, StorePlain<R>{Hx, ax},
-                             gv.k[0], s, prof::FFT_X, p.vx);
+    launch_strided_ik<R, +1>(p.nx, qx, p.tw_x, pad_cols(XLoadMask<R, 1>{F, gv}, p), scratch_store_x(H, p),
+                             scratch_store_x(Hx, p), gv.k[0], s, prof::FFT_X, p.vx);
     // one load of H: H in place and i k_y H -> Hy
     launch_strided_ik<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv},
                              StoreMasked<R>{Hy, ay, nv}, gv.k[1], s, prof::FFT_Y, p.vy);
