@@ -141,7 +141,9 @@ k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
 
 // Two strided transforms of every column of CB, sharing one load: out A =
 // FFT(x), out B = FFT(i k[t] x) with k the wavenumbers of the pass axis (t
-// the position along it).  The inputs of the first pass stay in registers.
+// the position along it).  The inputs of the first pass stay in registers,
+// which roughly doubles the registers of a single pass; with one LDS buffer
+// (no ping-pong) the narrow column block is bounded by registers alone.
 template <typename R, int N, int CB, int DIR, class LD, class STA, class STB>
 __global__ void __launch_bounds__(tpt_for(N) * CB)
 k_strided_ik(const C2<R> *__restrict__ tw_g, long Q, LD ld, STA sta, STB stb, const R *__restrict__ kax)
@@ -149,7 +151,7 @@ k_strided_ik(const C2<R> *__restrict__ tw_g, long Q, LD ld, STA sta, STB stb, co
     using C = C2<R>;
     constexpr int TPT = tpt_for(N);
     constexpr int WG = TPT * CB;
-    constexpr bool PING = 2 * N * CB * (int)sizeof(C) <= 32768;
+    constexpr bool PING = false;
     __shared__ C lds[(PING ? 2 : 1) * N * CB];
     __shared__ C tw[N];
     const int tid = threadIdx.x;
