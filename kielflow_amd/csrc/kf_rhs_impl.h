@@ -30,12 +30,18 @@ __device__ __forceinline__ R sv_factor(const GridView<R> &g, int i, int j, int k
     return (R)0.5 * ((R)1 - exp(-(R)8 * (s * s)));
 }
 
-// exp(-2000 kabs^15) with the z norm of operations.hpp:23 (rhs_standard.cpp:220)
+// exp(-2000 kabs^15) with the z norm of operations.hpp:23 (rhs_standard.cpp:220).
+// kabs^15 = s^7 sqrt(s) for s = kabs^2, by multiplication: a few ulps from a
+// general pow, at a fraction of its instructions (it runs for every element
+// of the passes that filter Phi)
 template <typename R>
 __device__ __forceinline__ R phi_filter(const GridView<R> &g, int i, int j, int k)
 {
-    const R kab = sqrt((R)((g.q[0][i] + g.q[1][j]) + g.qzf[k]));
-    return exp((R)-2000 * pow(kab, (R)15));
+    const R s = (R)((g.q[0][i] + g.q[1][j]) + g.qzf[k]);
+    const R kab = sqrt(s);
+    const R s2 = s * s;
+    const R k15 = ((s2 * s2) * (s2 * s)) * kab;
+    return exp((R)-2000 * k15);
 }
 
 template <typename R>
@@ -267,16 +273,16 @@ struct XStore {
             break;
         }
         case XK_F1: {
-            const R f = phi_filter(g, i, j, k);
+            const R f = phi_filter(g, i, j, k), rth = (R)1 / theta;
             C n = a0[idx];
-            o0[idx] = mk<C>(f * X.x + n.x / theta, f * X.y + n.y / theta);
+            o0[idx] = mk<C>(f * X.x + n.x * rth, f * X.y + n.y * rth);
             break;
         }
         case XK_LIN: {
             // rhs_standard.cpp:119-140, 219-247, 279-319 on an equidistant grid
-            const R f = phi_filter(g, i, j, k);
+            const R f = phi_filter(g, i, j, k), rth = (R)1 / theta;
             const C fn = a3[idx];
-            const C F1 = mk<C>(f * X.x + fn.x / theta, f * X.y + fn.y / theta);
+            const C F1 = mk<C>(f * X.x + fn.x * rth, f * X.y + fn.y * rth);
             const bool kp = keep23(g, i, j, k);
             const R sv = sv_on ? sv_factor(g, i, j, k) : (R)0;
             const R kx = g.k[0][i], ky = g.k[1][j], kz = g.k[2][k];
