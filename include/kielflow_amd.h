@@ -164,7 +164,8 @@ int kf_poisson_destroy(kf_poisson p);
 int kf_poisson_set_tolerance(kf_poisson p, double tol);
 int kf_poisson_set_max_iterations(kf_poisson p, int iters);
 /* multigrid smoothing of the Newton-MG solver: pre/post red-black sweeps per
- * level and sweeps on the coarsest level (defaults 2, 2, 24) */
+ * level and sweeps on the coarsest level (defaults 2, 2, 24 in FP64, 3, 3,
+ * 24 in mixed precision) */
 int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse);
 /* solve(Phi_IO, rho): phi FP64 in/out (warm start), rho FP64.  Returns
  * KF_ERR_NOT_CONVERGED when the budget is spent; iters/resid may be NULL. */
@@ -223,9 +224,10 @@ int kf_batch_box(kf_batch b, int box, double **FUx, double **FUy, double **FUz, 
 /* n_d and solid mask of a box (device FP64 real fields, copied) */
 int kf_batch_set_fields(kf_batch b, int box, const double *nd, const double *mask);
 /* record the graphs; Poisson solves inside run exactly poisson_iterations
- * Newton iterations (unconverged solves are counted as failures).  The
- * order of the potential warm start (extrapolation in time) is recorded
- * too: capture after at least 5 kf_batch_steps, when it is steady (quartic) */
+ * Newton iterations (unconverged solves are counted as failures) with two
+ * pre and two post sweeps per level.  The order of the potential warm start
+ * (extrapolation in time) is recorded too: capture after at least 5
+ * kf_batch_steps, when it is steady (quartic) */
 int kf_batch_capture(kf_batch b, int poisson_iterations);
 int kf_batch_steps(kf_batch b, int nsteps);
 int kf_batch_synchronize(kf_batch b);

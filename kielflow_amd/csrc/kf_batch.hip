@@ -97,6 +97,9 @@ void Batch::capture(int poisson_iterations)
     for (Box *x : box_) {
         if (x->exec) continue;
         x->ps->set_graph_iterations(poisson_iterations);
+        // a captured solve runs a fixed number of Newton iterations, so
+        // extra sweeps per iteration save nothing: two pre and two post
+        x->ps->set_smoothing(2, 2, 24);
         hipGraph_t graph = nullptr;
         KF_HIP(hipStreamBeginCapture(x->s, hipStreamCaptureModeThreadLocal));
         if (x->k32) x->k32->step(x->F);
