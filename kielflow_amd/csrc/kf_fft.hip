@@ -1,6 +1,8 @@
 // kf_fft.hip -- plans and plain 3D R2C / C2R drivers (OP_FFT / OP_iFFT
 // semantics: OP_FFT.cpp:49-63 unnormalised forward, OP_iFFT.cpp:49-64 inverse
 // scaled by 1/N; the caller's input is preserved in both directions).
+#include <cstring>
+
 #include "kf_fft.h"
 
 #include <algorithm>
@@ -53,14 +55,22 @@ static C2<R> *twiddles(int n, int count, hipStream_t s)
 // default (variant 0, padded rows) is kept only when it is at least 5%
 // faster: the timed passes carry plain loads, and the fused loads of the RHS
 // favour the wider column blocks, so near-ties go to the default.
-// KF_FFT_AUTOTUNE=0 keeps variant 0 and the padded rows.  Every choice
-// computes the same numbers; only the speed changes.
+// KF_FFT_AUTOTUNE=0 keeps variant 0 and the padded rows; KF_FFT_AUTOTUNE=alt
+// takes variant 1 wherever it differs and unpadded rows (the other end of
+// every choice, for tests).  Every choice computes the same numbers; only
+// the speed changes.
 template <typename R>
 static void autotune(Plan<R> &p, hipStream_t s)
 {
     p.vx = p.vy = 0;
     const char *env = std::getenv("KF_FFT_AUTOTUNE");
     if (env && env[0] == '0') return;
+    if (env && std::strcmp(env, "alt") == 0) {
+        p.vx = cb_variant<R>(p.nx, 1) != cb_variant<R>(p.nx, 0) ? 1 : 0;
+        p.vy = cb_variant<R>(p.ny, 1) != cb_variant<R>(p.ny, 0) ? 1 : 0;
+        p.nzs = p.nzc;
+        return;
+    }
     KF_HIP(hipMemsetAsync(p.scratch, 0, sizeof(C2<R>) * (size_t)p.nx * p.ny * p.nzs, s));
     hipEvent_t e0, e1;
     KF_HIP(hipEventCreate(&e0));

@@ -1,0 +1,57 @@
+"""Every choice of the FFT plan's autotuner (the column-block variant of the
+x and y passes, padded or unpadded scratch rows) gives the same numbers: an
+RHS evaluation (rhs_standard::solve) with the defaults (KF_FFT_AUTOTUNE=0)
+and with the other end of every choice (KF_FFT_AUTOTUNE=alt) agree bit for
+bit, the potential included."""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import initial_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _eval(ctx, kind, n, prec, general, setting):
+    import kielflow_amd as kf
+    old = os.environ.get("KF_FFT_AUTOTUNE")
+    os.environ["KF_FFT_AUTOTUNE"] = setting
+    try:
+        state, nd, mask, og = initial_state(kind, n, seed=3, Q=-0.2)
+        rng = np.random.default_rng(7)
+        for a in range(3):
+            state[a] = og.dealias23(og.r2c(1e-2 * rng.standard_normal(og.rshape)))
+        g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+        p = kf.KF_FP64 if prec == "fp64" else kf.KF_MIXED
+        ps = kf.Poisson(g, p, tol=1e-12 if prec == "fp64" else 1e-10, max_it=80)
+        rhs = kf.Rhs(g, ps, M=1.2, tau=0.1, theta=100.0, sv_on=True, pen_on=True, precision=p)
+        if general:
+            rhs.force_general(True)
+        phi = g.rfield()
+        rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+        dF = [ctx.to_device(s) for s in state]
+        rhs.solve(0.0, dF)
+        assert rhs.poisson_failures() == 0
+        return [f.get() for f in dF] + [phi.get()]
+    finally:
+        if old is None:
+            os.environ.pop("KF_FFT_AUTOTUNE", None)
+        else:
+            os.environ["KF_FFT_AUTOTUNE"] = old
+
+
+@pytest.mark.parametrize("kind,n,prec,general", [
+    ("eqst", 24, "fp64", False),
+    ("eqst", 24, "fp64", True),
+    ("eqst", 32, "mixed", False),
+    ("eqst", 48, "mixed", False),
+    ("sist", 16, "fp64", False),
+])
+def test_autotuner_choices_give_identical_results(ctx, kind, n, prec, general):
+    print("fft variants", kind, n, prec, general, flush=True)
+    a = _eval(ctx, kind, n, prec, general, "0")
+    b = _eval(ctx, kind, n, prec, general, "alt")
+    names = ["FUx", "FUy", "FUz", "Fn", "phi"]
+    for name, x, y in zip(names, a, b):
+        assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
