@@ -570,6 +570,55 @@ __global__ void k_rb(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b,
     x[p] = (A - b[p]) / (D + e[p]);
 }
 
+// The red update of point (i, j, k) from x (its neighbours are all black)
+template <typename R>
+__device__ __forceinline__ R red_update(const LevelView<R> &L, const R *__restrict__ x, const R *__restrict__ b,
+                                        const R *__restrict__ e, int i, int j, int k)
+{
+    R A, D;
+    stencil<R>(L, x, i, j, k, A, D);
+    const long p = ((long)i * L.n[1] + j) * L.n[2] + k;
+    return (A - b[p]) / (D + e[p]);
+}
+
+// the red values of a sweep of x, recomputed where a black point reads them
+template <typename R>
+struct RedOf {
+    const LevelView<R> &L;
+    const R *__restrict__ x;
+    const R *__restrict__ b;
+    const R *__restrict__ e;
+    __device__ __forceinline__ R operator()(unsigned q) const
+    {
+        const unsigned nz = (unsigned)L.n[2], ny = (unsigned)L.n[1];
+        const unsigned r = q / nz, k = q - r * nz, i = r / ny, j = r - i * ny;
+        return red_update<R>(L, x, b, e, (int)i, (int)j, (int)k);
+    }
+};
+
+// One whole red-black sweep of x written to y (a second buffer): red points
+// as k_rb colour 0 computes them, black points from those red values, each
+// recomputed from x where a black point needs it.  The same arithmetic as
+// the two in-place half sweeps, in one launch; for small levels, whose
+// sweeps are launch-bound.
+template <typename R>
+__global__ void k_rb_full(LevelView<R> L, const R *__restrict__ x, R *__restrict__ y, const R *__restrict__ b,
+                          const R *__restrict__ e, const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= L.N / 2) return;
+    const int nz2 = L.n[2] / 2, ny = L.n[1];
+    int i, j, kk;
+    dec3(q, ny, nz2, i, j, kk);
+    const int kr = 2 * kk + ((i + j) & 1), kb = 2 * kk + ((i + j + 1) & 1);
+    const long row = ((long)i * ny + j) * L.n[2];
+    y[row + kr] = red_update<R>(L, x, b, e, i, j, kr);
+    R A, D;
+    stencil_ld<R>(L, RedOf<R>{L, x, b, e}, i, j, kb, A, D);
+    y[row + kb] = (A - b[row + kb]) / (D + e[row + kb]);
+}
+
 // the first red half sweep from x = 0: red = -b / (D + e) (the sweep's own
 // arithmetic with zero neighbours), black = 0; replaces zeroing x first
 template <typename R>
@@ -1265,6 +1314,11 @@ __global__ void k_hto2h(int stage, const double *__restrict__ in, int cx, int cy
 // ======================================================================== host
 static inline unsigned nblocks(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// largest level swept by k_rb_full (paired whole sweeps): up to 64^3 the
+// launch saved outweighs the recomputed red values; at 128^3 a whole sweep
+// takes 17 us against 2 x 5.7 us for the two half sweeps
+constexpr long FULL_SWEEP_N = 64L * 64 * 64;
+
 Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed), s0_(s)
 {
     // mixed mode: FP32 sweeps are cheap beside the FP64 residual and update
@@ -1426,7 +1480,20 @@ void Poisson::smooth(int l, int sweeps, hipStream_t s, bool zero)
         zero_level<R>(l, s);
         zero = false;
     }
+    // coarse levels (not the finest, whose r the Newton and Krylov loops
+    // use) up to 64^3: sweeps in pairs x -> r -> x, one launch each
+    const bool fuse = rb && !sp && l > 0 && L.N <= FULL_SWEEP_N;
     for (int it = 0; it < sweeps; ++it) {
+        if (fuse && !(zero && it == 0) && it + 1 < sweeps) {
+            const unsigned nb = nblocks(L.N / 2, 256);
+            const int *ctl = d_ctl_;
+            hipLaunchKernelGGL(k_rb_full<R>, dim3(nb), dim3(256), 0, s, v, (const R *)x, r, (const R *)b,
+                               (const R *)e, ctl);
+            hipLaunchKernelGGL(k_rb_full<R>, dim3(nb), dim3(256), 0, s, v, (const R *)r, x, (const R *)b,
+                               (const R *)e, ctl);
+            ++it;
+            continue;
+        }
         if (rb) {
             for (int c = 0; c < 2; ++c) {
                 prof::Scope ps_(l == 0 ? prof::RB0 : 1000, s);
