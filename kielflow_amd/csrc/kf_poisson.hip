@@ -619,6 +619,40 @@ __global__ void k_rb_full(LevelView<R> L, const R *__restrict__ x, R *__restrict
     y[row + kb] = (A - b[row + kb]) / (D + e[row + kb]);
 }
 
+// the red values of the first sweep from x = 0, -b / (D + e)
+template <typename R>
+struct RedFromZero {
+    const LevelView<R> &L;
+    const R *__restrict__ b;
+    const R *__restrict__ e;
+    __device__ __forceinline__ R operator()(unsigned q) const
+    {
+        const unsigned nz = (unsigned)L.n[2], ny = (unsigned)L.n[1];
+        const unsigned r = q / nz, k = q - r * nz, i = r / ny, j = r - i * ny;
+        return ((R)0 - b[q]) / (diag<R>(L, (int)i, (int)j, (int)k) + e[q]);
+    }
+};
+
+// One whole sweep from x = 0 (k_rb_zero, then the black half sweep) in one
+// launch: it reads no x, so it writes x in place
+template <typename R>
+__global__ void k_rb_full_zero(LevelView<R> L, R *__restrict__ x, const R *__restrict__ b, const R *__restrict__ e,
+                               const int *ctl)
+{
+    if (ctl && ctl[0]) return;
+    const long q = (long)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (q >= L.N / 2) return;
+    const int nz2 = L.n[2] / 2, ny = L.n[1];
+    int i, j, kk;
+    dec3(q, ny, nz2, i, j, kk);
+    const int kr = 2 * kk + ((i + j) & 1), kb = 2 * kk + ((i + j + 1) & 1);
+    const long row = ((long)i * ny + j) * L.n[2];
+    x[row + kr] = ((R)0 - b[row + kr]) / (diag<R>(L, i, j, kr) + e[row + kr]);
+    R A, D;
+    stencil_ld<R>(L, RedFromZero<R>{L, b, e}, i, j, kb, A, D);
+    x[row + kb] = (A - b[row + kb]) / (D + e[row + kb]);
+}
+
 // the first red half sweep from x = 0: red = -b / (D + e) (the sweep's own
 // arithmetic with zero neighbours), black = 0; replaces zeroing x first
 template <typename R>
@@ -1484,7 +1518,12 @@ void Poisson::smooth(int l, int sweeps, hipStream_t s, bool zero)
     // use) up to 64^3: sweeps in pairs x -> r -> x, one launch each
     const bool fuse = rb && !sp && l > 0 && L.N <= FULL_SWEEP_N;
     for (int it = 0; it < sweeps; ++it) {
-        if (fuse && !(zero && it == 0) && it + 1 < sweeps) {
+        if (fuse && zero && it == 0) {
+            hipLaunchKernelGGL(k_rb_full_zero<R>, dim3(nblocks(L.N / 2, 256)), dim3(256), 0, s, v, x, (const R *)b,
+                               (const R *)e, (const int *)d_ctl_);
+            continue;
+        }
+        if (fuse && it + 1 < sweeps) {
             const unsigned nb = nblocks(L.N / 2, 256);
             const int *ctl = d_ctl_;
             hipLaunchKernelGGL(k_rb_full<R>, dim3(nb), dim3(256), 0, s, v, (const R *)x, r, (const R *)b,
