@@ -538,10 +538,13 @@ inline void zprod3(Grid &g, Plan<R> &p, const C2<R> *Hx, const C2<R> *Hy, const 
     const long rows = (long)p.nx * p.ny;
     const R *kz = g.view<R>().k[2];
     const R scale = (R)(1.0 / (double)p.nr);
+    // FP32: half the usual rows per workgroup.  The register count (112)
+    // allows four waves per SIMD either way; with four-wave workgroups a
+    // barrier holds fewer waves (111 us against 114-116 at 256^3).
     switch (p.nz / 2) {
 #define KF_CASE(NN)                                                                                            \
     case NN: {                                                                                                 \
-        constexpr int RB = cb_for<R>(NN);                                                                      \
+        constexpr int RB = cb_for<R>(NN) / (sizeof(R) == 4 ? 2 : 1);                                          \
         hipLaunchKernelGGL((k_zprod3<R, NN, RB>), dim3((unsigned)((rows + RB - 1) / RB)), dim3(tpt_for(NN) * RB), \
                            0, s, p.tw_m, p.tw_n, rows, p.nz, p.nzs, Hx, Hy, H, kz, U[0], U[1], U[2], adv, scale); \
         break;                                                                                                 \
