@@ -35,7 +35,7 @@ def parse():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
-    p.add_argument("--cpu-n", type=int, default=128, help="box size of the CPU sample")
+    p.add_argument("--cpu-n", type=int, default=64, help="box size of the CPU sample")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     p.add_argument("--adjoint-steps", type=int, default=200)
     p.add_argument("--no-batch", action="store_true", help="skip the batched-boxes leg")
@@ -106,24 +106,39 @@ def run_precision(kf, ctx, g, axes, prec, nd_d, mask_d, Fn0, steps, warmup, prof
 
 
 def cpu_baseline(n_sample, n_target):
-    """One RHS evaluation of the same configuration on a smaller box, timed
-    with the FP64 oracle on one host thread, scaled to n_target by the
-    n^3 log2 n work ratio of the transforms and per-point kernels."""
+    """The FP64 oracle (a C restatement of the reference path, no FFTW) timed
+    on the host: one RHS evaluation of the same configuration (Poisson to a
+    relative residual of 1e-10) on an n_sample^3 box, after one warm-up
+    evaluation, median of 3.  Once on one thread (the reference is
+    single-threaded: no OpenMP regions, no FFTW threads) and once on all host
+    threads.  A RK4 step is 4 RHS evaluations plus negligible AXPYs; the
+    rate is scaled to n_target^3 by the n^3 log2 n work ratio."""
     import kielflow_amd as kf
     from oracle import oracle as O
     O.build()
     og = O.Grid.cube(n_sample, L_BOX)
     axes = kf.cube(n_sample, L_BOX)
     ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=-0.3, a=2.0 * L_BOX / n_sample)
-    r = O.Rhs(og, M=1.5, tau=0.1, theta=100.0, sv_on=False, pen_on=True,
-              poisson_mode=O.POISSON_FAS, poisson_tol=1e-10, poisson_max=200)
-    r.set_fields(np.zeros(og.rshape), nd, mask)
-    st = [np.zeros(og.cshape, complex) for _ in range(3)] + [og.r2c(ln_n)]
-    t0 = time.perf_counter()
-    r.eval(0.0, st)
-    t_rhs = time.perf_counter() - t0
+    st0 = [np.zeros(og.cshape, complex) for _ in range(3)] + [og.r2c(ln_n)]
     scale = (n_target / n_sample) ** 3 * (np.log2(n_target) / np.log2(n_sample))
-    t_step = 4 * t_rhs * scale
+
+    def timed(threads):
+        O.set_threads(threads)
+        used = O.get_threads()
+        ts = []
+        for rep in range(4):
+            r = O.Rhs(og, M=1.5, tau=0.1, theta=100.0, sv_on=False, pen_on=True,
+                      poisson_mode=O.POISSON_FAS, poisson_tol=1e-10, poisson_max=200)
+            r.set_fields(np.zeros(og.rshape), nd, mask)
+            t0 = time.perf_counter()
+            r.eval(0.0, st0)
+            if rep:
+                ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), used
+
+    t1, c1 = timed(1)
+    all_threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    tn, cn = timed(all_threads)
     cpu = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -132,10 +147,15 @@ def cpu_baseline(n_sample, n_target):
                 break
     except OSError:
         pass
-    return {"value": 1.0 / t_step, "unit": "steps/s", "cores": 1, "kind": "port",
-            "sample": (f"one RHS evaluation (Poisson to 1e-10) of the same configuration at "
-                       f"{n_sample}^3 took {t_rhs:.2f} s on 1 thread of {cpu}; a {n_target}^3 RK4 step "
-                       f"(4 RHS) estimated as 4 x that x {scale:.2f} (n^3 log2 n ratio)"),
+    step1 = 4 * t1 * scale
+    stepn = 4 * tn * scale
+    return {"value": round(1.0 / step1, 6), "unit": "steps/s", "cores": c1, "kind": "port",
+            "sample": (f"one RHS evaluation (Poisson to 1e-10) of the same configuration at {n_sample}^3, "
+                       f"median of 3 after a warm-up: {t1:.3f} s on {c1} thread of {cpu}; a {n_target}^3 "
+                       f"RK4 step estimated as 4 RHS x {scale:.1f} (n^3 log2 n ratio)"),
+            "extrapolation_factor": round(4 * scale, 2),
+            "all_cores": {"value": round(1.0 / stepn, 6), "unit": "steps/s", "cores": cn,
+                          "rhs_seconds": round(tn, 3)},
             "host_threads_available": os.cpu_count()}
 
 

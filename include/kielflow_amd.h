@@ -94,7 +94,7 @@ int kf_memset(kf_ctx ctx, void *dst, int value, size_t bytes);
 int kf_grid_create(kf_ctx ctx, const kf_axis axes[3], kf_grid *out);
 int kf_grid_destroy(kf_grid g);
 int kf_grid_dims(kf_grid g, int *nx, int *ny, int *nz, size_t *nreal, size_t *ncplx);
-/* axis::val_at / k_val_at / dS (axis_interface.hpp:72-74, axis.hpp:44) */
+/* axis::val_at / k_val_at / dS (axis_interface.hpp:10-12, axis.hpp:44) */
 int kf_axis_values(const kf_axis *a, int index, double *val, double *k, double *dS);
 
 /* ---------------------------------------------------------------- FFT
@@ -167,6 +167,15 @@ int kf_poisson_set_max_iterations(kf_poisson p, int iters);
  * level and sweeps on the coarsest level (defaults 2, 2, 24 in FP64, 3, 3,
  * 24 in mixed precision) */
 int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse);
+/* solver_poisson_multigrid::set_level_control (solver_poisson_multigrid.cpp:
+ * 82-147), for the KF_POISSON_CASCADE mode: n level steps, each going down by
+ * -level_change[c] levels (1..3: lvl_down, lvl_down2, lvl_down3; injection),
+ * up one level (+1: lvl_up; OP_hto2h) or staying (0: lvl_keep), followed by
+ * the reference NLJ with steps[c] iterations (-1: up to 1000) and update
+ * tolerance tol[c].  The level may not rise above the base grid and must end
+ * on it.  Default: kielflow.cpp:207-217's {-3, 1, 1, 1}, {0, -1, -1, -1},
+ * {1e-2, 5e-4, 1e-4, 1e-4}. */
+int kf_poisson_set_cascade(kf_poisson p, int n, const int *level_change, const int *steps, const double *tol);
 /* solve(Phi_IO, rho): phi FP64 in/out (warm start), rho FP64.  Returns
  * KF_ERR_NOT_CONVERGED when the budget is spent; iters/resid may be NULL. */
 int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid);
@@ -203,6 +212,15 @@ int kf_rk4_solve(kf_rk4 k, double *FUx, double *FUy, double *FUz, double *Fn);
 int kf_rk4_steps(kf_rk4 k, int n, double *FUx, double *FUy, double *FUz, double *Fn);
 int kf_rk4_set_dt(kf_rk4 k, double dt);
 int kf_rk4_time(kf_rk4 k, double *t);
+/* The RK4 arithmetic of Runge_kutta_O4::solve for a right-hand side that is
+ * not kielflow_amd's (any interface_rhs): a stage input
+ * out = y0 + c k (RKO4.cpp:140-153, 167-180, 194-207; c = dt/2, dt/2, dt)
+ * and the update out = y0 + (dt/6)(1 k1 + 2 k2 + 2 k3 + 1 k4)
+ * (RKO4.cpp:225-238), over n doubles of device memory; out must not overlap
+ * the inputs. */
+int kf_rk4_stage_input(kf_ctx ctx, size_t n, double c, const double *y0, const double *k, double *out);
+int kf_rk4_combine(kf_ctx ctx, size_t n, double dt, const double *y0, const double *k1, const double *k2,
+                   const double *k3, const double *k4, double *out);
 
 /* cfl_all(dt, FUx, FUy, FUz) (CFL.cpp:29-82) */
 int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,

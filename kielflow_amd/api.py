@@ -81,6 +81,7 @@ SIGNATURES = {
     "kf_poisson_set_tolerance": (C.c_int, [_vp, C.c_double]),
     "kf_poisson_set_max_iterations": (C.c_int, [_vp, C.c_int]),
     "kf_poisson_set_smoothing": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "kf_poisson_set_cascade": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
     "kf_poisson_solve": (C.c_int, [_vp, _vp, _vp, _ip, _dp]),
     "kf_poisson_converged": (C.c_int, [_vp]),
     "kf_poisson_residual": (C.c_int, [_vp, _vp, _vp, _dp]),
@@ -99,6 +100,8 @@ SIGNATURES = {
     "kf_rk4_set_dt": (C.c_int, [_vp, C.c_double]),
     "kf_rk4_time": (C.c_int, [_vp, _dp]),
     "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
+    "kf_rk4_stage_input": (C.c_int, [_vp, C.c_size_t, C.c_double, _vp, _vp, _vp]),
+    "kf_rk4_combine": (C.c_int, [_vp, C.c_size_t, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
     "kf_adjoint_create": (C.c_int, [_vp, C.c_double, C.POINTER(_vp)]),
     "kf_adjoint_destroy": (C.c_int, [_vp]),
     "kf_adjoint_set_tolerance": (C.c_int, [_vp, C.c_double]),
@@ -176,6 +179,26 @@ def cube(n, L=16.0, type_=EQST, m=0.0):
 def _ax3(axes):
     arr = (Axis * 3)(*axes)
     return arr
+
+
+def _need(arr, shape, dtype, what):
+    """Raise unless `arr` is a device field of exactly this shape and dtype:
+    the library takes raw pointers and trusts their extents."""
+    if not hasattr(arr, "ptr") or not hasattr(arr, "nbytes"):
+        raise TypeError(f"{what}: expected a device field, got {type(arr).__name__}")
+    if arr.ptr is None:
+        raise ValueError(f"{what}: the device field has been freed")
+    dtype = np.dtype(dtype)
+    if tuple(arr.shape) != tuple(shape) or np.dtype(arr.dtype) != dtype:
+        raise ValueError(f"{what}: expected {dtype.name}{tuple(shape)}, got "
+                         f"{np.dtype(arr.dtype).name}{tuple(arr.shape)}")
+    return arr.ptr
+
+
+def _need4(F, grid, what):
+    if len(F) != 4:
+        raise ValueError(f"{what}: expected the 4 spectral fields (FUx, FUy, FUz, Fn), got {len(F)}")
+    return [_need(f, grid.cshape, np.complex128, f"{what}[{i}]") for i, f in enumerate(F)]
 
 
 class Context:
@@ -272,44 +295,54 @@ class Grid:
     def rfield(self, dtype=np.float64):
         return self.ctx.zeros(self.rshape, dtype)
 
+    def _c(self, X, what):
+        return _need(X, self.cshape, np.complex128, what)
+
+    def _r(self, X, what):
+        return _need(X, self.rshape, np.float64, what)
+
     def deriv(self, d: int, X: DeviceArray, out: DeviceArray):
-        check(self.ctx.lib.kf_partial_derivative(self.h, d, X.ptr, out.ptr))
+        check(self.ctx.lib.kf_partial_derivative(self.h, d, self._c(X, "input"), self._c(out, "output")))
 
     def dealias23(self, X: DeviceArray):
-        check(self.ctx.lib.kf_dealias23(self.h, X.ptr))
+        check(self.ctx.lib.kf_dealias23(self.h, self._c(X, "field")))
 
     def filter_phi(self, X: DeviceArray):
-        check(self.ctx.lib.kf_filter_phi(self.h, X.ptr))
+        check(self.ctx.lib.kf_filter_phi(self.h, self._c(X, "field")))
 
     # effect_* interfaces (terms of d/dt added to the output buffers)
     def effect_advection(self, FU, FXX, out):
-        check(self.ctx.lib.kf_effect_advection(self.h, FU[0].ptr, FU[1].ptr, FU[2].ptr, FXX.ptr, out.ptr))
+        check(self.ctx.lib.kf_effect_advection(self.h, *[self._c(f, "FU") for f in FU[:3]], self._c(FXX, "FXX"),
+                                               self._c(out, "out")))
 
     def effect_continuity(self, FU, Fni, out):
-        check(self.ctx.lib.kf_effect_continuity(self.h, FU[0].ptr, FU[1].ptr, FU[2].ptr, Fni.ptr, out.ptr))
+        check(self.ctx.lib.kf_effect_continuity(self.h, *[self._c(f, "FU") for f in FU[:3]], self._c(Fni, "Fni"),
+                                                self._c(out, "out")))
 
     def effect_force_E(self, FPhi, out):
-        check(self.ctx.lib.kf_effect_force_E(self.h, FPhi.ptr, out[0].ptr, out[1].ptr, out[2].ptr))
+        check(self.ctx.lib.kf_effect_force_E(self.h, self._c(FPhi, "FPhi"), *[self._c(o, "out") for o in out[:3]]))
 
     def effect_diffusion_u(self, theta, Fni, out):
-        check(self.ctx.lib.kf_effect_diffusion_u(self.h, theta, Fni.ptr, out[0].ptr, out[1].ptr, out[2].ptr))
+        check(self.ctx.lib.kf_effect_diffusion_u(self.h, theta, self._c(Fni, "Fni"),
+                                                 *[self._c(o, "out") for o in out[:3]]))
 
     def effect_spectral_viscosity(self, X, out):
-        check(self.ctx.lib.kf_effect_spectral_viscosity(self.h, X.ptr, out.ptr))
+        check(self.ctx.lib.kf_effect_spectral_viscosity(self.h, self._c(X, "input"), self._c(out, "out")))
 
     def effect_translation(self, v, X, out):
-        check(self.ctx.lib.kf_effect_translation(self.h, v, X.ptr, out.ptr))
+        check(self.ctx.lib.kf_effect_translation(self.h, v, self._c(X, "input"), self._c(out, "out")))
 
     def effect_force_linear(self, k, X, out):
-        check(self.ctx.lib.kf_effect_force_linear(self.h, k, X.ptr, out.ptr))
+        check(self.ctx.lib.kf_effect_force_linear(self.h, k, self._c(X, "input"), self._c(out, "out")))
 
     def effect_penalization(self, mask, eta, fix, fix_scaled_by_mask, X, out):
-        check(self.ctx.lib.kf_effect_penalization(self.h, mask.ptr, eta, fix, int(fix_scaled_by_mask), X.ptr,
-                                                  out.ptr))
+        check(self.ctx.lib.kf_effect_penalization(self.h, self._r(mask, "mask"), eta, fix, int(fix_scaled_by_mask),
+                                                  self._c(X, "input"), self._c(out, "out")))
 
     def cfl(self, dt, FUx, FUy, FUz):
         out = C.c_double(0)
-        check(self.ctx.lib.kf_cfl_all(self.h, dt, FUx.ptr, FUy.ptr, FUz.ptr, C.byref(out)))
+        check(self.ctx.lib.kf_cfl_all(self.h, dt, self._c(FUx, "FUx"), self._c(FUy, "FUy"), self._c(FUz, "FUz"),
+                                      C.byref(out)))
         return out.value
 
 
@@ -326,11 +359,20 @@ class FFT:
             self.grid.ctx.lib.kf_fft_destroy(self.h)
             self.h = None
 
+    def _dtypes(self):
+        if self.precision == KF_FP64:
+            return np.float64, np.complex128
+        return np.float32, np.complex64
+
     def r2c(self, x: DeviceArray, X: DeviceArray):
-        check(self.grid.ctx.lib.kf_fft_r2c(self.h, x.ptr, X.ptr))
+        rt, ct = self._dtypes()
+        check(self.grid.ctx.lib.kf_fft_r2c(self.h, _need(x, self.grid.rshape, rt, "r2c input"),
+                                           _need(X, self.grid.cshape, ct, "r2c output")))
 
     def c2r(self, X: DeviceArray, x: DeviceArray):
-        check(self.grid.ctx.lib.kf_fft_c2r(self.h, X.ptr, x.ptr))
+        rt, ct = self._dtypes()
+        check(self.grid.ctx.lib.kf_fft_c2r(self.h, _need(X, self.grid.cshape, ct, "c2r input"),
+                                           _need(x, self.grid.rshape, rt, "c2r output")))
 
 
 class Poisson:
@@ -356,17 +398,29 @@ class Poisson:
     def set_smoothing(self, pre, post, coarse=24):
         check(self.grid.ctx.lib.kf_poisson_set_smoothing(self.h, pre, post, coarse))
 
+    def set_cascade(self, level_change, steps, tol):
+        """Level program of the reference cascade (KF_POISSON_CASCADE)."""
+        n = len(level_change)
+        if not len(steps) == len(tol) == n:
+            raise ValueError("level_change, steps and tol need the same length")
+        check(self.grid.ctx.lib.kf_poisson_set_cascade(self.h, n, (C.c_int * n)(*level_change),
+                                                       (C.c_int * n)(*steps), (C.c_double * n)(*tol)))
+
     def solve(self, phi: DeviceArray, rho: DeviceArray, raise_on_error=True):
         it = C.c_int(0)
         res = C.c_double(0)
-        code = self.grid.ctx.lib.kf_poisson_solve(self.h, phi.ptr, rho.ptr, C.byref(it), C.byref(res))
+        rs = self.grid.rshape
+        code = self.grid.ctx.lib.kf_poisson_solve(self.h, _need(phi, rs, np.float64, "phi"),
+                                                  _need(rho, rs, np.float64, "rho"), C.byref(it), C.byref(res))
         if raise_on_error:
             check(code)
         return it.value, res.value, code
 
     def residual(self, phi: DeviceArray, rho: DeviceArray):
         res = C.c_double(0)
-        check(self.grid.ctx.lib.kf_poisson_residual(self.h, phi.ptr, rho.ptr, C.byref(res)))
+        rs = self.grid.rshape
+        check(self.grid.ctx.lib.kf_poisson_residual(self.h, _need(phi, rs, np.float64, "phi"),
+                                                    _need(rho, rs, np.float64, "rho"), C.byref(res)))
         return res.value
 
 
@@ -388,9 +442,15 @@ class Rhs:
             self.h = None
 
     def set_fields(self, phi=None, nd=None, mask=None):
-        check(self.grid.ctx.lib.kf_rhs_set_fields(
-            self.h, None if phi is None else phi.ptr, None if nd is None else nd.ptr,
-            None if mask is None else mask.ptr))
+        """n_d and the mask are copied into the library; phi is kept by
+        pointer (it is the warm start and the result of every later Poisson
+        solve), so this object keeps that field alive while it may be used."""
+        rs = self.grid.rshape
+        ptrs = [None if a is None else _need(a, rs, np.float64, w)
+                for a, w in ((phi, "phi"), (nd, "n_d"), (mask, "mask"))]
+        check(self.grid.ctx.lib.kf_rhs_set_fields(self.h, *ptrs))
+        if phi is not None:
+            self._phi = phi
 
     def set_parameters(self, M=None, tau=None, theta=None):
         if M is not None:
@@ -402,7 +462,7 @@ class Rhs:
         check(self.grid.ctx.lib.kf_rhs_set_parameters(self.h, C.byref(self.params)))
 
     def solve(self, t, F):
-        check(self.grid.ctx.lib.kf_rhs_solve(self.h, t, *[f.ptr for f in F]))
+        check(self.grid.ctx.lib.kf_rhs_solve(self.h, t, *_need4(F, self.grid, "rhs state")))
 
     def force_general(self, on=True):
         check(self.grid.ctx.lib.kf_rhs_force_general(self.h, int(on)))
@@ -426,7 +486,7 @@ class Rk4:
             self.h = None
 
     def step(self, F, n=1):
-        check(self.rhs.grid.ctx.lib.kf_rk4_steps(self.h, n, *[f.ptr for f in F]))
+        check(self.rhs.grid.ctx.lib.kf_rk4_steps(self.h, n, *_need4(F, self.rhs.grid, "rk4 state")))
 
     def set_dt(self, dt):
         check(self.rhs.grid.ctx.lib.kf_rk4_set_dt(self.h, dt))
@@ -436,10 +496,6 @@ class Rk4:
         v = C.c_double(0)
         check(self.rhs.grid.ctx.lib.kf_rk4_time(self.h, C.byref(v)))
         return v.value
-
-
-def _ptrs4(fields):
-    return (_vp * 4)(*[f.ptr for f in fields])
 
 
 class Adjoint:
@@ -464,29 +520,38 @@ class Adjoint:
     def reset(self):
         check(self.lib.kf_adjoint_reset(self.h))
 
+    def _p4(self, F, what):
+        return (_vp * 4)(*_need4(F, self.rhs.grid, what))
+
+    def _r(self, X, what):
+        return _need(X, self.rhs.grid.rshape, np.float64, what)
+
     def rhs_adjoint(self, S, ob, Fb):
-        check(self.lib.kf_rhs_adjoint(self.h, _ptrs4(S), _ptrs4(ob), _ptrs4(Fb)))
+        check(self.lib.kf_rhs_adjoint(self.h, self._p4(S, "S"), self._p4(ob, "ob"), self._p4(Fb, "Fb")))
 
     def step_adjoint(self, y0, yb):
-        check(self.lib.kf_rk4_step_adjoint(self.h, _ptrs4(y0), _ptrs4(yb)))
+        check(self.lib.kf_rk4_step_adjoint(self.h, self._p4(y0, "y0"), self._p4(yb, "yb")))
 
     def objective(self, F, dV):
         J = C.c_double(0)
-        check(self.lib.kf_objective(self.h, _ptrs4(F), dV.ptr, C.byref(J)))
+        check(self.lib.kf_objective(self.h, self._p4(F, "F"), self._r(dV, "dV"), C.byref(J)))
         return J.value
 
     def objective_adjoint(self, F, dV, Fb):
-        check(self.lib.kf_objective_adjoint(self.h, _ptrs4(F), dV.ptr, _ptrs4(Fb)))
+        check(self.lib.kf_objective_adjoint(self.h, self._p4(F, "F"), self._r(dV, "dV"), self._p4(Fb, "Fb")))
 
     def accumulators(self, nd_bar=None):
         m = C.c_double(0)
-        check(self.lib.kf_adjoint_accumulators(self.h, C.byref(m), None if nd_bar is None else nd_bar.ptr))
+        check(self.lib.kf_adjoint_accumulators(self.h, C.byref(m),
+                                               None if nd_bar is None else self._r(nd_bar, "nd_bar")))
         return m.value
 
     def run(self, y, nsteps, k, dV, G, dFn0):
         J, dM, dQ = C.c_double(0), C.c_double(0), C.c_double(0)
-        check(self.lib.kf_adjoint_run(self.h, _ptrs4(y), nsteps, k, dV.ptr, G.ptr, C.byref(J), C.byref(dM),
-                                      C.byref(dQ), dFn0.ptr))
+        cs = self.rhs.grid.cshape
+        check(self.lib.kf_adjoint_run(self.h, self._p4(y, "y"), nsteps, k, self._r(dV, "dV"), self._r(G, "G"),
+                                      C.byref(J), C.byref(dM), C.byref(dQ),
+                                      _need(dFn0, cs, np.complex128, "dFn0")))
         return J.value, dM.value, dQ.value
 
     def set_capture_budget(self, nbytes):
@@ -506,7 +571,8 @@ class Adjoint:
 
     def spectral_to_real(self, dF, out):
         """out = R2C^T dF: real-space cotangent of the field whose spectrum is F."""
-        check(self.lib.kf_adjoint_spectral_to_real(self.h, dF.ptr, out.ptr))
+        check(self.lib.kf_adjoint_spectral_to_real(
+            self.h, _need(dF, self.rhs.grid.cshape, np.complex128, "dF"), self._r(out, "out")))
 
 
 class _View:
@@ -553,7 +619,12 @@ class Batch:
                                                                                     np.float64)
 
     def set_fields(self, b, nd, mask):
-        check(self.ctx.lib.kf_batch_set_fields(self.h, b, nd.ptr, mask.ptr))
+        """n_d and the mask are copied into box b's solver."""
+        if not 0 <= b < len(self.shapes):
+            raise IndexError(f"box {b} of {len(self.shapes)}")
+        rs = self.shapes[b]
+        check(self.ctx.lib.kf_batch_set_fields(self.h, b, _need(nd, rs, np.float64, "n_d"),
+                                               _need(mask, rs, np.float64, "mask")))
 
     def capture(self, poisson_iterations=4):
         check(self.ctx.lib.kf_batch_capture(self.h, poisson_iterations))

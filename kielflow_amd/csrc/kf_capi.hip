@@ -506,6 +506,16 @@ int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse)
     });
 }
 
+int kf_poisson_set_cascade(kf_poisson p, int n, const int *level_change, const int *steps, const double *tol)
+{
+    return guard([&] {
+        KF_REQUIRE(p && level_change && steps && tol, "null argument");
+        KF_REQUIRE(p->p->mode() == POISSON_CASCADE, "the level program belongs to the KF_POISSON_CASCADE mode");
+        p->p->set_cascade(n, level_change, steps, tol);
+        return KF_OK;
+    });
+}
+
 int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid)
 {
     return guard([&] {
@@ -723,6 +733,30 @@ int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const
         KF_REQUIRE(g && FUx && FUy && FUz && dt_new, "null argument");
         const double2 *F[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
         *dt_new = cfl_all(*g->g, *g->p64, dt, F, g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+int kf_rk4_stage_input(kf_ctx ctx, size_t n, double c, const double *y0, const double *k, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx && y0 && k && out, "null argument");
+        set_dev(ctx);
+        rk4_stage_input((long)n, c, y0, k, out, ctx->stream);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_rk4_combine(kf_ctx ctx, size_t n, double dt, const double *y0, const double *k1, const double *k2,
+                   const double *k3, const double *k4, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(ctx && y0 && k1 && k2 && k3 && k4 && out, "null argument");
+        set_dev(ctx);
+        const double *k[4] = {k1, k2, k3, k4};
+        rk4_combine((long)n, dt, y0, k, out, ctx->stream);
+        KF_HIP(hipGetLastError());
         return KF_OK;
     });
 }

@@ -17,7 +17,7 @@ struct Grid {
     double *q[3] = {nullptr, nullptr, nullptr};
     double *qzf = nullptr;
     int *imax = nullptr;  // GridView::imax
-    // dealiased 1/S'(xi) per stretched axis (OP_partial_derivative.cpp:311-331);
+    // dealiased 1/S'(xi) per stretched axis (OP_partial_derivative.cpp:62-80);
     // nullptr on equidistant axes
     double *prof64[3] = {nullptr, nullptr, nullptr};
     float *prof32[3] = {nullptr, nullptr, nullptr};
@@ -25,6 +25,16 @@ struct Grid {
     // x index only (rhs_standard.cpp:39-49)
     double *bar64 = nullptr;
     float *bar32 = nullptr;
+
+    // scratch of the stretched OP_partial_derivative and of cfl_all,
+    // allocated on first use and kept with the grid: three real fields, the
+    // CFL spacing table and the per-block CFL maxima (device), and the
+    // pinned host mirror of those maxima
+    double *ws = nullptr;
+    double *ws_host = nullptr;
+    bool cfl_table = false;
+    unsigned cfl_blocks() const { return (unsigned)((nr + 255) / 256); }
+    double *workspace();
 
     Grid(const AxisDesc axes[3], hipStream_t s);
     ~Grid();

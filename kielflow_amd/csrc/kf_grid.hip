@@ -137,8 +137,20 @@ Grid::Grid(const AxisDesc axes[3], hipStream_t s)
     KF_HIP(hipStreamSynchronize(s));
 }
 
+double *Grid::workspace()
+{
+    if (!ws) {
+        const size_t n = 3 * (size_t)nr + (size_t)(nx + ny + nz) + 3 * (size_t)cfl_blocks();
+        KF_HIP(hipMalloc(&ws, sizeof(double) * n));
+        KF_HIP(hipHostMalloc(&ws_host, sizeof(double) * 3 * cfl_blocks(), hipHostMallocDefault));
+    }
+    return ws;
+}
+
 Grid::~Grid()
 {
+    (void)hipFree(ws);
+    (void)hipHostFree(ws_host);
     for (int d = 0; d < 3; ++d) {
         (void)hipFree(k64[d]);
         (void)hipFree(k32[d]);

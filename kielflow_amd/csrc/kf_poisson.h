@@ -33,7 +33,7 @@ struct AxisDesc {
     double l0, L, m;
 };
 
-// axis functions restated on the host (axis_EqSt.cpp:251-283,
+// axis functions restated on the host (axis_EqSt.cpp:52-84,
 // axis_SiSt.cpp:64-134, axis_HySt.cpp:49-80, axis_FoHySt.cpp:107-117)
 double axis_val_at(const AxisDesc &a, int index);
 double axis_k_val_at(const AxisDesc &a, int index);
@@ -109,7 +109,7 @@ class Poisson {
     PoissonStats last_stats();          // synchronises the stream
     int *device_fail_counter() { return d_fail_; }
 
-    // reference NLJ state (solver_poisson_jacobi_nlin.hpp:140-171)
+    // reference NLJ state (solver_poisson_jacobi_nlin.hpp:66-98)
     struct NljState {
         double norm_max = 0, norm_sum = 0, supremum = 0, infinum = 0;
         double norm_max_old = -1, norm_sum_old = -1;
@@ -118,6 +118,8 @@ class Poisson {
     };
     NljState &nlj() { return nlj_; }
     int nlj_solve_level(int level, double *phi, const double *rho, hipStream_t s);
+    // the cascade's level program (solver_poisson_multigrid::set_level_control)
+    void set_cascade(int n, const int *change, const int *steps, const double *tol);
 
     int levels() const { return (int)lv_.size(); }
     bool ax_desc_uniform() const { return ax_[0].type == 1 && ax_[1].type == 1 && ax_[2].type == 1; }
@@ -156,6 +158,9 @@ class Poisson {
     template <typename R>
     void restrict_e(hipStream_t s, const int *ctl);
     int solve_cascade(double *phi, const double *rho, hipStream_t s);
+    // level program of the reference cascade (kielflow.cpp:207-217 by default)
+    std::vector<int> casc_change_{-3, 1, 1, 1}, casc_steps_{0, -1, -1, -1};
+    std::vector<double> casc_tol_{0.01, 0.0005, 0.0001, 0.0001};
 
     AxisDesc ax_[3];
     bool mixed_;

@@ -11,7 +11,7 @@ namespace kf {
 // ======================================================================== axes
 double axis_val_at(const AxisDesc &a, int index)
 {
-    if (a.type == 1) return a.l0 + index * (a.L / a.n);  // axis_EqSt.cpp:251-254
+    if (a.type == 1) return a.l0 + index * (a.L / a.n);  // axis_EqSt.cpp:52-55
     if (a.type == 3) {                                  // axis_SiSt.cpp:64-89
         int N = a.n, shift = 0;
         if (index < 0) {
@@ -39,7 +39,7 @@ double axis_val_at(const AxisDesc &a, int index)
 
 double axis_k_val_at(const AxisDesc &a, int index)
 {
-    // axis_EqSt.cpp:256-260 (also SiSt, and the computational grid of HySt)
+    // axis_EqSt.cpp:57-61 (also SiSt, and the computational grid of HySt)
     double pi = acos(-1.);
     return ((index <= a.n / 2) ? (2. * pi * index / a.L) : (2. * pi * (index - a.n) / a.L));
 }
@@ -1917,34 +1917,64 @@ int Poisson::nlj_solve_level(int level, double *phi, const double *rho, hipStrea
     return rc < 0 ? rc : S.iteration;
 }
 
-// solver_poisson_multigrid::solve (cpp:246-418) with kielflow.cpp:207-217's
-// cycle {lvl_down3, lvl_up x3}, steps {0,-1,-1,-1}, tol {1e-2,5e-4,1e-4,1e-4}
+// solver_poisson_multigrid::set_level_control (cpp:82-147): the level
+// program of the cascade.  change[c] < 0 goes down by -change[c] levels
+// (injection by 2^-change[c]), +1 goes up one level (OP_hto2h), 0 stays.
+// The level may never rise above the finest grid (cpp:126-145) and must
+// end on it (cpp:406-410).
+void Poisson::set_cascade(int n, const int *change, const int *steps, const double *tol)
+{
+    if (n < 1) throw Error("cascade needs at least one level step");
+    int lvl = 0, depth = 0;
+    for (int c = 0; c < n; ++c) {
+        if (change[c] < -3 || change[c] > 1) throw Error("cascade level steps are -3..1 (down3 .. up)");
+        lvl -= change[c];
+        if (lvl < 0) throw Error("cascade level rises above the base level");
+        depth = std::max(depth, lvl);
+    }
+    if (lvl != 0) throw Error("cascade must end on the base level");
+    for (int d = 0; d < 3; ++d)
+        if ((ax_[d].n % (1 << depth)) != 0 || (ax_[d].n >> depth) < 2)
+            throw Error("cascade depth does not divide the grid extents");
+    casc_change_.assign(change, change + n);
+    casc_steps_.assign(steps, steps + n);
+    casc_tol_.assign(tol, tol + n);
+}
+
+// solver_poisson_multigrid::solve (cpp:246-418) over the level program set
+// by set_cascade (default: kielflow.cpp:207-217's {down3, up, up, up} with
+// steps {0, -1, -1, -1} and tolerances {1e-2, 5e-4, 1e-4, 1e-4})
 int Poisson::solve_cascade(double *phi, const double *rho, hipStream_t s)
 {
     const int nx = ax_[0].n, ny = ax_[1].n, nz = ax_[2].n;
-    if (nx < 16 || ny < 16 || nz < 16 || nx % 8 || ny % 8 || nz % 8)
-        throw Error("reference MG cascade needs extents that are multiples of 8 and >= 16");
+    int depth = 0, lv = 0;
+    for (int c : casc_change_) depth = std::max(depth, lv -= c);
+    if (nx % (1 << depth) || ny % (1 << depth) || nz % (1 << depth) || (std::min(nx, std::min(ny, nz)) >> depth) < 2)
+        throw Error("the cascade's depth does not divide the grid extents");
     const long N = (long)nx * ny * nz;
     double *a = nullptr, *b = nullptr, *rl = nullptr, *g = nullptr;
     KF_HIP(hipMalloc(&a, sizeof(double) * N));
     KF_HIP(hipMalloc(&b, sizeof(double) * N));
     KF_HIP(hipMalloc(&rl, sizeof(double) * N));
     KF_HIP(hipMalloc(&g, sizeof(double) * (nx + ny + nz)));
-    const int steps[4] = {0, -1, -1, -1};
-    const double tol[4] = {0.01, 0.0005, 0.0001, 0.0001};
-    int lvl = 3;
-    {
-        long Nc = N / 512;
-        hipLaunchKernelGGL(k_inject, dim3(nblocks(Nc, 256)), dim3(256), 0, s, (const double *)phi, nx, ny, nz,
-                           8, a);
-    }
+    KF_HIP(hipMemcpyAsync(a, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+    int lvl = 0;
     int total = 0, rc = 0;
-    for (int c = 0; c < 4; ++c) {
-        if (c > 0) {
+    for (size_t c = 0; c < casc_change_.size(); ++c) {
+        const int step = casc_change_[c];
+        const int cx = nx >> lvl, cy = ny >> lvl, cz = nz >> lvl;
+        if (step < 0) {
+            // injection by 2^-step from level lvl
+            const int f = 1 << (-step);
+            const long Nc = (long)(cx / f) * (cy / f) * (cz / f);
+            hipLaunchKernelGGL(k_inject, dim3(nblocks(Nc, 256)), dim3(256), 0, s, (const double *)a, cx, cy, cz, f,
+                               b);
+            std::swap(a, b);
+            lvl += -step;
+        } else if (step > 0) {
             // OP_hto2h from level lvl to lvl-1
-            int cx = nx >> lvl, cy = ny >> lvl, cz = nz >> lvl;
             lvl -= 1;
-            int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
+            const int fx = 2 * cx, fy = 2 * cy, fz = 2 * cz;
             std::vector<double> gam(fx + fy + fz, 0.);
             int ff[3] = {fx, fy, fz};
             size_t off = 0;
@@ -1968,14 +1998,16 @@ int Poisson::solve_cascade(double *phi, const double *rho, hipStream_t s)
                                cz, b, gy);
             hipLaunchKernelGGL(k_hto2h, dim3(nblocks(Nf, 256)), dim3(256), 0, s, 3, (const double *)a, cx, cy,
                                cz, b, gx);
+            // the host gamma table is read by the copy above before it goes out of scope
+            KF_HIP(hipStreamSynchronize(s));
             std::swap(a, b);
         }
-        // rho on this level: injection from the finest grid
+        // rho on this level: injection from the finest grid (cpp:284-359)
         const int f = 1 << lvl;
         long Nl = N >> (3 * lvl);
         hipLaunchKernelGGL(k_inject, dim3(nblocks(Nl, 256)), dim3(256), 0, s, rho, nx, ny, nz, f, rl);
-        nlj_.max_iterations = steps[c];
-        nlj_.limit_max = tol[c];
+        nlj_.max_iterations = casc_steps_[c];
+        nlj_.limit_max = casc_tol_[c];
         int it = nlj_solve_level(-lvl, a, rl, s);
         if (it < 0) { rc = it; break; }
         total += it;

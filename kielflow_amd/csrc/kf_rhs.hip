@@ -587,8 +587,7 @@ void op_partial_derivative(Grid &g, Plan<double> &p, int dir, const double2 *in,
                            0, 0);
         return;
     }
-    double *tmp = nullptr;
-    KF_HIP(hipMalloc(&tmp, sizeof(double) * g.nr));
+    double *tmp = g.workspace();
     ZStore<double> z = zs<double>(ZK_PROF, p);
     z.o0 = tmp;
     z.prof = g.prof<double>(dir);
@@ -597,8 +596,6 @@ void op_partial_derivative(Grid &g, Plan<double> &p, int dir, const double2 *in,
     XStore<double> x = xs<double>(XK_WRITE, g);
     x.o0 = out;
     r2c_store<double, double>(g, p, tmp, x, s);
-    KF_HIP(hipStreamSynchronize(s));
-    (void)hipFree(tmp);
 }
 
 void op_dealias23(Grid &g, double2 *inout, hipStream_t s)
@@ -645,38 +642,65 @@ __global__ void k_cfl(const double *__restrict__ Ux, const double *__restrict__ 
 double cfl_all(Grid &g, Plan<double> &p, double dt, const double2 *F[3], hipStream_t s)
 {
     const long nr = g.nr;
-    double *U = nullptr, *dd = nullptr, *part = nullptr;
-    const unsigned nb = nbk(nr);
-    KF_HIP(hipMalloc(&U, sizeof(double) * 3 * nr));
-    KF_HIP(hipMalloc(&dd, sizeof(double) * (g.nx + g.ny + g.nz)));
-    KF_HIP(hipMalloc(&part, sizeof(double) * 3 * nb));
-    std::vector<double> h(g.nx + g.ny + g.nz);
-    int n3[3] = {g.nx, g.ny, g.nz};
-    size_t off = 0;
-    for (int d = 0; d < 3; ++d) {
-        for (int i = 0; i < n3[d]; ++i) {
-            // get_delta (CFL.cpp:6-12) at i and i+1, smaller of the two
-            double d0 = axis_val_at(g.ax[d], i + 1) - axis_val_at(g.ax[d], i);
-            double d1 = axis_val_at(g.ax[d], i + 2) - axis_val_at(g.ax[d], i + 1);
-            h[off + i] = d0 < d1 ? d0 : d1;
+    double *U = g.workspace();
+    double *dd = U + 3 * nr;
+    double *part = dd + (g.nx + g.ny + g.nz);
+    const unsigned nb = g.cfl_blocks();
+    if (!g.cfl_table) {
+        std::vector<double> h(g.nx + g.ny + g.nz);
+        int n3[3] = {g.nx, g.ny, g.nz};
+        size_t off = 0;
+        for (int d = 0; d < 3; ++d) {
+            for (int i = 0; i < n3[d]; ++i) {
+                // get_delta (CFL.cpp:6-12) at i and i+1, smaller of the two
+                double d0 = axis_val_at(g.ax[d], i + 1) - axis_val_at(g.ax[d], i);
+                double d1 = axis_val_at(g.ax[d], i + 2) - axis_val_at(g.ax[d], i + 1);
+                h[off + i] = d0 < d1 ? d0 : d1;
+            }
+            off += n3[d];
         }
-        off += n3[d];
+        KF_HIP(hipMemcpy(dd, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice));
+        g.cfl_table = true;
     }
-    KF_HIP(hipMemcpyAsync(dd, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, s));
     for (int b = 0; b < 3; ++b) c2r<double>(p, F[b], U + b * nr, s);
     hipLaunchKernelGGL(k_cfl, dim3(nb), dim3(256), 0, s, (const double *)U, (const double *)(U + nr),
                        (const double *)(U + 2 * nr), (const double *)dd, (const double *)(dd + g.nx),
                        (const double *)(dd + g.nx + g.ny), g.nx, g.ny, g.nz, dt, part);
-    std::vector<double> hp(3 * (size_t)nb);
-    KF_HIP(hipMemcpyAsync(hp.data(), part, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, s));
+    KF_HIP(hipMemcpyAsync(g.ws_host, part, sizeof(double) * 3 * nb, hipMemcpyDeviceToHost, s));
     KF_HIP(hipStreamSynchronize(s));
-    (void)hipFree(U);
-    (void)hipFree(dd);
-    (void)hipFree(part);
     double cmax = 0.;
-    for (double v : hp) cmax = v > cmax ? v : cmax;
+    for (unsigned i = 0; i < 3 * nb; ++i) cmax = g.ws_host[i] > cmax ? g.ws_host[i] : cmax;
     if (.4 * dt / cmax >= 0.01) return 0.01;
     return .4 * dt / cmax;
+}
+
+// RKO4.cpp:140-153 (and :167-180, :194-207): out = y0 + c k, n doubles
+__global__ void k_rk_axpy(long n, double c, const double *__restrict__ y0, const double *__restrict__ k,
+                          double *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = y0[i] + c * k[i];
+}
+
+// RKO4.cpp:225-238: out = y0 + (dt/6)(1 k1 + 2 k2 + 2 k3 + 1 k4)
+__global__ void k_rk_combine(long n, double dt6, const double *__restrict__ y0, const double *__restrict__ k1,
+                             const double *__restrict__ k2, const double *__restrict__ k3,
+                             const double *__restrict__ k4, double *__restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = y0[i] + dt6 * (1. * k1[i] + 2. * k2[i] + 2. * k3[i] + 1. * k4[i]);
+}
+
+void rk4_stage_input(long n, double c, const double *y0, const double *k, double *out, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_rk_axpy, dim3(nbk(n)), dim3(256), 0, s, n, c, y0, k, out);
+}
+
+void rk4_combine(long n, double dt, const double *y0, const double *const k[4], double *out, hipStream_t s)
+{
+    if (n > 0)
+        hipLaunchKernelGGL(k_rk_combine, dim3(nbk(n)), dim3(256), 0, s, n, dt / 6., y0, k[0], k[1], k[2], k[3],
+                           out);
 }
 
 template class Rhs<float>;

@@ -4,8 +4,11 @@
 // rhs_standard, Runge_kutta_O4 and the time loop.  Inputs are synthetic
 // (seeded noise) instead of an HDF5 file.
 //
-//   kf_driver N STEPS OUT [eqst|sist|hyst] [newton|nlj|cascade] [M] [Q]
+//   kf_driver N STEPS OUT [eqst|sist|hyst] [newton|nlj|cascade] [M] [Q] [direct|wrapped]
 //
+// "wrapped" hands Runge_kutta_O4 an interface_rhs of the driver's own that
+// forwards to rhs_standard, so the stepper takes its path for right-hand
+// sides that are not the library's (four solve() calls per step).
 // writes FUx, FUy, FUz, Fni (complex, [N][N][N/2+1]) and Phi (real) to OUT
 // as raw little-endian doubles and prints one summary line.
 #include <chrono>
@@ -15,15 +18,30 @@
 
 #include "kielflow_compat.hpp"
 
+// an interface_rhs that is not rhs_standard (RKO4.cpp:105-212 calls it)
+class forwarding_rhs : public interface_rhs {
+  public:
+    explicit forwarding_rhs(interface_rhs &inner) : inner_(inner) {}
+    void solve(const double &t, field_imag &FUx, field_imag &FUy, field_imag &FUz, field_imag &Fni) override
+    {
+        inner_.solve(t, FUx, FUy, FUz, Fni);
+    }
+
+  private:
+    interface_rhs &inner_;
+};
+
 int main(int argc, char **argv)
 {
     if (argc < 4) {
-        std::fprintf(stderr, "usage: %s N STEPS OUT [eqst|sist|hyst] [newton|nlj|cascade] [M] [Q]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s N STEPS OUT [eqst|sist|hyst] [newton|nlj|cascade] [M] [Q] [direct|wrapped]\n",
+                     argv[0]);
         return 2;
     }
     const int N = std::atoi(argv[1]), steps = std::atoi(argv[2]);
     const std::string out = argv[3], kind = argc > 4 ? argv[4] : "eqst", solver = argc > 5 ? argv[5] : "newton";
     const double M = argc > 6 ? std::atof(argv[6]) : 1.0, Q = argc > 7 ? std::atof(argv[7]) : -0.1;
+    const bool wrapped = argc > 8 && std::string(argv[8]) == "wrapped";
     const double L = 16.;
     try {
         axis *ax;
@@ -64,7 +82,8 @@ int main(int argc, char **argv)
         field_real Ph(Omega);
         std::memset(Ph.val, 0, sizeof(double) * Ph.N);
         rhs_standard rhs(Params, *ps, Ph, nd, mask, Omega, false, true);
-        Runge_kutta_O4 time_integrator(rhs, 1e-3);
+        forwarding_rhs fwd(rhs);
+        Runge_kutta_O4 time_integrator(wrapped ? static_cast<interface_rhs &>(fwd) : rhs, 1e-3);
         const auto t0 = std::chrono::steady_clock::now();
         for (int step = 0; step < steps; ++step) time_integrator.solve(FUx, FUy, FUz, Fni);
         const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -624,12 +624,15 @@ class kf_poisson_adaptor : public interface_relaxation_solver {
             kf_compat::check(kf_poisson_create(g_->handle(), KF_FP64, mode_, &p_));
             kf_compat::check(kf_poisson_set_tolerance(p_, tol_));
             kf_compat::check(kf_poisson_set_max_iterations(p_, max_it_));
+            on_create();
         }
         return p_;
     }
 
   protected:
     explicit kf_poisson_adaptor(int mode, double tol, int max_it) : mode_(mode), tol_(tol), max_it_(max_it) {}
+    // settings held until the device solver exists
+    virtual void on_create() {}
     int mode_;
     double tol_;
     int max_it_;
@@ -662,19 +665,43 @@ class solver_poisson_multigrid : public kf_poisson_adaptor {
     {
     }
     void set_method(interface_relaxation_solver &method) { method_ = &method; }
-    // only the cascade of kielflow.cpp:207-217 is implemented on the GPU
+    // the cascade's level program, steps and tolerances
+    // (solver_poisson_multigrid.cpp:82-147); the isotropic level controls
+    // are implemented, the per-axis ones (lvl_down_x ... lvl_up_z) throw
     void set_level_control(const int &N, int *lvl_steps, MG_lvl_control *lvl_C, double *lvl_tol)
     {
-        const MG_lvl_control shape[4] = {lvl_down3, lvl_up, lvl_up, lvl_up};
-        bool ok = (N == 4);
-        for (int i = 0; ok && i < 4; ++i) ok = (lvl_C[i] == shape[i]);
-        (void)lvl_steps;
-        (void)lvl_tol;
-        if (!ok) throw std::runtime_error("solver_poisson_multigrid: only {down3, up, up, up} is implemented");
+        change_.assign(N, 0);
+        steps_.assign(lvl_steps, lvl_steps + N);
+        tol_lv_.assign(lvl_tol, lvl_tol + N);
+        for (int i = 0; i < N; ++i) {
+            switch (lvl_C[i]) {
+            case lvl_keep: change_[i] = 0; break;
+            case lvl_up: change_[i] = 1; break;
+            case lvl_down: change_[i] = -1; break;
+            case lvl_down2: change_[i] = -2; break;
+            case lvl_down3: change_[i] = -3; break;
+            default:
+                throw std::runtime_error("solver_poisson_multigrid: per-axis level controls are not implemented");
+            }
+        }
+        if (p_) apply_levels();
+    }
+
+  protected:
+    void on_create() override
+    {
+        if (!change_.empty()) apply_levels();
     }
 
   private:
+    void apply_levels()
+    {
+        kf_compat::check(kf_poisson_set_cascade(p_, (int)change_.size(), change_.data(), steps_.data(),
+                                                tol_lv_.data()));
+    }
     interface_relaxation_solver *method_;
+    std::vector<int> change_, steps_;
+    std::vector<double> tol_lv_;
 };
 
 // Newton-multigrid with a residual-based stop (the GPU default)
@@ -689,9 +716,13 @@ class solver_poisson_newton_mg : public kf_poisson_adaptor {
 // ------------------------------------------------------------------ RHS and RK4
 class rhs_standard : public interface_rhs {
   public:
+    // the term switches as rhs_standard.hpp:22-30 ships them: _RHS_SVISCOSITY
+    // defined, _RHS_PENALIZATION commented out
+    static constexpr bool default_spectral_viscosity = true;
+    static constexpr bool default_penalization = false;
     rhs_standard(parameters &Params, interface_poisson_solver &Solver, field_real &potential,
-                 field_real &density_dust, field_real &solid_mask, const grid &domain, bool spectral_viscosity = true,
-                 bool penalization = true)
+                 field_real &density_dust, field_real &solid_mask, const grid &domain,
+                 bool spectral_viscosity = default_spectral_viscosity, bool penalization = default_penalization)
         : my_params(Params), Phi(potential), nd(density_dust), my_solids(solid_mask), g_(domain)
     {
         auto *ad = dynamic_cast<kf_poisson_adaptor *>(&Solver);
@@ -731,15 +762,25 @@ class rhs_standard : public interface_rhs {
 
 class Runge_kutta_O4 : public interface_ode_solver {
   public:
+    // With kielflow_amd's rhs_standard the whole step runs in the library
+    // (kf_rk4_*).  Any other interface_rhs is evaluated four times per step
+    // through its own solve(), as RKO4.cpp:105-212 does, and the stage
+    // inputs and the update run on the device (kf_rk4_stage_input,
+    // kf_rk4_combine) with the reference's arithmetic.
     explicit Runge_kutta_O4(interface_rhs &rhs, double dt = 0.01) : dt_(dt), my_rhs(rhs)
     {
-        auto *r = dynamic_cast<rhs_standard *>(&rhs);
-        if (!r) throw std::runtime_error("Runge_kutta_O4: the RHS must be kielflow_amd's rhs_standard");
-        kf_compat::check(kf_rk4_create(r->handle(), dt, &k_));
+        if (auto *r = dynamic_cast<rhs_standard *>(&rhs)) kf_compat::check(kf_rk4_create(r->handle(), dt, &k_));
     }
-    ~Runge_kutta_O4() override { kf_rk4_destroy(k_); }
+    ~Runge_kutta_O4() override
+    {
+        if (k_) kf_rk4_destroy(k_);
+    }
     void solve(field_imag &FUx, field_imag &FUy, field_imag &FUz, field_imag &Fni) override
     {
+        if (!k_) {
+            solve_any(FUx, FUy, FUz, Fni);
+            return;
+        }
         auto &r = dynamic_cast<rhs_standard &>(my_rhs);
         r.sync_parameters();
         kf_compat::check(kf_rk4_solve(k_, (double *)FUx.upload(), (double *)FUy.upload(), (double *)FUz.upload(),
@@ -749,10 +790,52 @@ class Runge_kutta_O4 : public interface_ode_solver {
         FUz.download();
         Fni.download();
         r.potential().download();
+        t_ += dt_;
     }
 
   private:
+    // RKO4.cpp:50-248 around an arbitrary interface_rhs
+    void solve_any(field_imag &FUx, field_imag &FUy, field_imag &FUz, field_imag &Fni)
+    {
+        field_imag *y[4] = {&FUx, &FUy, &FUz, &Fni};
+        const size_t n = 2 * (size_t)FUx.N;
+        const kf_ctx ctx = kf_compat::context();
+        // RK0 (the step's start) and the four coefficients, host copies
+        // (reserved: a copy of a field does not carry its device mirror)
+        std::vector<field_imag> y0, k[4];
+        y0.reserve(4);
+        for (int f = 0; f < 4; ++f) y0.emplace_back(*y[f]);
+        for (int f = 0; f < 4; ++f) y0[f].upload();
+        for (int s = 0; s < 4; ++s) {
+            k[s].reserve(4);
+            for (int f = 0; f < 4; ++f) k[s].emplace_back(*y[f]);
+        }
+        const double c[3] = {0.5 * dt_, 0.5 * dt_, 1.0 * dt_};
+        const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
+        for (int s = 0; s < 4; ++s) {
+            if (s > 0) {
+                // stage input RK_{s+1} = RK0 + c_s RK_s (the coefficient just evaluated)
+                for (int f = 0; f < 4; ++f) {
+                    kf_compat::check(kf_rk4_stage_input(ctx, n, c[s - 1], (const double *)y0[f].dev(),
+                                                        (const double *)k[s - 1][f].upload(),
+                                                        (double *)k[s][f].dev()));
+                    k[s][f].download();
+                }
+            }
+            my_rhs.solve(tt[s], k[s][0], k[s][1], k[s][2], k[s][3]);
+        }
+        for (int f = 0; f < 4; ++f) {
+            kf_compat::check(kf_rk4_combine(ctx, n, dt_, (const double *)y0[f].dev(),
+                                            (const double *)k[0][f].upload(), (const double *)k[1][f].upload(),
+                                            (const double *)k[2][f].upload(), (const double *)k[3][f].upload(),
+                                            (double *)y[f]->dev()));
+            y[f]->download();
+        }
+        t_ += dt_;
+    }
+
     double dt_;
+    double t_ = 0.;
     interface_rhs &my_rhs;
     kf_rk4 k_ = nullptr;
 };

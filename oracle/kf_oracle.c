@@ -35,7 +35,7 @@ static double sist_S(const or_axis *a, double x)
 double or_axis_val_at(const or_axis *a, int index)
 {
     if (a->type == OR_EQST) {
-        /* axis_EqSt.cpp:251-254 */
+        /* axis_EqSt.cpp:52-55 */
         return a->l0 + index * (a->L / a->N);
     }
     if (a->type == OR_SIST) {
@@ -65,7 +65,7 @@ double or_axis_val_at(const or_axis *a, int index)
     }
 }
 
-/* axis_EqSt.cpp:256-260, axis_SiSt.cpp:92-96; HySt uses the same
+/* axis_EqSt.cpp:57-61, axis_SiSt.cpp:92-96; HySt uses the same
  * computational-grid wavenumber (its own k_val_at returns a coordinate,
  * axis_HySt.cpp:82-113, which the derivative cannot use). */
 double or_axis_k_val_at(const or_axis *a, int index)
@@ -76,7 +76,7 @@ double or_axis_k_val_at(const or_axis *a, int index)
     return ((index <= N / 2) ? (2. * pi * index / L) : (2. * pi * (index - N) / L));
 }
 
-/* axis_EqSt.cpp:280-283, axis_SiSt.cpp:129-134, out_sourced/axis_FoHySt.cpp:107-117 */
+/* axis_EqSt.cpp:81-84, axis_SiSt.cpp:129-134, out_sourced/axis_FoHySt.cpp:107-117 */
 double or_axis_dS(const or_axis *a, int index)
 {
     if (a->type == OR_EQST) return 1.;
@@ -209,7 +209,7 @@ struct or_grid_s {
     fftplan *pl[3];
     cplx *full;     /* nx*ny*nz complex work */
     cplx *line, *line2;
-    double *dSval[3];  /* dealiased 1/S' (OP_partial_derivative.cpp:311-331), lazily */
+    double *dSval[3];  /* dealiased 1/S' (OP_partial_derivative.cpp:62-80), lazily */
     double *d23;       /* dealias table */
     double *ws_c;      /* complex scratch nc */
     double *ws_r;      /* real scratch nr */
@@ -451,7 +451,7 @@ void or_barrier_table(const or_grid *g, double *out)
             }
 }
 
-/* OP_partial_derivative ctor (OP_partial_derivative.cpp:311-331): dealiased
+/* OP_partial_derivative ctor (OP_partial_derivative.cpp:62-80): dealiased
  * 1/S' sampled on the full grid */
 static const double *dS_val(or_grid *g, int dir)
 {

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-/* axis type ids as in axis_EqSt.cpp:214, axis_SiSt.cpp:29, IO.cpp:153-164 */
+/* axis type ids as in axis_EqSt.cpp:15, axis_SiSt.cpp:29, IO.cpp:153-164 */
 enum { OR_EQST = 1, OR_HYST = 2, OR_SIST = 3 };
 
 typedef struct {
@@ -66,7 +66,7 @@ void or_dealias23_table(const or_grid *g, double *out_nc);
 
 /* ---- Poisson ---------------------------------------------------------- */
 typedef struct {
-    /* reference NLJ state (solver_poisson_jacobi_nlin.hpp:140-171) */
+    /* reference NLJ state (solver_poisson_jacobi_nlin.hpp:66-98) */
     double norm_max, norm_sum, supremum, infinum;
     double norm_max_old, norm_sum_old;
     double limit_max;

@@ -47,3 +47,57 @@ def test_driver_matches_c_abi_path(ctx, tmp_path, kind, solver):
     assert rel_l2(phi_d, phi.get()) <= 1e-12
     for o in (rk, fft, rhs, ps, g):
         o.close()
+
+
+def test_driver_stepper_over_a_foreign_rhs(ctx, tmp_path):
+    """Runge_kutta_O4 around an interface_rhs that is not the library's
+    calls its solve() four times per step (RKO4.cpp:105-212).  The same
+    sequence through the C ABI's RHS entry and the RK4 stage arithmetic
+    gives the same state."""
+    import kielflow_amd as kf
+    n, steps, M, Q, dt = 16, 2, 1.0, -0.1, 1e-3
+    print("driver wrapped", flush=True)
+    out = tmp_path / "state.bin"
+    r = subprocess.run([DRIVER, str(n), str(steps), str(out), "eqst", "newton", str(M), str(Q), "wrapped"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    nc = n * n * (n // 2 + 1)
+    got = [raw[2 * a * nc:2 * (a + 1) * nc].view(np.complex128).reshape(n, n, n // 2 + 1) for a in range(4)]
+    axes, _ = axes_for("eqst", n)
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=Q, a=grain_radius("eqst", n))
+    g = kf.Grid(ctx, axes)
+    ps = kf.Poisson(g, kf.KF_FP64, tol=1e-10, max_it=100)
+    rhs = kf.Rhs(g, ps, M=M, tau=0.1, theta=100.0, sv_on=False, pen_on=True)
+    phi = g.rfield()
+    rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+    fft = kf.FFT(g, kf.KF_FP64)
+    F3 = g.cfield()
+    fft.r2c(ctx.to_device(ln_n), F3)
+    y = [np.zeros(g.cshape, complex) for _ in range(3)] + [F3.get()]
+    lib = ctx.lib
+    t = 0.0
+    for _ in range(steps):
+        d_y0 = [ctx.to_device(v) for v in y]
+        ks = []
+        prev = None
+        for s, c in enumerate((None, 0.5 * dt, 0.5 * dt, 1.0 * dt)):
+            d_k = [g.cfield() for _ in range(4)]
+            for f in range(4):
+                if prev is None:
+                    d_k[f].copy_from(d_y0[f])
+                else:
+                    kf.api.check(lib.kf_rk4_stage_input(ctx.h, 2 * nc, c, d_y0[f].ptr, prev[f].ptr, d_k[f].ptr))
+            rhs.solve(t + (0.0 if s == 0 else (0.5 * dt if s < 3 else dt)), d_k)
+            ks.append(d_k)
+            prev = d_k
+        d_out = [g.cfield() for _ in range(4)]
+        for f in range(4):
+            kf.api.check(lib.kf_rk4_combine(ctx.h, 2 * nc, dt, d_y0[f].ptr, ks[0][f].ptr, ks[1][f].ptr,
+                                            ks[2][f].ptr, ks[3][f].ptr, d_out[f].ptr))
+        y = [d.get() for d in d_out]
+        t += dt
+    for a in range(4):
+        assert rel_l2(got[a], y[a]) <= 1e-13, (a, rel_l2(got[a], y[a]))
+    for o in (fft, rhs, ps, g):
+        o.close()

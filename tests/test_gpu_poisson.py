@@ -106,3 +106,33 @@ def test_reference_cascade_matches_oracle(ctx, oracle):
     dphi = g.rfield()
     it, res, code = p.solve(dphi, ctx.to_device(rho))
     assert np.abs(dphi.get() - ref).max() < 1e-11
+
+
+@pytest.mark.parametrize("program", [((-2, 1, 1), (6, -1, -1), (1e-3, 1e-4, 1e-7)),
+                                     ((-1, 0, -1, 1, 1), (4, 3, 8, -1, -1), (1e-2, 1e-2, 1e-3, 1e-4, 1e-8))])
+def test_reference_cascade_level_program_matches_oracle(ctx, oracle, program):
+    """solver_poisson_multigrid::set_level_control (cpp:82-147): a caller's
+    level steps, iteration counts and tolerances are the ones that run."""
+    import kielflow_amd as kf
+    from helpers import axes_for
+    lvl, steps, tol = program
+    print("cascade program", lvl, flush=True)
+    axes, oaxes = axes_for("sist", 32)
+    og = oracle.Grid(*oaxes)
+    rho = 1.0 + og.gauss(-0.1, 1.0)
+    ref, it_o, st = og.mg_cascade(np.zeros(og.rshape), rho, lvl=lvl, steps=steps, tol=tol)
+    g = kf.Grid(ctx, axes)
+    p = kf.Poisson(g, kf.KF_FP64, mode=kf.POISSON_CASCADE)
+    p.set_cascade(lvl, steps, tol)
+    dphi = g.rfield()
+    it, res, code = p.solve(dphi, ctx.to_device(rho))
+    assert np.abs(dphi.get() - ref).max() < 1e-11
+    # the default program differs: the result is not the default cascade's
+    ref_default, _, _ = og.mg_cascade(np.zeros(og.rshape), rho)
+    assert np.abs(ref_default - ref).max() > 1e-9
+    with pytest.raises(kf.api.KfError):
+        p.set_cascade((1, -1), (1, 1), (1e-3, 1e-3))      # above the base level
+    with pytest.raises(kf.api.KfError):
+        p.set_cascade((-1,), (1,), (1e-3,))               # does not end on the base level
+    p.close()
+    g.close()
