@@ -35,6 +35,9 @@ def parse():
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison run")
+    p.add_argument("--fp64-only", action="store_true",
+                   help="run only the FP64 leg of the workload and report its rate as value (for profiling "
+                        "the FP64 kernels; implies --no-cpu --no-adjoint --no-batch)")
     p.add_argument("--cpu-n", type=int, default=64, help="box size of the CPU sample")
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     p.add_argument("--adjoint-steps", type=int, default=200)
@@ -392,6 +395,13 @@ def main():
     nd_d, mask_d = ctx.to_device(nd), ctx.to_device(mask)
     del d_ln
 
+    if args.fp64_only:
+        r = run_precision(kf, ctx, g, axes, "fp64", nd_d, mask_d, Fn0, args.steps, args.warmup)
+        print(json.dumps({"metric": "RK4 steps/s at 256^3, FP64 leg only", "value": round(r["steps_per_s"], 3),
+                          "unit": "steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                          "ms_per_step": round(r["ms_per_step"], 3), "dtype": "f64",
+                          "poisson_failures": r["poisson_failures"]}))
+        return
     fams = (kf.api.PROF_FFT_ZC2R, kf.api.PROF_FFT_Y, kf.api.PROF_RB0)
     mixed = run_precision(kf, ctx, g, axes, "mixed", nd_d, mask_d, Fn0, args.steps, args.warmup, fams)
     fp64 = None

@@ -105,7 +105,17 @@ class Poisson {
     // Enqueues the solve on stream s.  When sync is true the call waits and
     // fills stats (and returns nonzero if the solve did not converge);
     // otherwise stats are available from last_stats() after the stream ends.
-    int solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync);
+    // With from_guess the initial guess is read from guess_buffer() (which
+    // the solve then uses as scratch) and the result is written to phi.
+    int solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync,
+              bool from_guess = false);
+    double *guess_buffer() { return alt64_; }
+    // solve() in two halves, so that work that does not read phi can be
+    // enqueued between them (Newton-MG; other modes solve in solve_begin):
+    // solve_begin enqueues the corrections the last solve needed, solve_end
+    // waits for convergence, enqueuing more corrections as needed.
+    void solve_begin(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool from_guess);
+    void solve_end();
     PoissonStats last_stats();          // synchronises the stream
     int *device_fail_counter() { return d_fail_; }
 
@@ -145,7 +155,22 @@ class Poisson {
     template <typename R>
     void zero_level(int l, hipStream_t s);
     template <typename R>
-    void newton(double *phi, const double *rho64, const float *rho32, hipStream_t s);
+    void newton_begin(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool from_guess);
+    template <typename R>
+    void newton_end();
+    template <typename R>
+    void newton_check(int it);
+    template <typename R>
+    void newton_correct();
+    // the Newton solve between newton_begin and newton_end
+    struct NewtonRun {
+        double *phi = nullptr, *A = nullptr, *B = nullptr;
+        const double *rho64 = nullptr;
+        const float *rho32 = nullptr;
+        hipStream_t s = nullptr;
+        bool from_guess = false, graph = false, active = false;
+        int it = 0, max_it = 0;
+    } run_;
     // BiCGStab on J x = b (level-0 b holds the right side on entry), right
     // preconditioned by one V-cycle; result in level-0 x
     template <typename R>
@@ -186,6 +211,7 @@ class Poisson {
     int *h_ctl_ = nullptr;
     double *corr64_ = nullptr;    // FP64 scratch (mixed defect correction)
     double *alt64_ = nullptr;     // second potential buffer of the fused Newton update
+    int expect_ = 1;              // corrections the last solve needed (host sync point of the next)
     bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
     int kiters_ = 2;
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)

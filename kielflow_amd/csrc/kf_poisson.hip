@@ -487,10 +487,13 @@ k_newton_z4(LevelView<double> L, const double *__restrict__ phi_in, const R *__r
     }
 }
 
-// the newest iterate back into the caller's buffer when it sits in the other one
-__global__ void k_phi_final(double *__restrict__ phi, const double *__restrict__ alt, long n, const int *ctl)
+// the newest iterate into the result buffer when it sits in the other one
+// (ctl[3] names the buffer of the newest iterate; `other` is the value of
+// ctl[3] that means the other buffer)
+__global__ void k_phi_final(double *__restrict__ phi, const double *__restrict__ alt, long n, const int *ctl,
+                            int other)
 {
-    if (ctl[3] != 1) return;
+    if (ctl[3] != other) return;
     const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (p < n) phi[p] = alt[p];
 }
@@ -1429,6 +1432,10 @@ Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed
         for (auto &v : kv_) KF_HIP(hipMalloc(&v, es * lv_[0].N));
     }
     KF_HIP(hipHostMalloc(&h_ctl_, sizeof(int) * 8, hipHostMallocDefault));
+    // the second potential buffer of the Newton update (also where a caller
+    // puts an initial guess, guess_buffer()); allocated here so that nothing
+    // allocates inside a solve
+    KF_HIP(hipMalloc(&alt64_, sizeof(double) * lv_[0].N));
     KF_HIP(hipDeviceSynchronize());
 }
 
@@ -1657,11 +1664,14 @@ void Poisson::vcycle(int l, hipStream_t s, bool zero)
 }
 
 // Newton iterations with FP64 residual/update and R-precision linear MG.
-// The host stays one iteration ahead: while the GPU corrects iteration k,
-// the host waits for the convergence flag of iteration k's check.
+// newton_check(it): the FP64 residual at iterate it (iterate it >= 1 is
+// iterate it-1 plus the correction of iteration it-1), the right side and
+// coefficient of the next correction, and the device convergence test;
+// the flag is copied to the host behind an event.
 template <typename R>
-void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipStream_t s)
+void Poisson::newton_check(int it)
 {
+    NewtonRun &r = run_;
     Level &L0 = lv_[0];
     const long N = L0.N;
     const bool z4 = (L0.n[2] % 4) == 0;
@@ -1670,72 +1680,123 @@ void Poisson::newton(double *phi, const double *rho64, const float *rho32, hipSt
     R *x0, *b0, *e0;
     if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
     else { x0 = L0.x32; b0 = L0.b32; e0 = L0.e32; }
+    hipStream_t s = r.s;
+    const double *rho64 = r.rho64;
+    const float *rho32 = r.rho32;
     const int *ctl = d_ctl_;
-    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
-    const bool graph = graph_it_ > 0;
-    const int max_it = graph ? graph_it_ : max_it_;
-    if (!alt64_) KF_HIP(hipMalloc(&alt64_, sizeof(double) * N));
-    // iteration it >= 1 applies the correction of iteration it-1: odd
-    // iterations write the alternate buffer, even ones the caller's
-    auto check = [&](int it) {
-        if (z4) {
-            const double *src = (it == 0 || (it & 1)) ? phi : alt64_;
-            double *dst = (it & 1) ? alt64_ : phi;
-            const R *xc = it == 0 ? nullptr : (const R *)x0;
-            if (split_level<R>(0)) {
-                if (rho64)
-                    hipLaunchKernelGGL((k_newton_z4<double, R, true>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
-                                       xc, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
-                else
-                    hipLaunchKernelGGL((k_newton_z4<float, R, true>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
-                                       xc, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
-            } else if (rho64) {
-                hipLaunchKernelGGL((k_newton_z4<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
-                                   it & 1, rho64, b0, e0, d_red_, d_ctl_);
-            } else {
-                hipLaunchKernelGGL((k_newton_z4<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
-                                   it & 1, rho32, b0, e0, d_red_, d_ctl_);
-            }
-        } else if (it == 0) {
+    // odd iterations write buffer B, even ones buffer A; the guess is in A
+    double *A = r.A, *B = r.B;
+    if (z4) {
+        const double *src = (it == 0 || (it & 1)) ? A : B;
+        double *dst = (it & 1) ? B : A;
+        const R *xc = it == 0 ? nullptr : (const R *)x0;
+        if (split_level<R>(0)) {
             if (rho64)
-                hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
-                                   (const double *)phi, rho64, b0, e0, d_red_, ctl);
+                hipLaunchKernelGGL((k_newton_z4<double, R, true>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc,
+                                   dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
             else
-                hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64,
-                                   (const double *)phi, rho32, b0, e0, d_red_, ctl);
+                hipLaunchKernelGGL((k_newton_z4<float, R, true>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc,
+                                   dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
+        } else if (rho64) {
+            hipLaunchKernelGGL((k_newton_z4<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
+                               it & 1, rho64, b0, e0, d_red_, d_ctl_);
         } else {
-            const double *src = (it & 1) ? phi : alt64_;
-            double *dst = (it & 1) ? alt64_ : phi;
-            if (rho64)
-                hipLaunchKernelGGL((k_newton_upd_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
-                                   (const R *)x0, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
-            else
-                hipLaunchKernelGGL((k_newton_upd_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
-                                   (const R *)x0, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
+            hipLaunchKernelGGL((k_newton_z4<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src, xc, dst,
+                               it & 1, rho32, b0, e0, d_red_, d_ctl_);
         }
-        hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr,
-                           it, max_it, tol_, d_ctl_, d_fail_, d_stat_);
-        if (graph) return;
-        KF_HIP(hipMemcpyAsync(h_ctl_ + 4 * (it & 1), d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
-        KF_HIP(hipEventRecord(ev_[it & 1], s));
-    };
-    auto correct = [&]() {
-        restrict_e<R>(s, ctl);
-        if (krylov_) {
-            bicgstab<R>(kiters_, s);
-        } else {
-            for (int c = 0; c < inner_; ++c) vcycle<R>(0, s, c == 0);
-        }
-    };
-    check(0);
-    for (int it = 0; it < max_it; ++it) {
-        correct();
-        check(it + 1);
-        if (graph) continue;
-        KF_HIP(hipEventSynchronize(ev_[it & 1]));
-        if (h_ctl_[4 * (it & 1)]) break;
+    } else if (it == 0) {
+        if (rho64)
+            hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, (const double *)A,
+                               rho64, b0, e0, d_red_, ctl);
+        else
+            hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, (const double *)A,
+                               rho32, b0, e0, d_red_, ctl);
+    } else {
+        const double *src = (it & 1) ? A : B;
+        double *dst = (it & 1) ? B : A;
+        if (rho64)
+            hipLaunchKernelGGL((k_newton_upd_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
+                               (const R *)x0, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
+        else
+            hipLaunchKernelGGL((k_newton_upd_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
+                               (const R *)x0, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
     }
-    hipLaunchKernelGGL(k_phi_final, dim3(nblocks(N, 256)), dim3(256), 0, s, phi, (const double *)alt64_, N, ctl);
+    hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr, it,
+                       r.max_it, tol_, d_ctl_, d_fail_, d_stat_);
+    if (r.graph) return;
+    KF_HIP(hipMemcpyAsync(h_ctl_ + 4 * (it & 1), d_ctl_, sizeof(int) * 4, hipMemcpyDeviceToHost, s));
+    KF_HIP(hipEventRecord(ev_[it & 1], s));
+}
+
+// the correction of one Newton iteration: J x = b by one linear V-cycle
+// (or BiCGStab preconditioned by V-cycles on stretched grids)
+template <typename R>
+void Poisson::newton_correct()
+{
+    hipStream_t s = run_.s;
+    restrict_e<R>(s, d_ctl_);
+    if (krylov_) {
+        bicgstab<R>(kiters_, s);
+    } else {
+        for (int c = 0; c < inner_; ++c) vcycle<R>(0, s, c == 0);
+    }
+}
+
+// Enqueues the first check and the corrections up to the count the last
+// solve needed (all of them in graph mode).  From a guess, the guess is in
+// guess_buffer() (A) and the result goes to phi (B), so that a solve that
+// stops after one correction, the usual case with an extrapolated guess,
+// leaves the result where it belongs without a copy; in place, A is phi.
+template <typename R>
+void Poisson::newton_begin(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool from_guess)
+{
+    NewtonRun &r = run_;
+    r.phi = phi;
+    r.rho64 = rho64;
+    r.rho32 = rho32;
+    r.s = s;
+    r.from_guess = from_guess;
+    r.A = from_guess ? alt64_ : phi;
+    r.B = from_guess ? phi : alt64_;
+    r.graph = graph_it_ > 0;
+    r.max_it = r.graph ? graph_it_ : max_it_;
+    r.it = 0;
+    r.active = true;
+    hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
+    newton_check<R>(0);
+    const int upto = r.graph ? r.max_it : std::min(std::max(1, expect_), r.max_it);
+    while (r.it < upto) {
+        newton_correct<R>();
+        newton_check<R>(++r.it);
+    }
+    if (r.graph) newton_end<R>();
+}
+
+// From the last enqueued check on, the host waits for each check before
+// enqueuing another correction.  (A correction enqueued after the converged
+// check still launches every kernel of a V-cycle, and on the finest levels
+// an early-exiting launch costs nearly as much as a working one.)
+template <typename R>
+void Poisson::newton_end()
+{
+    NewtonRun &r = run_;
+    if (!r.active) return;
+    r.active = false;
+    const long N = lv_[0].N;
+    if (!r.graph) {
+        while (true) {
+            KF_HIP(hipEventSynchronize(ev_[r.it & 1]));
+            if (h_ctl_[4 * (r.it & 1)] || r.it >= r.max_it) break;
+            newton_correct<R>();
+            newton_check<R>(++r.it);
+        }
+        // ctl[1]: the iteration of the check that ended the solve
+        expect_ = std::max(1, h_ctl_[4 * (r.it & 1) + 1]);
+    }
+    // result in phi: in place (A = phi) copy when the newest iterate is in B
+    // (ctl[3] == 1); from a guess (B = phi) copy when it is in A (ctl[3] == 0)
+    hipLaunchKernelGGL(k_phi_final, dim3(nblocks(N, 256)), dim3(256), 0, r.s, r.phi, (const double *)alt64_, N,
+                       (const int *)d_ctl_, r.from_guess ? 0 : 1);
 }
 
 template <typename R>
@@ -1800,17 +1861,35 @@ void Poisson::bicgstab(int iters, hipStream_t s)
     hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, x0, (const R *)X, N, ctl);
 }
 
-int Poisson::solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync)
+void Poisson::solve_begin(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool from_guess)
+{
+    if (mode_ != POISSON_NEWTON_MG) {
+        solve(phi, rho64, rho32, s, false, from_guess);
+        return;
+    }
+    if (mixed_) newton_begin<float>(phi, rho64, rho32, s, from_guess);
+    else newton_begin<double>(phi, rho64, rho32, s, from_guess);
+}
+
+void Poisson::solve_end()
+{
+    if (mixed_) newton_end<float>();
+    else newton_end<double>();
+}
+
+int Poisson::solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync, bool from_guess)
 {
     prof::Scope ps_(prof::POISSON, s);
     if (mode_ == POISSON_NEWTON_MG) {
-        if (mixed_) newton<float>(phi, rho64, rho32, s);
-        else newton<double>(phi, rho64, rho32, s);
+        solve_begin(phi, rho64, rho32, s, from_guess);
+        solve_end();
         if (!sync) return 0;
         PoissonStats st = last_stats();
         return st.converged ? 0 : 1;
     }
     if (!rho64) throw Error("reference Poisson modes need an FP64 rho");
+    if (from_guess)
+        KF_HIP(hipMemcpyAsync(phi, alt64_, sizeof(double) * lv_[0].N, hipMemcpyDeviceToDevice, s));
     if (mode_ == POISSON_NLJ) {
         nlj_.max_iterations = max_it_;
         nlj_.limit_max = tol_;
@@ -2143,9 +2222,11 @@ int Poisson::linear_solve_T(double *x, const double *b, double rtol, int max_it,
     return it;
 }
 
-template void Poisson::newton<float>(double *, const double *, const float *, hipStream_t);
+template void Poisson::newton_begin<float>(double *, const double *, const float *, hipStream_t, bool);
+template void Poisson::newton_end<float>();
 template void Poisson::bicgstab<float>(int, hipStream_t);
 template void Poisson::bicgstab<double>(int, hipStream_t);
-template void Poisson::newton<double>(double *, const double *, const float *, hipStream_t);
+template void Poisson::newton_begin<double>(double *, const double *, const float *, hipStream_t, bool);
+template void Poisson::newton_end<double>();
 
 }  // namespace kf

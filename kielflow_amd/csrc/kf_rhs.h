@@ -54,6 +54,8 @@ class Rhs {
     // of the last five converged potentials of stage s (of lower order while
     // fewer exist).
     void set_stage(int s);
+    // the per-stage potential histories (six FP64 fields per stage)
+    void alloc_history();
     // evaluations store their intermediates into *c (general path) while set
     void set_capture(Capture *c) { cap_ = c; }
     // evaluations at RK stage s store their intermediates into c4[s] while set
@@ -86,6 +88,9 @@ class Rhs {
     void eval_general(C2<R> *F[4]);
     // rho from Fn (or from its staged x/y passes), then the Poisson solve
     void poisson_step(const C2<R> *Fn, const C2<R> *staged = nullptr);
+    // poisson_step in two halves (work that does not read phi may go between)
+    void poisson_begin(const C2<R> *Fn, const C2<R> *staged = nullptr);
+    void poisson_end();
     void output(C2<R> *F[4]);
     void deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accumulate);
 
@@ -97,6 +102,8 @@ class Rhs {
     fft::Plan<R> plan_;
     bool force_general_ = false;
     double *phi_ = nullptr;     // caller-owned
+    double *phi_cur_ = nullptr; // the potential of the current evaluation (phi_ or a history slot)
+    double *slot_ = nullptr;    // history slot the current solve writes (or null)
     R *phiR_ = nullptr;         // FP32 copy (mixed) for the forward transform
     R *nd_ = nullptr, *mask_ = nullptr;
     R *U_[3] = {nullptr, nullptr, nullptr};

@@ -232,18 +232,43 @@ constexpr int HIST = 6;
 struct Hist {
     const double *h[HIST];
 };
+// four consecutive points per thread (n is a multiple of 4 on every grid
+// with an even Nz; the tail is handled point by point otherwise), so that
+// each thread has all of its loads in flight at once
+__device__ __forceinline__ double extrap1(const double *const *h, int order, long i)
+{
+    if (order >= 5) return 6. * (h[5][i] + h[1][i]) - 15. * (h[4][i] + h[2][i]) + 20. * h[3][i] - h[0][i];
+    if (order == 4) return 5. * (h[5][i] - h[2][i]) - 10. * (h[4][i] - h[3][i]) + h[1][i];
+    if (order == 3) return 4. * (h[5][i] + h[3][i]) - 6. * h[4][i] - h[2][i];
+    if (order == 2) return 3. * (h[5][i] - h[4][i]) + h[3][i];
+    return 2. * h[5][i] - h[4][i];
+}
 __global__ void k_extrap(double *__restrict__ phi, Hist H, int order, long n)
 {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const long i0 = 4 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
+    if (i0 >= n) return;
     const double *const *h = H.h;
-    double v;
-    if (order >= 5) v = 6. * (h[5][i] + h[1][i]) - 15. * (h[4][i] + h[2][i]) + 20. * h[3][i] - h[0][i];
-    else if (order == 4) v = 5. * (h[5][i] - h[2][i]) - 10. * (h[4][i] - h[3][i]) + h[1][i];
-    else if (order == 3) v = 4. * (h[5][i] + h[3][i]) - 6. * h[4][i] - h[2][i];
-    else if (order == 2) v = 3. * (h[5][i] - h[4][i]) + h[3][i];
-    else v = 2. * h[5][i] - h[4][i];
-    phi[i] = v;
+    if (i0 + 4 > n || (n & 3)) {
+        for (long i = i0; i < n && i < i0 + 4; ++i) phi[i] = extrap1(h, order, i);
+        return;
+    }
+    double4 v;
+    double4 a5 = *reinterpret_cast<const double4 *>(h[5] + i0), a4 = *reinterpret_cast<const double4 *>(h[4] + i0);
+    if (order == 4) {
+        const double4 a3 = *reinterpret_cast<const double4 *>(h[3] + i0);
+        const double4 a2 = *reinterpret_cast<const double4 *>(h[2] + i0);
+        const double4 a1 = *reinterpret_cast<const double4 *>(h[1] + i0);
+        v.x = 5. * (a5.x - a2.x) - 10. * (a4.x - a3.x) + a1.x;
+        v.y = 5. * (a5.y - a2.y) - 10. * (a4.y - a3.y) + a1.y;
+        v.z = 5. * (a5.z - a2.z) - 10. * (a4.z - a3.z) + a1.z;
+        v.w = 5. * (a5.w - a2.w) - 10. * (a4.w - a3.w) + a1.w;
+    } else {
+        v.x = extrap1(h, order, i0);
+        v.y = extrap1(h, order, i0 + 1);
+        v.z = extrap1(h, order, i0 + 2);
+        v.w = extrap1(h, order, i0 + 3);
+    }
+    *reinterpret_cast<double4 *>(phi + i0) = v;
 }
 
 // h[0] <- h[1] <- ... <- h[HIST-1] <- phi
@@ -260,18 +285,31 @@ __global__ void k_shift_hist(HistW H, const double *__restrict__ phi, long n)
 }
 
 template <typename R>
+void Rhs<R>::alloc_history()
+{
+    if (ph_[0][0]) return;
+    for (auto &st : ph_)
+        for (auto &p : st) KF_HIP(hipMalloc(&p, sizeof(double) * g_.nr));
+}
+
+template <typename R>
 void Rhs<R>::set_stage(int s)
 {
     stage_ = s;
-    if (s >= 0 && !ph_[0][0]) {
-        for (auto &st : ph_)
-            for (auto &p : st) KF_HIP(hipMalloc(&p, sizeof(double) * g_.nr));
-    }
+    if (s >= 0) alloc_history();
 }
 
 // rhs_standard.cpp:100-113: rho = exp(iFFT(D23 Fn)) + n_d, then Poisson
 template <typename R>
 void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
+{
+    poisson_begin(Fn, staged);
+    poisson_end();
+}
+
+// rho, the guess, and the solve's first corrections (enqueued)
+template <typename R>
+void Rhs<R>::poisson_begin(const C2<R> *Fn, const C2<R> *staged)
 {
     ZStore<R> z = zs<R>(rho64_on_ ? ZK_RHO64 : ZK_RHO, plan_);
     z.o0 = L_;
@@ -283,30 +321,55 @@ void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
     else c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
     if (!phi_) throw Error("RHS potential field not set");
     const long nr = g_.nr;
+    phi_cur_ = phi_;
+    slot_ = nullptr;
+    const double *rho64 = std::is_same<R, double>::value ? (const double *)rho_ : (rho64_on_ ? rho64_ : nullptr);
+    const float *rho32 = std::is_same<R, double>::value || rho64_on_ ? nullptr : (const float *)rho_;
     if (stage_ >= 0 && hc_[stage_] >= 2) {
         double **h = ph_[stage_];
         const int order = std::min(hc_[stage_] - 1, extrap_order_);
         Hist H{};
         for (int m = 0; m < HIST; ++m) H.h[m] = h[m];
-        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, H, order, nr);
-    }
-    if constexpr (std::is_same<R, double>::value) ps_.solve(phi_, rho_, nullptr, s_, false);
-    else if (rho64_on_) ps_.solve(phi_, rho64_, nullptr, s_, false);
-    else ps_.solve(phi_, nullptr, rho_, s_, false);
-    if (stage_ >= 0) {
-        double **h = ph_[stage_];
-        if (graph_safe_) {
-            HistW H{};
-            for (int m = 0; m < HIST; ++m) H.h[m] = h[m];
-            hipLaunchKernelGGL(k_shift_hist, dim3(nbk(nr)), dim3(256), 0, s_, H, (const double *)phi_, nr);
-        } else {
-            double *oldest = h[0];
-            for (int m = 0; m < HIST - 1; ++m) h[m] = h[m + 1];
-            h[HIST - 1] = oldest;
-            KF_HIP(hipMemcpyAsync(h[HIST - 1], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        if (!graph_safe_) {
+            // Extrapolated guess into the solver's guess buffer; the solve
+            // writes its result straight into the oldest potential of this
+            // stage's history, which then becomes the newest, and the rest of
+            // the evaluation reads it there.  The caller's potential receives
+            // the last stage's solution once per step.
+            hipLaunchKernelGGL(k_extrap, dim3(nbk(nr, 1024)), dim3(256), 0, s_, ps_.guess_buffer(), H, order, nr);
+            slot_ = h[0];
+            ps_.solve_begin(slot_, rho64, rho32, s_, true);
+            return;
         }
-        hc_[stage_]++;
+        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr, 1024)), dim3(256), 0, s_, phi_, H, order, nr);
     }
+    ps_.solve_begin(phi_, rho64, rho32, s_, false);
+}
+
+// convergence of the solve, then the potential history
+template <typename R>
+void Rhs<R>::poisson_end()
+{
+    ps_.solve_end();
+    if (stage_ < 0) return;
+    const long nr = g_.nr;
+    double **h = ph_[stage_];
+    if (slot_) {
+        for (int m = 0; m < HIST - 1; ++m) h[m] = h[m + 1];
+        h[HIST - 1] = slot_;
+        phi_cur_ = slot_;
+        if (stage_ == 3) KF_HIP(hipMemcpyAsync(phi_, slot_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+    } else if (graph_safe_) {
+        HistW H{};
+        for (int m = 0; m < HIST; ++m) H.h[m] = h[m];
+        hipLaunchKernelGGL(k_shift_hist, dim3(nbk(nr)), dim3(256), 0, s_, H, (const double *)phi_, nr);
+    } else {
+        double *oldest = h[0];
+        for (int m = 0; m < HIST - 1; ++m) h[m] = h[m + 1];
+        h[HIST - 1] = oldest;
+        KF_HIP(hipMemcpyAsync(h[HIST - 1], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+    }
+    hc_[stage_]++;
 }
 
 template <typename R>
@@ -345,7 +408,21 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
     // x and y passes of every state field, shared by its value and its
     // three derivatives
     for (int a = 0; a < 4; ++a) stage_xy<R>(g_, plan_, F[a], H_[a][0], H_[a][1], H_[a][2], s_);
-    poisson_step(nullptr, H_[3][0]);
+    // the Poisson solve is enqueued, then the velocities and the products,
+    // which do not read the potential, and only then does the host wait for
+    // the solve's convergence test
+    poisson_begin(nullptr, H_[3][0]);
+    // velocities (rhs_standard.cpp:258-260)
+    for (int b = 0; b < 3; ++b) {
+        ZStore<R> z = zs<R>(ZK_WRITE, plan_);
+        z.o0 = U_[b];
+        z_staged<R>(g_, plan_, H_[b][0], false, z, s_);
+    }
+    // advection / continuity products (rhs_standard.cpp:323-343, 403-437):
+    // adv_a = sum_b U_b dF_a/db, one z pass per field
+    const R *U[3] = {U_[0], U_[1], U_[2]};
+    for (int a = 0; a < 4; ++a) zprod3<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, adv_[a], s_);
+    poisson_end();
     // FFT(Phi) -> filter -> + Fn/theta -> all linear terms (4 outputs)
     XStore<R> x = xs<R>(XK_LIN, g_);
     x.o0 = lin_[0];
@@ -360,17 +437,7 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
     x.M = (R)pa_.M;
     x.theta = (R)pa_.theta;
     x.sv_on = sw_.sv_on;
-    r2c_store<R, double>(g_, plan_, phi_, x, s_);
-    // velocities (rhs_standard.cpp:258-260)
-    for (int b = 0; b < 3; ++b) {
-        ZStore<R> z = zs<R>(ZK_WRITE, plan_);
-        z.o0 = U_[b];
-        z_staged<R>(g_, plan_, H_[b][0], false, z, s_);
-    }
-    // advection / continuity products (rhs_standard.cpp:323-343, 403-437):
-    // adv_a = sum_b U_b dF_a/db, one z pass per field
-    const R *U[3] = {U_[0], U_[1], U_[2]};
-    for (int a = 0; a < 4; ++a) zprod3<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, adv_[a], s_);
+    r2c_store<R, double>(g_, plan_, phi_cur_, x, s_);
     output(F);
 }
 
@@ -406,13 +473,13 @@ void Rhs<R>::eval_general(C2<R> *F[4])
     const long nr = g_.nr;
     if (cap) {
         KF_HIP(hipMemcpyAsync(cap->L, L_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
-        KF_HIP(hipMemcpyAsync(cap->phi, phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        KF_HIP(hipMemcpyAsync(cap->phi, phi_cur_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
     }
     XStore<R> x = xs<R>(XK_F1, g_);
     x.o0 = F1_;
     x.a0 = F[3];
     x.theta = (R)pa_.theta;
-    r2c_store<R, double>(g_, plan_, phi_, x, s_);
+    r2c_store<R, double>(g_, plan_, phi_cur_, x, s_);
     hipLaunchKernelGGL(k_lin_init<R>, dim3(nbk(nc)), dim3(256), 0, s_, g_.view<R>(), (const C2<R> *)F[0],
                        (const C2<R> *)F[1], (const C2<R> *)F[2], (const C2<R> *)F[3], lin_[0], lin_[1],
                        lin_[2], lin_[3], (R)pa_.tau, sw_.sv_on);
@@ -524,6 +591,9 @@ __global__ void k_rk_final(Quad<R> q, long n, double dt6, int next)
 template <typename R>
 Rk4<R>::Rk4(Rhs<R> &rhs, double dt) : rhs_(rhs), dt_(dt)
 {
+    // the per-stage potential histories exist before any step (and so before
+    // any graph capture of one)
+    rhs.alloc_history();
     const long nc = rhs.grid().nc;
     for (int f = 0; f < 4; ++f) {
         S_[f] = ralloc<C2<R>>(nc);
