@@ -163,37 +163,40 @@ __device__ __forceinline__ void pass_io(const C *tw, int tt, const SRC &src, con
 // barriers per pass).  LAST_LDS: dst is the single LDS buffer itself.
 // The first pass uses no twiddles, so a table the caller fills in LDS is
 // read only after the first barrier.  Returns with every LDS write of the
-// chain issued but not synchronised.
-template <typename C, int N, int REM, int NS, int TPT, int DIR, bool PING, bool LAST_LDS, class SRC, class DST,
-          class BUF>
+// chain issued but not synchronised.  FIRST_LDS: src is the single LDS
+// buffer itself (the first pass synchronises between its reads and writes).
+template <typename C, int N, int REM, int NS, int TPT, int DIR, bool PING, bool LAST_LDS, bool FIRST_LDS,
+          class SRC, class DST, class BUF>
 __device__ __forceinline__ void chain_io(const C *tw, int tt, const SRC &src, const DST &dst, const BUF &in,
                                          const BUF &out)
 {
     static_assert(!(PING && LAST_LDS), "an LDS destination needs the single-buffer chain");
+    static_assert(!(PING && FIRST_LDS), "an LDS source needs the single-buffer chain");
     constexpr int R_ = next_radix(REM);
     constexpr bool first = NS == 1, last = REM == R_;
     if constexpr (first && last) {
-        pass_io<C, N, R_, NS, TPT, DIR, false>(tw, tt, src, dst);
+        pass_io<C, N, R_, NS, TPT, DIR, FIRST_LDS>(tw, tt, src, dst);
     } else if constexpr (first) {
-        pass_io<C, N, R_, NS, TPT, DIR, false>(tw, tt, src, out);
+        pass_io<C, N, R_, NS, TPT, DIR, FIRST_LDS>(tw, tt, src, out);
         __syncthreads();
-        chain_io<C, N, REM / R_, NS * R_, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, out, PING ? in : out);
+        chain_io<C, N, REM / R_, NS * R_, TPT, DIR, PING, LAST_LDS, false>(tw, tt, src, dst, out, PING ? in : out);
     } else if constexpr (last) {
         pass_io<C, N, R_, NS, TPT, DIR, LAST_LDS>(tw, tt, in, dst);
     } else {
         pass_io<C, N, R_, NS, TPT, DIR, !PING>(tw, tt, in, out);
         __syncthreads();
-        chain_io<C, N, REM / R_, NS * R_, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, out, PING ? in : out);
+        chain_io<C, N, REM / R_, NS * R_, TPT, DIR, PING, LAST_LDS, false>(tw, tt, src, dst, out, PING ? in : out);
     }
 }
 
 // Transform of length N by TPT threads (this one is tt): see chain_io.
 // A and B are the LDS buffers (the same buffer unless PING).
-template <typename C, int N, int TPT, int DIR, bool PING, bool LAST_LDS, class SRC, class DST, class BUF>
+template <typename C, int N, int TPT, int DIR, bool PING, bool LAST_LDS, bool FIRST_LDS = false, class SRC,
+          class DST, class BUF>
 __device__ __forceinline__ void fft_io(const C *tw, int tt, const SRC &src, const DST &dst, const BUF &A,
                                        const BUF &B)
 {
-    chain_io<C, N, N, 1, TPT, DIR, PING, LAST_LDS>(tw, tt, src, dst, B, A);
+    chain_io<C, N, N, 1, TPT, DIR, PING, LAST_LDS, FIRST_LDS>(tw, tt, src, dst, B, A);
 }
 
 // inputs per thread of the first pass of a length-n transform by tpt threads

@@ -101,6 +101,10 @@ class Rhs {
     hipStream_t s_;
     fft::Plan<R> plan_;
     bool force_general_ = false;
+    // penalization fused into one z pass per field with the products (the
+    // environment variable KF_PEN_UNFUSED selects the separate passes, which
+    // compute the same numbers)
+    bool pen_fused_ = true;
     double *phi_ = nullptr;     // caller-owned
     double *phi_cur_ = nullptr; // the potential of the current evaluation (phi_ or a history slot)
     double *slot_ = nullptr;    // history slot the current solve writes (or null)

@@ -152,6 +152,7 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     plan_init<R>(plan_, g.nx, g.ny, g.nz, s);
     // KF_EXTRAP_ORDER=1..5 caps the order of the potential extrapolation
     if (const char *env = std::getenv("KF_EXTRAP_ORDER")) extrap_order_ = std::max(1, std::min(5, std::atoi(env)));
+    pen_fused_ = std::getenv("KF_PEN_UNFUSED") == nullptr;
     const long nr = g.nr, nc = g.nc;
     nd_ = ralloc<R>(nr);
     mask_ = ralloc<R>(nr);
@@ -419,9 +420,12 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
         z_staged<R>(g_, plan_, H_[b][0], false, z, s_);
     }
     // advection / continuity products (rhs_standard.cpp:323-343, 403-437):
-    // adv_a = sum_b U_b dF_a/db, one z pass per field
+    // adv_a = sum_b U_b dF_a/db, one z pass per field (with penalization,
+    // inside the output's z pass instead)
     const R *U[3] = {U_[0], U_[1], U_[2]};
-    for (int a = 0; a < 4; ++a) zprod3<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, adv_[a], s_);
+    const bool pen_fused = sw_.pen_on && pen_fused_;
+    if (!pen_fused)
+        for (int a = 0; a < 4; ++a) zprod3<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, adv_[a], s_);
     poisson_end();
     // FFT(Phi) -> filter -> + Fn/theta -> all linear terms (4 outputs)
     XStore<R> x = xs<R>(XK_LIN, g_);
@@ -438,7 +442,22 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
     x.theta = (R)pa_.theta;
     x.sv_on = sw_.sv_on;
     r2c_store<R, double>(g_, plan_, phi_cur_, x, s_);
-    output(F);
+    if (!pen_fused) {
+        output(F);
+        return;
+    }
+    // rhs_standard.cpp:403-437 and 459-533, per field: the inverse x and y
+    // passes of the linear terms, one z pass with the products, the blend and
+    // the forward z transform, then the forward y and x passes with -D23
+    for (int a = 0; a < 4; ++a) {
+        c2r_xy<R>(g_, plan_, lin_[a], 0, s_);
+        const R c1 = (a == 0) ? (R)pa_.M : (a == 3 ? (R)10 : (R)0);
+        zpen<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, plan_.scratch, (a < 3) ? U_[a] : L_, mask_, (R)2, c1,
+                a == 0 ? 1 : 0, s_);
+        XStore<R> xo = xs<R>(XK_NEG_D23, g_);
+        xo.o0 = F[a];
+        r2c_yx<R>(g_, plan_, xo, s_);
+    }
 }
 
 // d_b X of OP_partial_derivative.cpp:104-136 into out (write or accumulate)

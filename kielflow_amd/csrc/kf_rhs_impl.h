@@ -556,6 +556,188 @@ inline void zprod3(Grid &g, Plan<R> &p, const C2<R> *Hx, const C2<R> *Hy, const 
     }
 }
 
+// The penalization blend of rhs_standard.cpp:483-490, eps1 H (f + c H) +
+// (1-H) B + b f, applied to the C2R output of the linear terms: the thread's
+// slots hold the products sum_b U_b dF_a/db (the real-space part of B) on
+// entry and the blend on exit.  The last pass of every C2R of one length
+// gives a thread the same output positions, so slot e sits at pos[e] in
+// each of the four transforms.
+template <typename R>
+struct PenBlend {
+    C2<R> *acc;
+    const R *f, *mask;
+    R b;
+    long row;
+    int nz;
+    R scale, c0, c1;
+    int flag;
+    bool live;
+    __device__ __forceinline__ void st(int t, C2<R> v, int slot) const
+    {
+        if (!live) return;
+        using C = C2<R>;
+        const R x0 = v.x * scale, x1 = v.y * scale;
+        const C ad = acc[slot];
+        const C ff = reinterpret_cast<const C *>(f + row * nz)[t];
+        const C H = reinterpret_cast<const C *>(mask + row * nz)[t];
+        const R t0 = flag ? c1 * H.x : c1, t1 = flag ? c1 * H.y : c1;
+        R p0 = c0 * H.x * (ff.x + t0);
+        R p1 = c0 * H.y * (ff.y + t1);
+        p0 += ((R)1 - H.x) * (x0 + ad.x);
+        p1 += ((R)1 - H.y) * (x1 + ad.y);
+        p0 += b * ff.x;
+        p1 += b * ff.y;
+        acc[slot] = mk<C>(p0, p1);
+    }
+};
+
+// One field's products, penalization blend and forward z transform in one
+// z pass (rhs_standard.cpp:403-437 and 459-533).  Per row: the three C2R
+// transforms of the staged Hx, Hy and i k_z H accumulate sum_b U_b dF_a/db
+// in registers (as k_zprod3); the C2R of the linear terms' x/y-transformed
+// spectrum (lin, rows of nzs) is blended with them, with f (U_a or ln n),
+// the mask and the barrier; the blend goes to LDS and its R2C overwrites
+// the row of lin in place.  Same arithmetic as k_zprod3, the ZK_PEN z pass
+// and k_zr2c one after another, without the two real fields in between.
+template <typename R, int M, int RB>
+__global__ void __launch_bounds__(tpt_for(M) * RB)
+k_zpen(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, int nz, int nzs, int ny,
+       const C2<R> *Hx, const C2<R> *Hy, const C2<R> *H, const R *kz, const R *U0, const R *U1, const R *U2,
+       C2<R> *lin, const R *f, const R *mask, const R *bar, R c0, R c1, int flag, R scale)
+{
+    using C = C2<R>;
+    using Buf = RowBuf<C, M>;
+    constexpr int TPT = tpt_for(M);
+    constexpr int WG = TPT * RB;
+    constexpr int E = out_slots(M, TPT);
+    __shared__ C lds[RB * Buf::SC];
+    __shared__ C tw[M];
+    __shared__ C twN[M + 1];
+    const int tid = threadIdx.x;
+    const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
+    for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
+    for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
+    __syncthreads();
+    const int c = tid / TPT, tt = tid % TPT;
+    const long row = r0 + c;
+    const bool live = row < rows;
+    const Buf A{lds, c};
+    C acc[E];
+    int pos[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) pos[e] = -1;
+    const RowLoadC<R> lx{Hx, nzs}, ly{Hy, nzs}, ll{lin, nzs};
+    const RowLoadCK<R> lz{H, nzs, kz};
+    const R b = live ? bar[(unsigned)row / (unsigned)ny] : (R)0;
+    const PenBlend<R> pen{acc, f, mask, b, row, nz, scale, c0, c1, flag, live};
+    if constexpr (merge_in_regs(M)) {
+        C z[M / TPT];
+        merge_regs<C, M>(lx, row, live, tt, twN, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z}, ProdAcc<R>{acc, pos, U0, row, nz, scale, true, live},
+                                            A, A);
+        __syncthreads();
+        merge_regs<C, M>(ly, row, live, tt, twN, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z},
+                                            ProdAcc<R>{acc, pos, U1, row, nz, scale, false, live}, A, A);
+        __syncthreads();
+        merge_regs<C, M>(lz, row, live, tt, twN, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z},
+                                            ProdAcc<R>{acc, pos, U2, row, nz, scale, false, live}, A, A);
+        __syncthreads();
+        merge_regs<C, M>(ll, row, live, tt, twN, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z}, pen, A, A);
+    } else {
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{lx, row, live, twN},
+                                            ProdAcc<R>{acc, pos, U0, row, nz, scale, true, live}, A, A);
+        __syncthreads();
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ly, row, live, twN},
+                                            ProdAcc<R>{acc, pos, U1, row, nz, scale, false, live}, A, A);
+        __syncthreads();
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadCK<R>>{lz, row, live, twN},
+                                            ProdAcc<R>{acc, pos, U2, row, nz, scale, false, live}, A, A);
+        __syncthreads();
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ll, row, live, twN}, pen, A, A);
+    }
+    // the blended row into LDS (pairs 2t, 2t+1 at t), then its forward
+    // Nz/2-point transform in place
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (pos[e] >= 0) A.st(pos[e], live ? acc[e] : mk<C>((R)0, (R)0));
+    __syncthreads();
+    fft_io<C, M, TPT, -1, false, true, true>(tw, tt, A, A, A, A);
+    __syncthreads();
+    // split: X[k] = (Z[k] + conj Z[M-k])/2 - i/2 W^k (Z[k] - conj Z[M-k]), as k_zr2c
+    for (int e = tid; e < RB * (M + 1); e += WG) {
+        const int cc = e / (M + 1), k = e % (M + 1);
+        const long rr = r0 + cc;
+        if (rr < rows) {
+            const Buf Z{lds, cc};
+            C zk = Z.ld(k == M ? 0 : k);
+            C zm = cconj(Z.ld(k == 0 ? 0 : M - k));
+            C ev = mk<C>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y + zm.y));
+            C od = mk<C>((R)0.5 * (zk.x - zm.x), (R)0.5 * (zk.y - zm.y));
+            C wod = cmul(twN[k], od);
+            lin[rr * nzs + k] = mk<C>(ev.x + wod.y, ev.y - wod.x);
+        }
+    }
+}
+
+// k_zpen over all rows: lin (scratch layout, after the inverse x and y
+// passes of the linear terms) is replaced by the forward z transform of the
+// blend
+template <typename R>
+inline void zpen(Grid &g, Plan<R> &p, const C2<R> *Hx, const C2<R> *Hy, const C2<R> *H, const R *const U[3],
+                 C2<R> *lin, const R *f, const R *mask, R c0, R c1, int flag, hipStream_t s)
+{
+    prof::Scope ps_(prof::FFT_ZC2R, s);
+    const long rows = (long)p.nx * p.ny;
+    const R *kz = g.view<R>().k[2];
+    const R scale = (R)(1.0 / (double)p.nr);
+    switch (p.nz / 2) {
+#define KF_CASE(NN)                                                                                            \
+    case NN: {                                                                                                 \
+        constexpr int RB = cb_for<R>(NN) / (sizeof(R) == 4 ? 2 : 1);                                          \
+        hipLaunchKernelGGL((k_zpen<R, NN, RB>), dim3((unsigned)((rows + RB - 1) / RB)), dim3(tpt_for(NN) * RB), \
+                           0, s, p.tw_m, p.tw_n, rows, p.nz, p.nzs, p.ny, Hx, Hy, H, kz, U[0], U[1], U[2], lin, f, \
+                           mask, g.bar<R>(), c0, c1, flag, scale);                                             \
+        break;                                                                                                 \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported z length " + std::to_string(p.nz));
+    }
+}
+
+// the x and y passes of an inverse transform of src (2/3 rule at the load
+// when d23) into the plan's scratch, for a z pass of the caller's own
+template <typename R>
+inline void c2r_xy(Grid &g, Plan<R> &p, const C2<R> *src, int d23, hipStream_t s)
+{
+    GridView<R> gv = g.view<R>();
+    const StridedAddr ay = saddr_y(p);
+    with_xload<R>(xload<R>(src, gv, d23, -1), [&](const auto &ld) {
+        launch_strided<R, +1>(p.nx, scratch_cols_x(p), p.tw_x, pad_cols(ld, p), scratch_store_x(p.scratch, p), s,
+                              prof::FFT_X, p.vx);
+    });
+    launch_strided<R, +1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
+                          StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
+}
+
+// the y and x passes of a forward transform from the plan's scratch (after
+// a z pass of the caller's own) into an XStore
+template <typename R>
+inline void r2c_yx(Grid &g, Plan<R> &p, const XStore<R> &st, hipStream_t s)
+{
+    const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
+    launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
+                          StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
+    with_xstore<R>(st, [&](const auto &x) {
+        launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, x, s, prof::FFT_X, p.vx);
+    });
+}
+
 // z pass of a staged field into a ZStore; with i k_z when dz
 template <typename R>
 inline void z_staged(Grid &g, Plan<R> &p, const C2<R> *H, bool dz, const ZStore<R> &st, hipStream_t s)

@@ -13,10 +13,13 @@ from helpers import initial_state
 pytestmark = pytest.mark.gpu
 
 
-def _eval(ctx, kind, n, prec, general, setting):
+def _eval(ctx, kind, n, prec, general, setting, env="KF_FFT_AUTOTUNE"):
     import kielflow_amd as kf
-    old = os.environ.get("KF_FFT_AUTOTUNE")
-    os.environ["KF_FFT_AUTOTUNE"] = setting
+    old = os.environ.get(env)
+    if setting is None:
+        os.environ.pop(env, None)
+    else:
+        os.environ[env] = setting
     try:
         state, nd, mask, og = initial_state(kind, n, seed=3, Q=-0.2)
         rng = np.random.default_rng(7)
@@ -36,9 +39,9 @@ def _eval(ctx, kind, n, prec, general, setting):
         return [f.get() for f in dF] + [phi.get()]
     finally:
         if old is None:
-            os.environ.pop("KF_FFT_AUTOTUNE", None)
+            os.environ.pop(env, None)
         else:
-            os.environ["KF_FFT_AUTOTUNE"] = old
+            os.environ[env] = old
 
 
 @pytest.mark.parametrize("kind,n,prec,general", [
@@ -52,6 +55,19 @@ def test_autotuner_choices_give_identical_results(ctx, kind, n, prec, general):
     print("fft variants", kind, n, prec, general, flush=True)
     a = _eval(ctx, kind, n, prec, general, "0")
     b = _eval(ctx, kind, n, prec, general, "alt")
+    names = ["FUx", "FUy", "FUz", "Fn", "phi"]
+    for name, x, y in zip(names, a, b):
+        assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
+
+
+@pytest.mark.parametrize("n,prec", [(24, "fp64"), (32, "fp64"), (48, "mixed"), (96, "mixed"), (64, "fp64")])
+def test_fused_penalization_pass_gives_identical_results(ctx, n, prec):
+    """The products, penalization blend and forward z transform of a field in
+    one z pass give the numbers of the separate passes (KF_PEN_UNFUSED) bit
+    for bit, on radix-2 and radix-3 lengths."""
+    print("fused penalization", n, prec, flush=True)
+    a = _eval(ctx, "eqst", n, prec, False, None, env="KF_PEN_UNFUSED")
+    b = _eval(ctx, "eqst", n, prec, False, "1", env="KF_PEN_UNFUSED")
     names = ["FUx", "FUy", "FUz", "Fn", "phi"]
     for name, x, y in zip(names, a, b):
         assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
