@@ -18,6 +18,9 @@ SOURCES = ["kf_fft.hip", "kf_grid.hip", "kf_poisson.hip", "kf_rhs.hip", "kf_adjo
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("KF_OFFLOAD_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+         # a multiply and an add are fused only within one source expression, so
+         # that every instantiation of a pass computes the same numbers
+         "-ffp-contract=on",
          # packed FP32 instructions are disabled for every kernel of this library
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]

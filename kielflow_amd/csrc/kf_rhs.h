@@ -34,6 +34,17 @@ struct Capture {
     double2 *dF0[4];
 };
 
+// The RK4 update of the fields, run by the last store of an RHS evaluation
+// when its path allows (the fused equidistant path with penalization):
+// after stage s < 3, acc = (s == 0 ? k : acc + 2 k) and S = y0 + c k; after
+// the last stage, y = y0 + c (acc + k) and, with next, S = y.
+struct RkEpilogue {
+    double2 *y[4];
+    double2 *acc[4];
+    double c;
+    int final_, first, next;
+};
+
 // RHS in compute precision R (double: FP64 mode; float: mixed mode).
 template <typename R>
 class Rhs {
@@ -56,6 +67,14 @@ class Rhs {
     void set_stage(int s);
     // the per-stage potential histories (six FP64 fields per stage)
     void alloc_history();
+    // the next evaluation runs *e as its last store when it can; done tells
+    // whether it did (else the caller runs the RK4 update itself)
+    void set_rk_epilogue(const RkEpilogue *e)
+    {
+        rk_ = e;
+        rk_done_ = false;
+    }
+    bool rk_epilogue_done() const { return rk_done_; }
     // evaluations store their intermediates into *c (general path) while set
     void set_capture(Capture *c) { cap_ = c; }
     // evaluations at RK stage s store their intermediates into c4[s] while set
@@ -105,6 +124,8 @@ class Rhs {
     // environment variable KF_PEN_UNFUSED selects the separate passes, which
     // compute the same numbers)
     bool pen_fused_ = true;
+    const RkEpilogue *rk_ = nullptr;
+    bool rk_done_ = false;
     double *phi_ = nullptr;     // caller-owned
     double *phi_cur_ = nullptr; // the potential of the current evaluation (phi_ or a history slot)
     double *slot_ = nullptr;    // history slot the current solve writes (or null)

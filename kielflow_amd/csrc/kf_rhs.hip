@@ -454,10 +454,18 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
         const R c1 = (a == 0) ? (R)pa_.M : (a == 3 ? (R)10 : (R)0);
         zpen<R>(g_, plan_, H_[a][1], H_[a][2], H_[a][0], U, plan_.scratch, (a < 3) ? U_[a] : L_, mask_, (R)2, c1,
                 a == 0 ? 1 : 0, s_);
-        XStore<R> xo = xs<R>(XK_NEG_D23, g_);
+        XStore<R> xo = xs<R>(rk_ ? (rk_->final_ ? XK_RK_FINAL : XK_RK_STAGE) : XK_NEG_D23, g_);
         xo.o0 = F[a];
+        if (rk_) {
+            xo.ry = rk_->y[a];
+            xo.racc = rk_->acc[a];
+            xo.rc = rk_->c;
+            xo.rfirst = rk_->first;
+            xo.rnext = rk_->next;
+        }
         r2c_yx<R>(g_, plan_, xo, s_);
     }
+    rk_done_ = rk_ != nullptr;
 }
 
 // d_b X of OP_partial_derivative.cpp:104-136 into out (write or accumulate)
@@ -651,17 +659,33 @@ void Rk4<R>::steps(double2 *y[4], int n)
     }
     dim3 grid(nbk(nc), 4);
     const double c[3] = {0.5 * dt_, 0.5 * dt_, 1.0 * dt_};
+    // the update of each stage runs in the RHS's last store when the
+    // evaluation path allows it, else as its own kernel
+    RkEpilogue e{};
+    for (int f = 0; f < 4; ++f) {
+        e.y[f] = y[f];
+        e.acc[f] = acc_[f];
+    }
     for (int it = 0; it < n; ++it) {
         if (it == 0) hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
         const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
         for (int st = 0; st < 4; ++st) {
             rhs_.set_stage(st);
+            e.final_ = st == 3;
+            e.c = st < 3 ? c[st] : dt_ / 6.;
+            e.first = st == 0;
+            e.next = it + 1 < n;
+            rhs_.set_rk_epilogue(&e);
             rhs_.eval(tt[st], S_);
+            const bool fused = rhs_.rk_epilogue_done();
+            rhs_.set_rk_epilogue(nullptr);
+            if (fused) continue;
             if (st < 3)
                 hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0);
+            else
+                hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, nc, dt_ / 6., it + 1 < n ? 1 : 0);
         }
         rhs_.set_stage(-1);
-        hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, nc, dt_ / 6., it + 1 < n ? 1 : 0);
         KF_HIP(hipGetLastError());
         t_ += dt_;
     }

@@ -10,7 +10,7 @@ namespace kf {
 using namespace fft;
 
 enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN, ZK_RHO64 };
-enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN, XK_ADJ };
+enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN, XK_ADJ, XK_RK_STAGE, XK_RK_FINAL };
 
 // ------------------------------------------------------------------ spectral factors
 // 2/3-rule membership of mode (i, j, k): the exact per-element test
@@ -217,6 +217,13 @@ struct XStore {
     // XK_ADJ: o0 (+)= coef * [D23] * [-i k_dir] * W'/N * X   (adjoint of C2R)
     int d23, dir, acc;
     R coef, invN;
+    // XK_RK_STAGE / XK_RK_FINAL: the RHS k = -D23 X of one field (in R, as
+    // XK_NEG_D23 stores it) goes straight into the RK4 update of that field
+    // (the arithmetic of k_rk_stage / k_rk_final): ry the FP64 state,
+    // racc the FP64 accumulator, o0 the next stage input
+    double2 *ry, *racc;
+    double rc;
+    int rfirst, rnext;
     // linear terms of velocity component a (rhs_standard.cpp:119-140,
     // 219-247, 279-310) from its spectrum u; adds i k_a [D23] u to ln
     __device__ __forceinline__ C2<R> lin_u(C2<R> u, R fac, C2<R> F1, bool kp, R kx, R ka, C2<R> &ln) const
@@ -267,6 +274,37 @@ struct XStore {
         case XK_NEG_D23:
             o0[idx] = keep23(g, i, j, k) ? mk<C>(-X.x, -X.y) : mk<C>((R)0, (R)0);
             break;
+        case XK_RK_STAGE: {
+            // after stage s: acc = (s == 1 ? 1 k : acc + 2 k); S = y0 + c k
+            const C kk = keep23(g, i, j, k) ? mk<C>(-X.x, -X.y) : mk<C>((R)0, (R)0);
+            const double kx = (double)kk.x, ky = (double)kk.y;
+            double2 a;
+            if (rfirst) {
+                a.x = 1. * kx;
+                a.y = 1. * ky;
+            } else {
+                a = racc[idx];
+                a.x = a.x + 2. * kx;
+                a.y = a.y + 2. * ky;
+            }
+            racc[idx] = a;
+            const double2 y0 = ry[idx];
+            o0[idx] = mk<C>((R)(y0.x + rc * kx), (R)(y0.y + rc * ky));
+            break;
+        }
+        case XK_RK_FINAL: {
+            // y = y0 + (dt/6) (acc + 1 k4)   (RKO4.cpp:225-238); with rnext,
+            // also the next step's first stage input S = y
+            const C kk = keep23(g, i, j, k) ? mk<C>(-X.x, -X.y) : mk<C>((R)0, (R)0);
+            const double2 a = racc[idx];
+            const double2 y0 = ry[idx];
+            double2 o;
+            o.x = y0.x + rc * (a.x + 1. * (double)kk.x);
+            o.y = y0.y + rc * (a.y + 1. * (double)kk.y);
+            ry[idx] = o;
+            if (rnext) o0[idx] = mk<C>((R)o.x, (R)o.y);
+            break;
+        }
         case XK_NEG_ACC: {
             C l = a0[idx];
             o0[idx] = mk<C>(-(l.x + X.x), -(l.y + X.y));
@@ -334,6 +372,8 @@ inline void with_xstore(const XStore<R> &x, F &&f)
 {
     switch (x.kind) {
     case XK_NEG_D23: f(XStoreK<R, XK_NEG_D23>{x}); break;
+    case XK_RK_STAGE: f(XStoreK<R, XK_RK_STAGE>{x}); break;
+    case XK_RK_FINAL: f(XStoreK<R, XK_RK_FINAL>{x}); break;
     case XK_NEG_ACC: f(XStoreK<R, XK_NEG_ACC>{x}); break;
     case XK_LIN: f(XStoreK<R, XK_LIN>{x}); break;
     default: f(x);

@@ -49,6 +49,7 @@ def _eval(ctx, kind, n, prec, general, setting, env="KF_FFT_AUTOTUNE"):
     ("eqst", 24, "fp64", True),
     ("eqst", 32, "mixed", False),
     ("eqst", 48, "mixed", False),
+    ("eqst", 96, "mixed", False),
     ("sist", 16, "fp64", False),
 ])
 def test_autotuner_choices_give_identical_results(ctx, kind, n, prec, general):
@@ -68,6 +69,52 @@ def test_fused_penalization_pass_gives_identical_results(ctx, n, prec):
     print("fused penalization", n, prec, flush=True)
     a = _eval(ctx, "eqst", n, prec, False, None, env="KF_PEN_UNFUSED")
     b = _eval(ctx, "eqst", n, prec, False, "1", env="KF_PEN_UNFUSED")
+    names = ["FUx", "FUy", "FUz", "Fn", "phi"]
+    for name, x, y in zip(names, a, b):
+        assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
+
+
+def _rk_run(ctx, n, prec, steps, unfused):
+    import kielflow_amd as kf
+    old = os.environ.get("KF_PEN_UNFUSED")
+    if unfused:
+        os.environ["KF_PEN_UNFUSED"] = "1"
+    else:
+        os.environ.pop("KF_PEN_UNFUSED", None)
+    try:
+        state, nd, mask, og = initial_state("eqst", n, seed=4, Q=-0.2)
+        rng = np.random.default_rng(8)
+        for a in range(3):
+            state[a] = og.dealias23(og.r2c(1e-2 * rng.standard_normal(og.rshape)))
+        g = kf.Grid(ctx, [kf.axis(ax.type, ax.N, ax.l0, ax.L, ax.m) for ax in og.axes])
+        p = kf.KF_FP64 if prec == "fp64" else kf.KF_MIXED
+        ps = kf.Poisson(g, p, tol=1e-12 if prec == "fp64" else 1e-10, max_it=80)
+        rhs = kf.Rhs(g, ps, M=1.2, tau=0.1, theta=100.0, sv_on=False, pen_on=True, precision=p)
+        phi = g.rfield()
+        rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+        dF = [ctx.to_device(s) for s in state]
+        rk = kf.Rk4(rhs, 1e-3)
+        rk.step(dF, steps)          # one call: the final update also writes the next stage input
+        rk.step(dF, 1)              # and a call of one step
+        assert rhs.poisson_failures() == 0
+        out = [f.get() for f in dF] + [phi.get()]
+        for o in (rk, rhs, ps, g):
+            o.close()
+        return out
+    finally:
+        if old is None:
+            os.environ.pop("KF_PEN_UNFUSED", None)
+        else:
+            os.environ["KF_PEN_UNFUSED"] = old
+
+
+@pytest.mark.parametrize("n,prec", [(32, "fp64"), (48, "mixed")])
+def test_fused_rk_update_gives_identical_steps(ctx, n, prec):
+    """The RK4 update run by the RHS's last store gives the state of the
+    separate update kernels bit for bit, over several steps."""
+    print("fused rk update", n, prec, flush=True)
+    a = _rk_run(ctx, n, prec, 7, False)
+    b = _rk_run(ctx, n, prec, 7, True)
     names = ["FUx", "FUy", "FUz", "Fn", "phi"]
     for name, x, y in zip(names, a, b):
         assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
