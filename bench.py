@@ -421,15 +421,19 @@ def main():
     nc = n * n * (n // 2 + 1)
     C, Rb = 8 * nc, 4 * n ** 3  # bytes of one FP32 complex / real field
     # The dominant kernel family: the z passes of the inverse transforms
-    # (half spectrum -> real rows, FP32).  One RHS evaluation runs 12 of them;
-    # with their fused loads and stores they move 20 C + 40 R bytes:
-    #   rho: C + n_d (FP64) in, ln n + rho (FP64) out = C + 5R;
-    #   velocities: 3 (C + R);
-    #   sum_b U_b dF_a/db, one pass per field (3 transforms): 4 (3C + 4R);
-    #   penalization blend, 4 fields: C + 4R each (U or ln n, adv, mask in; adv out)
-    roof = roofline(kf.api.PROF_FFT_ZC2R, (20 * C + 40 * Rb) / 12,
-                    "FFT z passes (half spectrum -> real rows with fused products, FP32, Nz = 256)",
-                    (("k_zc2r<float, 128", "k_zprod3<float, 128"), None))
+    # (half spectrum -> real rows, FP32), timed together.  One RHS evaluation
+    # runs 8 of them; with their fused loads and stores they move 24 C + 28 R
+    # bytes:
+    #   rho: C + n_d (FP64) in, ln n + rho (FP64) out              = C + 5R
+    #   velocities: C in, U_b out, 3 times                         = 3 (C + R)
+    #   per field (k_zpen): the three staged derivative rows and the
+    #   linear terms' rows in (4C), U_0..2, f (U_a or ln n) and the
+    #   mask in (5R), the blend's forward z transform out (C), 4 times
+    #                                                              = 4 (5C + 5R)
+    roof = roofline(kf.api.PROF_FFT_ZC2R, (24 * C + 28 * Rb) / 8,
+                    "FFT z passes of the inverse transforms (k_zc2r, k_zpen: products, penalization blend and "
+                    "forward z transform fused; FP32, Nz = 256)",
+                    (("k_zc2r<float, 128", "k_zpen<float, 128"), None))
     # the FFT y passes (strided, FP32, length 256).  Per RHS evaluation: 4
     # that read a staged field once and write it and its i k_y multiple (3 C),
     # and 13 that read and write one complex field (2 C): 38 C over 17

@@ -233,43 +233,18 @@ constexpr int HIST = 6;
 struct Hist {
     const double *h[HIST];
 };
-// four consecutive points per thread (n is a multiple of 4 on every grid
-// with an even Nz; the tail is handled point by point otherwise), so that
-// each thread has all of its loads in flight at once
-__device__ __forceinline__ double extrap1(const double *const *h, int order, long i)
-{
-    if (order >= 5) return 6. * (h[5][i] + h[1][i]) - 15. * (h[4][i] + h[2][i]) + 20. * h[3][i] - h[0][i];
-    if (order == 4) return 5. * (h[5][i] - h[2][i]) - 10. * (h[4][i] - h[3][i]) + h[1][i];
-    if (order == 3) return 4. * (h[5][i] + h[3][i]) - 6. * h[4][i] - h[2][i];
-    if (order == 2) return 3. * (h[5][i] - h[4][i]) + h[3][i];
-    return 2. * h[5][i] - h[4][i];
-}
 __global__ void k_extrap(double *__restrict__ phi, Hist H, int order, long n)
 {
-    const long i0 = 4 * ((long)blockIdx.x * blockDim.x + threadIdx.x);
-    if (i0 >= n) return;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
     const double *const *h = H.h;
-    if (i0 + 4 > n || (n & 3)) {
-        for (long i = i0; i < n && i < i0 + 4; ++i) phi[i] = extrap1(h, order, i);
-        return;
-    }
-    double4 v;
-    double4 a5 = *reinterpret_cast<const double4 *>(h[5] + i0), a4 = *reinterpret_cast<const double4 *>(h[4] + i0);
-    if (order == 4) {
-        const double4 a3 = *reinterpret_cast<const double4 *>(h[3] + i0);
-        const double4 a2 = *reinterpret_cast<const double4 *>(h[2] + i0);
-        const double4 a1 = *reinterpret_cast<const double4 *>(h[1] + i0);
-        v.x = 5. * (a5.x - a2.x) - 10. * (a4.x - a3.x) + a1.x;
-        v.y = 5. * (a5.y - a2.y) - 10. * (a4.y - a3.y) + a1.y;
-        v.z = 5. * (a5.z - a2.z) - 10. * (a4.z - a3.z) + a1.z;
-        v.w = 5. * (a5.w - a2.w) - 10. * (a4.w - a3.w) + a1.w;
-    } else {
-        v.x = extrap1(h, order, i0);
-        v.y = extrap1(h, order, i0 + 1);
-        v.z = extrap1(h, order, i0 + 2);
-        v.w = extrap1(h, order, i0 + 3);
-    }
-    *reinterpret_cast<double4 *>(phi + i0) = v;
+    double v;
+    if (order >= 5) v = 6. * (h[5][i] + h[1][i]) - 15. * (h[4][i] + h[2][i]) + 20. * h[3][i] - h[0][i];
+    else if (order == 4) v = 5. * (h[5][i] - h[2][i]) - 10. * (h[4][i] - h[3][i]) + h[1][i];
+    else if (order == 3) v = 4. * (h[5][i] + h[3][i]) - 6. * h[4][i] - h[2][i];
+    else if (order == 2) v = 3. * (h[5][i] - h[4][i]) + h[3][i];
+    else v = 2. * h[5][i] - h[4][i];
+    phi[i] = v;
 }
 
 // h[0] <- h[1] <- ... <- h[HIST-1] <- phi
@@ -337,12 +312,12 @@ void Rhs<R>::poisson_begin(const C2<R> *Fn, const C2<R> *staged)
             // stage's history, which then becomes the newest, and the rest of
             // the evaluation reads it there.  The caller's potential receives
             // the last stage's solution once per step.
-            hipLaunchKernelGGL(k_extrap, dim3(nbk(nr, 1024)), dim3(256), 0, s_, ps_.guess_buffer(), H, order, nr);
+            hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, ps_.guess_buffer(), H, order, nr);
             slot_ = h[0];
             ps_.solve_begin(slot_, rho64, rho32, s_, true);
             return;
         }
-        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr, 1024)), dim3(256), 0, s_, phi_, H, order, nr);
+        hipLaunchKernelGGL(k_extrap, dim3(nbk(nr)), dim3(256), 0, s_, phi_, H, order, nr);
     }
     ps_.solve_begin(phi_, rho64, rho32, s_, false);
 }
