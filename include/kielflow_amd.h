@@ -226,6 +226,20 @@ int kf_rk4_combine(kf_ctx ctx, size_t n, double dt, const double *y0, const doub
 int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
                double *dt_new);
 
+/* Allocation-free variant of kf_cfl_all; it has no counterpart in the
+ * reference.  It is for callers that cannot allocate device or pinned host
+ * memory: code under graph capture or in a tight loop, and harnesses that
+ * forbid hipMalloc.  Its scratch (one real field, the spacing table and the
+ * per-block maxima) lives in a static device array of KF_NOALLOC_SCRATCH_BYTES
+ * per GPU, held until the process ends, so grids of up to about 4.1 million
+ * points (e.g. 128^3) fit; larger grids return KF_ERR_ARG.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race.  The result is
+ * kf_cfl_all's to rounding (the spacing table is formed on the device, where
+ * its multiply-adds may be fused). */
+#define KF_NOALLOC_SCRATCH_BYTES (32u << 20)
+int kf_cfl_all_noalloc(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
+                       double *dt_new);
+
 /* ---------------------------------------------------------------- batched boxes
  * Parameter sweeps over many independent boxes of uneven sizes (no
  * counterpart in the reference, which runs one box per process).  Each box

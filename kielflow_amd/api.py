@@ -100,6 +100,7 @@ SIGNATURES = {
     "kf_rk4_set_dt": (C.c_int, [_vp, C.c_double]),
     "kf_rk4_time": (C.c_int, [_vp, _dp]),
     "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
+    "kf_cfl_all_noalloc": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
     "kf_rk4_stage_input": (C.c_int, [_vp, C.c_size_t, C.c_double, _vp, _vp, _vp]),
     "kf_rk4_combine": (C.c_int, [_vp, C.c_size_t, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
     "kf_adjoint_create": (C.c_int, [_vp, C.c_double, C.POINTER(_vp)]),

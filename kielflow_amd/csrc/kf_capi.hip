@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 
 #include "kf_adjoint.h"
@@ -92,6 +93,32 @@ static int guard(F &&f)
 
 static void set_dev(kf_ctx c) { KF_HIP(hipSetDevice(c->device)); }
 
+// ------------------------------------------------------------------ static scratch
+// Device scratch of the allocation-free entry points (kf_cfl_all_noalloc):
+// one array of KF_NOALLOC_SCRATCH_BYTES per GPU, held until the process
+// ends.  In host code the array's name is not a device address: the
+// address comes from hipGetSymbolAddress, once per device with that device
+// current (kf_ctx_create), and is kept per device.  Calls that overlap on
+// one GPU share the array and race.
+static __device__ __align__(256) double scratch[KF_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+constexpr int KF_MAX_DEVICES = 64;
+static std::mutex g_scratch_mtx;
+static double *g_scratch_ptr[KF_MAX_DEVICES] = {};
+
+// the current device's address of `scratch` (the device must be current)
+static double *noalloc_scratch(int device)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mtx);
+    if (device < 0 || device >= KF_MAX_DEVICES) throw Error("device index beyond the scratch table");
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) throw Error(std::string("hip: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<double *>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
 extern "C" {
 
 const char *kf_last_error(void) { return g_err.c_str(); }
@@ -104,9 +131,11 @@ int kf_ctx_create(int device, kf_ctx *out)
         KF_HIP(hipGetDeviceCount(&n));
         KF_REQUIRE(device >= 0 && device < n, "no such device");
         KF_HIP(hipSetDevice(device));
-        auto *c = new kf_ctx_s{device, nullptr};
+        std::unique_ptr<kf_ctx_s> c(new kf_ctx_s{device, nullptr});
         KF_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-        *out = c;
+        // the device address of the static scratch, while the device is current
+        (void)noalloc_scratch(device);
+        *out = c.release();
         return KF_OK;
     });
 }
@@ -733,6 +762,25 @@ int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const
         KF_REQUIRE(g && FUx && FUy && FUz && dt_new, "null argument");
         const double2 *F[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
         *dt_new = cfl_all(*g->g, *g->p64, dt, F, g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+int kf_cfl_all_noalloc(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,
+                       double *dt_new)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FUx && FUy && FUz && dt_new, "null argument");
+        set_dev(g->ctx);
+        const double2 *F[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
+        const size_t need = cfl_all_scratch_doubles(*g->g);
+        if (need > KF_NOALLOC_SCRATCH_BYTES / sizeof(double)) {
+            g_err = "kf_cfl_all_noalloc: the grid needs " + std::to_string(need * sizeof(double)) +
+                    " bytes of scratch, more than the " + std::to_string((size_t)KF_NOALLOC_SCRATCH_BYTES) +
+                    " of the static scratch (use kf_cfl_all)";
+            return KF_ERR_ARG;
+        }
+        *dt_new = cfl_all_into(*g->g, *g->p64, dt, F, noalloc_scratch(g->ctx->device), g->ctx->stream);
         return KF_OK;
     });
 }

@@ -33,9 +33,35 @@ struct AxisDesc {
     double l0, L, m;
 };
 
-// axis functions restated on the host (axis_EqSt.cpp:52-84,
-// axis_SiSt.cpp:64-134, axis_HySt.cpp:49-80, axis_FoHySt.cpp:107-117)
-double axis_val_at(const AxisDesc &a, int index);
+// axis functions restated (axis_EqSt.cpp:52-84, axis_SiSt.cpp:64-134,
+// axis_HySt.cpp:49-80, axis_FoHySt.cpp:107-117); val_at also on the device
+__host__ __device__ inline double axis_val_at(const AxisDesc &a, int index)
+{
+    if (a.type == 1) return a.l0 + index * (a.L / a.n);  // axis_EqSt.cpp:52-55
+    if (a.type == 3) {                                  // axis_SiSt.cpp:64-89
+        int N = a.n, shift = 0;
+        if (index < 0) {
+            do { shift += 1; } while (index + shift * N < 0);
+        }
+        if (index >= N) {
+            do { shift -= 1; } while (index + shift * N >= N - 1);
+        }
+        const double pi = acos(-1.);
+        double x = a.l0 + a.L * (double)(index + shift * N) / (N);
+        return (x - a.m * sin(2. * pi * x / a.L)) - shift * a.L;
+    }
+    // axis_HySt.cpp:49-80
+    double L = a.L;
+    double x_val = a.l0 + L * (double)(index) / (a.n);
+    int shift = 0;
+    if (x_val < -L / 2.) {
+        do { x_val += L; shift -= 1; } while (x_val < -L / 2.);
+    }
+    if (x_val >= L / 2.) {
+        do { x_val -= L; shift += 1; } while (x_val >= a.l0 + L);
+    }
+    return shift * L + (L / 2.) * sinh(2 * (x_val) * a.m / L) / sinh(a.m);
+}
 double axis_k_val_at(const AxisDesc &a, int index);
 double axis_dS(const AxisDesc &a, int index);
 

@@ -191,6 +191,10 @@ void op_dealias23(Grid &g, double2 *inout, hipStream_t s);
 void op_filter_phi(Grid &g, double2 *inout, hipStream_t s);
 // CFL.cpp:29-82 (perturbation velocity, 0.01 cap)
 double cfl_all(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], hipStream_t s);
+// the same with a caller-provided device scratch of at least
+// cfl_all_scratch_doubles(g) doubles (nothing is allocated)
+size_t cfl_all_scratch_doubles(const Grid &g);
+double cfl_all_into(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], double *ws, hipStream_t s);
 // the RK4 arithmetic of RKO4.cpp for a right-hand side evaluated elsewhere:
 // out = y0 + c k, and out = y0 + (dt/6)(k1 + 2 k2 + 2 k3 + k4), over n doubles
 void rk4_stage_input(long n, double c, const double *y0, const double *k, double *out, hipStream_t s);

@@ -762,6 +762,86 @@ double cfl_all(Grid &g, Plan<double> &p, double dt, const double2 *F[3], hipStre
     return .4 * dt / cmax;
 }
 
+// CFL.cpp:6-12 spacing table of the three axes, h[i] = min(x(i+1) - x(i),
+// x(i+2) - x(i+1)), axis by axis
+__global__ void k_cfl_spacing(AxisDesc a0, AxisDesc a1, AxisDesc a2, double *__restrict__ h)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n0 = a0.n, n1 = a1.n, n2 = a2.n;
+    if (t >= n0 + n1 + n2) return;
+    const AxisDesc a = t < n0 ? a0 : (t < n0 + n1 ? a1 : a2);
+    const int i = t < n0 ? t : (t < n0 + n1 ? t - n0 : t - n0 - n1);
+    const double d0 = axis_val_at(a, i + 1) - axis_val_at(a, i);
+    const double d1 = axis_val_at(a, i + 2) - axis_val_at(a, i + 1);
+    h[t] = d0 < d1 ? d0 : d1;
+}
+
+// CFL.cpp:43-82 for one velocity component: per-block max of dt |U| / d
+__global__ void k_cfl1(const double *__restrict__ U, const double *__restrict__ d, int comp, int ny, int nz,
+                       long N, double dt, double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    double c = 0.;
+    if (p < N) {
+        int i, j, k;
+        dec3c(p, ny, nz, i, j, k);
+        c = dt * fabs(U[p]) / d[comp == 0 ? i : (comp == 1 ? j : k)];
+    }
+    sh[threadIdx.x] = c;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// max of n partial maxima into *out (one workgroup)
+__global__ void k_max_final(const double *__restrict__ part, long n, double *__restrict__ out)
+{
+    __shared__ double sh[256];
+    double m = 0.;
+    for (long q = threadIdx.x; q < n; q += 256) m = fmax(m, part[q]);
+    sh[threadIdx.x] = m;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + st]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = sh[0];
+}
+
+size_t cfl_all_scratch_doubles(const Grid &g)
+{
+    return (size_t)g.nr + (size_t)(g.nx + g.ny + g.nz) + 3 * (size_t)nbk(g.nr) + 4;
+}
+
+// cfl_all (CFL.cpp:29-82) on a scratch of cfl_all_scratch_doubles(g)
+// doubles: one velocity component at a time in real space, then the
+// maximum, which is independent of the order of the reductions, so the
+// result is cfl_all's
+double cfl_all_into(Grid &g, Plan<double> &p, double dt, const double2 *F[3], double *ws, hipStream_t s)
+{
+    const long nr = g.nr;
+    const unsigned nb = nbk(nr);
+    double *U = ws, *dd = U + nr, *part = dd + (g.nx + g.ny + g.nz), *out = part + 3 * (size_t)nb;
+    const int nt = g.nx + g.ny + g.nz;
+    hipLaunchKernelGGL(k_cfl_spacing, dim3((nt + 255) / 256), dim3(256), 0, s, g.ax[0], g.ax[1], g.ax[2], dd);
+    const long off[3] = {0, g.nx, g.nx + g.ny};
+    for (int b = 0; b < 3; ++b) {
+        c2r<double>(p, F[b], U, s);
+        hipLaunchKernelGGL(k_cfl1, dim3(nb), dim3(256), 0, s, (const double *)U, (const double *)(dd + off[b]), b,
+                           g.ny, g.nz, nr, dt, part + (size_t)b * nb);
+    }
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(256), 0, s, (const double *)part, 3 * (long)nb, out);
+    double cmax = 0.;
+    KF_HIP(hipMemcpyAsync(&cmax, out, sizeof(double), hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+    if (.4 * dt / cmax >= 0.01) return 0.01;
+    return .4 * dt / cmax;
+}
+
 // RKO4.cpp:140-153 (and :167-180, :194-207): out = y0 + c k, n doubles
 __global__ void k_rk_axpy(long n, double c, const double *__restrict__ y0, const double *__restrict__ k,
                           double *__restrict__ out)
