@@ -176,6 +176,25 @@ int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse);
  * on it.  Default: kielflow.cpp:207-217's {-3, 1, 1, 1}, {0, -1, -1, -1},
  * {1e-2, 5e-4, 1e-4, 1e-4}. */
 int kf_poisson_set_cascade(kf_poisson p, int n, const int *level_change, const int *steps, const double *tol);
+/* Boltzmann ions in the equation, as the reference frontend builds the
+ * solver (solver_poisson_jacobi_nlin.cpp:192-195, __FRONTEND__):
+ * L_h phi = exp(phi) - exp(-theta phi) - rho; theta = 0 turns them off.
+ * FP64 solvers only.  The reference modes keep the reference's local Newton
+ * (the ion term is in f, not in its derivative); Newton-MG uses the exact
+ * Jacobian. */
+int kf_poisson_set_ion_boltzmann(kf_poisson p, double theta);
+/* The equilibrium start of the reference frontend (create_input_from_MGsolver,
+ * frontend.cpp:133-198, and the smoothing of frontend.cpp:291-310): with the
+ * grain's charge density nd (FP64 real, from fkt3d_Gauss), solve the
+ * equation above with rho = nd by the given mode (KF_POISSON_CASCADE: the
+ * reference cascade with frontend.cpp:170-183's level program, tol and
+ * max_iterations unused; KF_POISSON_NLJ: its update test at tol;
+ * KF_POISSON_NEWTON_MG: the residual test at tol), set ni = exp(-theta phi),
+ * and pass ni and phi through exp(-2000 k^15) (dealiasing_undesignated).
+ * phi: in/out (the initial guess); ni: out.  The frontend's velocities are
+ * zero. */
+int kf_frontend_equilibrium(kf_grid g, double theta, const double *nd, int mode, double tol, int max_iterations,
+                            double *phi, double *ni, int *iters);
 /* solve(Phi_IO, rho): phi FP64 in/out (warm start), rho FP64.  Returns
  * KF_ERR_NOT_CONVERGED when the budget is spent; iters/resid may be NULL. */
 int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid);

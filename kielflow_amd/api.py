@@ -82,6 +82,8 @@ SIGNATURES = {
     "kf_poisson_set_max_iterations": (C.c_int, [_vp, C.c_int]),
     "kf_poisson_set_smoothing": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
     "kf_poisson_set_cascade": (C.c_int, [_vp, C.c_int, _ip, _ip, _dp]),
+    "kf_poisson_set_ion_boltzmann": (C.c_int, [_vp, C.c_double]),
+    "kf_frontend_equilibrium": (C.c_int, [_vp, C.c_double, _vp, C.c_int, C.c_double, C.c_int, _vp, _vp, _ip]),
     "kf_poisson_solve": (C.c_int, [_vp, _vp, _vp, _ip, _dp]),
     "kf_poisson_converged": (C.c_int, [_vp]),
     "kf_poisson_residual": (C.c_int, [_vp, _vp, _vp, _dp]),
@@ -340,6 +342,15 @@ class Grid:
         check(self.ctx.lib.kf_effect_penalization(self.h, self._r(mask, "mask"), eta, fix, int(fix_scaled_by_mask),
                                                   self._c(X, "input"), self._c(out, "out")))
 
+    def frontend_equilibrium(self, theta, nd, phi, ni, mode=POISSON_CASCADE, tol=1e-10, max_iterations=100):
+        """The equilibrium start of the reference frontend (frontend.cpp:133-198,
+        291-310): phi (in/out, the initial guess) and ni = exp(-theta phi),
+        both filtered; returns the solver's iteration count."""
+        it = C.c_int(0)
+        check(self.ctx.lib.kf_frontend_equilibrium(self.h, theta, self._r(nd, "n_d"), mode, tol, max_iterations,
+                                                   self._r(phi, "phi"), self._r(ni, "ni"), C.byref(it)))
+        return it.value
+
     def cfl(self, dt, FUx, FUy, FUz):
         out = C.c_double(0)
         check(self.ctx.lib.kf_cfl_all(self.h, dt, self._c(FUx, "FUx"), self._c(FUy, "FUy"), self._c(FUz, "FUz"),
@@ -398,6 +409,10 @@ class Poisson:
 
     def set_smoothing(self, pre, post, coarse=24):
         check(self.grid.ctx.lib.kf_poisson_set_smoothing(self.h, pre, post, coarse))
+
+    def set_ion_boltzmann(self, theta):
+        """Boltzmann ions exp(-theta phi) in the equation (the frontend's solver)."""
+        check(self.grid.ctx.lib.kf_poisson_set_ion_boltzmann(self.h, theta))
 
     def set_cascade(self, level_change, steps, tol):
         """Level program of the reference cascade (KF_POISSON_CASCADE)."""

@@ -545,6 +545,61 @@ int kf_poisson_set_cascade(kf_poisson p, int n, const int *level_change, const i
     });
 }
 
+int kf_poisson_set_ion_boltzmann(kf_poisson p, double theta)
+{
+    return guard([&] {
+        KF_REQUIRE(p, "null solver");
+        p->p->set_ion_boltzmann(theta);
+        return KF_OK;
+    });
+}
+
+int kf_frontend_equilibrium(kf_grid g, double theta, const double *nd, int mode, double tol, int max_iterations,
+                            double *phi, double *ni, int *iters)
+{
+    return guard([&] {
+        KF_REQUIRE(g && nd && phi && ni, "null argument");
+        KF_REQUIRE(theta > 0., "theta must be positive");
+        KF_REQUIRE(mode >= 0 && mode <= 2, "unknown Poisson mode");
+        set_dev(g->ctx);
+        hipStream_t s = g->ctx->stream;
+        Poisson ps(g->g->ax, false, s);
+        ps.set_mode(mode);
+        ps.set_ion_boltzmann(theta);
+        if (mode != POISSON_CASCADE) {
+            ps.set_tolerance(tol);
+            ps.set_max_iterations(max_iterations);
+        }
+        // frontend.cpp:164-190: the potential of the grain with Boltzmann
+        // electrons and ions (rho = n_d), then n_i = exp(-theta phi)
+        const int rc = ps.solve(phi, nd, nullptr, s, true);
+        const PoissonStats st = ps.last_stats();
+        if (iters) *iters = st.cycles;
+        if (rc == -2) {
+            g_err = "NaN in the potential";
+            return KF_ERR_NAN;
+        }
+        const long nr = g->g->nr, nc = g->g->nc;
+        boltzmann_ions(phi, theta, ni, nr, s);
+        // frontend.cpp:296-303: both through dealiasing_undesignated with
+        // exp(-2000 k^15)
+        double2 *F = nullptr;
+        KF_HIP(hipMalloc(&F, sizeof(double2) * nc));
+        for (double *f : {ni, phi}) {
+            fft::r2c<double>(*g->p64, f, F, s);
+            op_filter_phi(*g->g, F, s);
+            fft::c2r<double>(*g->p64, F, f, s);
+        }
+        KF_HIP(hipStreamSynchronize(s));
+        (void)hipFree(F);
+        if (mode == POISSON_NEWTON_MG && !st.converged) {
+            g_err = "equilibrium solve did not converge within the iteration budget";
+            return KF_ERR_NOT_CONVERGED;
+        }
+        return KF_OK;
+    });
+}
+
 int kf_poisson_solve(kf_poisson p, double *phi, const double *rho, int *iters, double *resid)
 {
     return guard([&] {

@@ -154,6 +154,11 @@ class Poisson {
     };
     NljState &nlj() { return nlj_; }
     int nlj_solve_level(int level, double *phi, const double *rho, hipStream_t s);
+    // Boltzmann ions exp(-theta phi) in the equation (the reference
+    // frontend's build of the solver, solver_poisson_jacobi_nlin.cpp:
+    // 192-195): L_h phi = exp(phi) - exp(-theta phi) - rho; 0 turns them off.
+    // FP64 solvers only.
+    void set_ion_boltzmann(double theta);
     // the cascade's level program (solver_poisson_multigrid::set_level_control)
     void set_cascade(int n, const int *change, const int *steps, const double *tol);
 
@@ -238,6 +243,7 @@ class Poisson {
     double *corr64_ = nullptr;    // FP64 scratch (mixed defect correction)
     double *alt64_ = nullptr;     // second potential buffer of the fused Newton update
     int expect_ = 1;              // corrections the last solve needed (host sync point of the next)
+    double ion_theta_ = 0.;       // Boltzmann-ion theta (0: none)
     bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
     int kiters_ = 2;
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)

@@ -156,7 +156,7 @@ constexpr int NR_PTS = 4;
 template <typename RIN, typename R>
 __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *__restrict__ rho,
-               R *__restrict__ b, R *__restrict__ e, double *__restrict__ part, const int *ctl)
+               R *__restrict__ b, R *__restrict__ e, double *__restrict__ part, const int *ctl, double theta)
 {
     __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
@@ -179,6 +179,12 @@ k_newton_resid(LevelView<double> L, const double *__restrict__ phi, const RIN *_
             ex[u] = exp(x);
             rh[u] = (double)rho[p];
             F[u] = A - D * x - ex[u] + rh[u];
+            if (theta != 0.) {
+                // Boltzmann ions: F += exp(-theta x), J gains -theta exp(-theta x)
+                const double ei = exp(-theta * x);
+                F[u] += ei;
+                ex[u] += theta * ei;
+            }
         }
     }
 #pragma unroll
@@ -207,7 +213,7 @@ template <typename RIN, typename R>
 __global__ void __launch_bounds__(RED_BLOCK)
 k_newton_upd_resid(LevelView<double> L, const double *__restrict__ phi_in, const R *__restrict__ x,
                    double *__restrict__ phi_out, int out_id, const RIN *__restrict__ rho, R *__restrict__ b,
-                   R *__restrict__ e, double *__restrict__ part, int *ctl)
+                   R *__restrict__ e, double *__restrict__ part, int *ctl, double theta)
 {
     __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
@@ -232,6 +238,11 @@ k_newton_upd_resid(LevelView<double> L, const double *__restrict__ phi_in, const
             ex[u] = exp(xn[u]);
             rh[u] = (double)rho[p];
             F[u] = A - D * xn[u] - ex[u] + rh[u];
+            if (theta != 0.) {
+                const double ei = exp(-theta * xn[u]);
+                F[u] += ei;
+                ex[u] += theta * ei;
+            }
         }
     }
 #pragma unroll
@@ -1213,18 +1224,28 @@ __global__ void k_apply_J(LevelView<R> L, const R *__restrict__ x, const R *__re
 
 // ======================================================================== reference NLJ kernels
 // newton_method::solve (newton_method.hpp:17-63), f = A - D x - e^x
-__device__ __forceinline__ double newton_ref(double x, double A, double D)
+// the local function of solver_poisson_jacobi_nlin.cpp:190-199; with
+// Boltzmann ions (the __FRONTEND__ build, theta > 0) exp(-theta x) is added
+// to f but not to its derivative (cpp:192-195)
+__device__ __forceinline__ double nlj_f(double x, double A, double D, double theta)
+{
+    double f = A - D * x - exp(x);
+    if (theta != 0.) f += exp(-theta * x);
+    return f;
+}
+
+__device__ __forceinline__ double newton_ref(double x, double A, double D, double theta)
 {
     const double eps = 0.05;
     const int Nmax = 20;
     double omega = 1.;
     const double g = 0.8;
     double dx = 0.;
-    double dx_alt = (A - D * x - exp(x)) / (-D - exp(x));
+    double dx_alt = nlj_f(x, A, D, theta) / (-D - exp(x));
     int i = 0;
     do {
         x -= omega * dx;
-        dx = (A - D * x - exp(x)) / (-D - exp(x));
+        dx = nlj_f(x, A, D, theta) / (-D - exp(x));
         omega = fabs(dx) > fabs(dx_alt) ? g * omega : (1. - g) * omega + g * 1.;
         dx_alt = dx;
         ++i;
@@ -1237,7 +1258,7 @@ __device__ __forceinline__ double newton_ref(double x, double A, double D)
 __global__ void __launch_bounds__(RED_BLOCK)
 k_nlj_ref(int nx, int ny, int nz, const double *__restrict__ HX, const double *__restrict__ HY,
           const double *__restrict__ HZ, const double *__restrict__ in, double *__restrict__ out,
-          const double *__restrict__ rho, double omega, double *__restrict__ part)
+          const double *__restrict__ rho, double omega, double theta, double *__restrict__ part)
 {
     __shared__ double sh[RED_BLOCK / 64];
     const long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x;
@@ -1262,7 +1283,7 @@ k_nlj_ref(int nx, int ny, int nz, const double *__restrict__ HX, const double *_
         A = 2. * A;
         A += rho[p];
         const double P_old = in[p];
-        const double P_new = newton_ref(P_old, A, D);
+        const double P_new = newton_ref(P_old, A, D, theta);
         const double delta = fabs(P_new - P_old);
         out[p] = (1. - omega) * P_old + omega * P_new;
         sup = P_new;
@@ -1646,7 +1667,8 @@ void Poisson::newton_check(int it)
     NewtonRun &r = run_;
     Level &L0 = lv_[0];
     const long N = L0.N;
-    const bool z4 = (L0.n[2] % 4) == 0;
+    // (the quad kernel has no Boltzmann-ion term: with ions, the generic one)
+    const bool z4 = (L0.n[2] % 4) == 0 && ion_theta_ == 0.;
     const unsigned nbr = z4 ? nblocks(N / 4, RED_BLOCK) : nblocks(N, RED_BLOCK * NR_PTS);
     LevelView<double> v64 = view<double>(0);
     R *x0, *b0, *e0;
@@ -1679,19 +1701,19 @@ void Poisson::newton_check(int it)
     } else if (it == 0) {
         if (rho64)
             hipLaunchKernelGGL((k_newton_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, (const double *)A,
-                               rho64, b0, e0, d_red_, ctl);
+                               rho64, b0, e0, d_red_, ctl, ion_theta_);
         else
             hipLaunchKernelGGL((k_newton_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, (const double *)A,
-                               rho32, b0, e0, d_red_, ctl);
+                               rho32, b0, e0, d_red_, ctl, ion_theta_);
     } else {
         const double *src = (it & 1) ? A : B;
         double *dst = (it & 1) ? B : A;
         if (rho64)
             hipLaunchKernelGGL((k_newton_upd_resid<double, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
-                               (const R *)x0, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_);
+                               (const R *)x0, dst, it & 1, rho64, b0, e0, d_red_, d_ctl_, ion_theta_);
         else
             hipLaunchKernelGGL((k_newton_upd_resid<float, R>), dim3(nbr), dim3(RED_BLOCK), 0, s, v64, src,
-                               (const R *)x0, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_);
+                               (const R *)x0, dst, it & 1, rho32, b0, e0, d_red_, d_ctl_, ion_theta_);
     }
     hipLaunchKernelGGL(k_newton_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr, it,
                        r.max_it, tol_, d_ctl_, d_fail_, d_stat_);
@@ -1833,6 +1855,13 @@ void Poisson::bicgstab(int iters, hipStream_t s)
     hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, x0, (const R *)X, N, ctl);
 }
 
+void Poisson::set_ion_boltzmann(double theta)
+{
+    if (theta < 0.) throw Error("the ion temperature ratio theta must be >= 0");
+    if (theta != 0. && mixed_) throw Error("Boltzmann ions are an FP64 solve (the equilibrium start)");
+    ion_theta_ = theta;
+}
+
 void Poisson::solve_begin(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool from_guess)
 {
     if (mode_ != POISSON_NEWTON_MG) {
@@ -1933,7 +1962,7 @@ int Poisson::nlj_solve_level(int level, double *phi, const double *rho, hipStrea
         KF_HIP(hipMemcpyAsync(&in0, phi, sizeof(double), hipMemcpyDeviceToHost, s));
         hipLaunchKernelGGL(k_nlj_ref, dim3(nb), dim3(RED_BLOCK), 0, s, n[0], n[1], n[2], (const double *)dH,
                            (const double *)(dH + n[0] + 1), (const double *)(dH + n[0] + n[1] + 2),
-                           (const double *)phi, tmp, rho, S.omega_SOR, d_red_);
+                           (const double *)phi, tmp, rho, S.omega_SOR, ion_theta_, d_red_);
         KF_HIP(hipMemcpyAsync(part.data(), d_red_, sizeof(double) * 4 * nb, hipMemcpyDeviceToHost, s));
         KF_HIP(hipMemcpyAsync(phi, tmp, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
         KF_HIP(hipStreamSynchronize(s));

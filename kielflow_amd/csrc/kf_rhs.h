@@ -195,6 +195,8 @@ double cfl_all(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], hi
 // cfl_all_scratch_doubles(g) doubles (nothing is allocated)
 size_t cfl_all_scratch_doubles(const Grid &g);
 double cfl_all_into(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], double *ws, hipStream_t s);
+// n_i = exp(-theta phi) (frontend.cpp:186-189)
+void boltzmann_ions(const double *phi, double theta, double *ni, long n, hipStream_t s);
 // the RK4 arithmetic of RKO4.cpp for a right-hand side evaluated elsewhere:
 // out = y0 + c k, and out = y0 + (dt/6)(k1 + 2 k2 + 2 k3 + k4), over n doubles
 void rk4_stage_input(long n, double c, const double *y0, const double *k, double *out, hipStream_t s);

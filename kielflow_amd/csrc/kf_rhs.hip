@@ -842,6 +842,18 @@ double cfl_all_into(Grid &g, Plan<double> &p, double dt, const double2 *F[3], do
     return .4 * dt / cmax;
 }
 
+// Boltzmann ions of the equilibrium start (frontend.cpp:186-189)
+__global__ void k_boltzmann_ions(const double *__restrict__ phi, double theta, double *__restrict__ ni, long n)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < n) ni[p] = exp(-theta * phi[p]);
+}
+
+void boltzmann_ions(const double *phi, double theta, double *ni, long n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_boltzmann_ions, dim3(nbk(n)), dim3(256), 0, s, phi, theta, ni, n);
+}
+
 // RKO4.cpp:140-153 (and :167-180, :194-207): out = y0 + c k, n doubles
 __global__ void k_rk_axpy(long n, double c, const double *__restrict__ y0, const double *__restrict__ k,
                           double *__restrict__ out)

@@ -534,18 +534,28 @@ static inline int wrap(int i, int n)
 #define PIDX(i, j, k) ((size_t)(k) + (size_t)nz * ((size_t)(j) + (size_t)(i) * ny))
 
 /* newton_method::solve (newton_method.hpp:17-63) for f = A - D x - e^x */
-static double newton_ref(double x, double A, double D)
+/* the local function of solver_poisson_jacobi_nlin.cpp:190-199; with the
+ * __FRONTEND__ build (theta > 0) the Boltzmann ions add exp(-theta x) to f
+ * but not to its derivative (cpp:192-195) */
+static double nlj_f(double x, double A, double D, double theta)
+{
+    double f = A - D * x - exp(x);
+    if (theta != 0.) f += exp(-theta * x);
+    return f;
+}
+
+static double newton_ref(double x, double A, double D, double theta)
 {
     const double eps = 0.05;
     const int Nmax = 20;
     double omega = 1.;
     double gg = 0.8;
     double dx = 0.;
-    double dx_alt = (A - D * x - exp(x)) / (-D - exp(x));
+    double dx_alt = nlj_f(x, A, D, theta) / (-D - exp(x));
     int i = 0;
     do {
         x -= omega * dx;
-        dx = (A - D * x - exp(x)) / (-D - exp(x));
+        dx = nlj_f(x, A, D, theta) / (-D - exp(x));
         omega = fabs(dx) > fabs(dx_alt) ? gg * omega : (1. - gg) * omega + gg * 1.;
         dx_alt = dx;
         ++i;
@@ -590,7 +600,7 @@ static void nlj_iteration(or_nlj_state *s, const spacing *sp, const double *in, 
                 A = 2. * A;
                 A += rho[PIDX(i, j, k)];
                 double P_old = in[PIDX(i, j, k)];
-                double P_new = newton_ref(P_old, A, D);
+                double P_new = newton_ref(P_old, A, D, s->ion_theta);
                 double delta = fabs(P_new - P_old);
                 out[PIDX(i, j, k)] = (1. - s->omega_SOR) * P_old + s->omega_SOR * P_new;
                 s->supremum = omax(s->supremum, P_new);
@@ -1720,6 +1730,45 @@ double or_objective_tl(or_rhs *r, const double *Fn, const double *Tn, const doub
     if (dJ) *dJ = t;
     free(c1); free(c2); free(L); free(dL); free(rho); free(drho); free(phi); free(dphi); free(E); free(dE);
     return J;
+}
+
+/* create_input_from_MGsolver (frontend.cpp:133-198) and the smoothing of
+ * frontend.cpp:291-310: the potential of the grain's charge n_d with
+ * Boltzmann electrons and ions, solver_poisson_jacobi_nlin's __FRONTEND__
+ * equation L_h phi = exp(phi) - exp(-theta phi) - n_d (cpp:192-195), by the
+ * reference cascade (the level program of frontend.cpp:170-183) or, with
+ * use_cascade = 0, by that NLJ run to its update test at limit `tol` with
+ * at most max_it iterations; then n_i = exp(-theta phi), and n_i and phi
+ * both through exp(-2000 k^15) (dealiasing_undesignated, operations.hpp:
+ * 20-46).  phi: in/out (the initial guess); returns the iterations or < 0. */
+int or_frontend_init(const or_grid *g, double theta, const double *nd, int use_cascade, double tol, int max_it,
+                     double *phi, double *ni)
+{
+    or_nlj_state s;
+    or_nlj_init(&s);
+    s.ion_theta = theta;
+    int it;
+    if (use_cascade) {
+        const int lvl[4] = {-3, 1, 1, 1}, steps[4] = {0, -1, -1, -1};
+        const double tl[4] = {.01, 0.0005, 0.0001, 0.0001};
+        it = or_mg_cascade_solve(&s, g, phi, nd, 4, lvl, steps, tl);
+    } else {
+        s.max_iterations = max_it;
+        s.limit_max = tol;
+        it = or_nlj_solve(&s, g, phi, nd);
+    }
+    if (it < 0) return it;
+    const size_t nr = g->nr, n2 = 2 * g->nc;
+    double *F = (double *)malloc(sizeof(double) * n2);
+    for (size_t q = 0; q < nr; ++q) ni[q] = exp(-theta * phi[q]);
+    or_fft_r2c(g, ni, F);
+    or_filter_phi(g, F);
+    or_fft_c2r(g, F, ni);
+    or_fft_r2c(g, phi, F);
+    or_filter_phi(g, F);
+    or_fft_c2r(g, F, phi);
+    free(F);
+    return it;
 }
 
 /* the cascade's transfer operators, exported for their known-answer tests */

@@ -72,6 +72,8 @@ typedef struct {
     double limit_max;
     double omega_SOR;
     int iteration, invocation, max_iterations, converged;
+    /* theta of the __FRONTEND__ build's Boltzmann ions (cpp:192-195), 0: off */
+    double ion_theta;
 } or_nlj_state;
 
 void or_nlj_init(or_nlj_state *s);
@@ -91,6 +93,11 @@ int or_poisson_fas(const or_grid *g, double *phi, const double *rho, double tol,
                    int max_cycles, double *resid_out);
 /* residual of the discrete equation (solver_poisson_jacobi_nlin.cpp:177-199) */
 double or_poisson_residual(const or_grid *g, const double *phi, const double *rho, double *r_out);
+
+/* equilibrium start of the reference frontend (frontend.cpp:133-198,
+ * 291-310); see kf_oracle.c */
+int or_frontend_init(const or_grid *g, double theta, const double *nd, int use_cascade, double tol, int max_it,
+                     double *phi, double *ni);
 
 /* ---- RHS (rhs_standard.cpp:89-566) ----------------------------------- */
 enum { OR_POISSON_FAS = 0, OR_POISSON_NLJ = 1, OR_POISSON_CASCADE = 2 };

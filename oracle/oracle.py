@@ -47,7 +47,8 @@ class NljState(C.Structure):
                 ("norm_max_old", C.c_double), ("norm_sum_old", C.c_double),
                 ("limit_max", C.c_double), ("omega_SOR", C.c_double),
                 ("iteration", C.c_int), ("invocation", C.c_int),
-                ("max_iterations", C.c_int), ("converged", C.c_int)]
+                ("max_iterations", C.c_int), ("converged", C.c_int),
+                ("ion_theta", C.c_double)]
 
 
 _lib = None
@@ -106,6 +107,8 @@ def lib():
         L.or_rk4_step_tl.argtypes = [vp, C.c_double, C.POINTER(vp), C.POINTER(vp), C.c_double, vp]
         L.or_objective_tl.restype = C.c_double
         L.or_objective_tl.argtypes = [vp, vp, vp, vp, vp, dp]
+        L.or_frontend_init.restype = C.c_int
+        L.or_frontend_init.argtypes = [vp, C.c_double, vp, C.c_int, C.c_double, C.c_int, vp, vp]
         L.or_cfl.restype = C.c_double
         L.or_cfl.argtypes = [vp, C.c_double, vp, vp, vp]
         L.or_philox4x32_10.argtypes = [vp, vp, vp]
@@ -259,6 +262,22 @@ class Grid:
         it = lib().or_mg_cascade_solve(C.byref(state), self.h, _p(phi), _p(rho), len(lvl),
                                        _p(lv), _p(st), _p(tl))
         return phi, it, state
+
+    def frontend_init(self, theta, nd, cascade=True, tol=1e-12, max_it=5000, phi0=None):
+        """(phi, n_i, iterations) of the reference frontend's equilibrium
+        start (frontend.cpp:133-198, 291-310)."""
+        nd = np.ascontiguousarray(nd, dtype=np.float64)
+        phi = np.zeros(self.rshape) if phi0 is None else np.array(phi0, dtype=np.float64, order="C")
+        ni = np.empty(self.rshape, np.float64)
+        it = lib().or_frontend_init(self.h, theta, _p(nd), int(cascade), tol, max_it, _p(phi), _p(ni))
+        return phi, ni, it
+
+    def nlj_ion(self, phi, rho, theta, max_iter=5000, tol=1e-13):
+        """The __FRONTEND__ NLJ (solver_poisson_jacobi_nlin.cpp:192-195)."""
+        state = NljState()
+        lib().or_nlj_init(C.byref(state))
+        state.ion_theta = theta
+        return self.nlj_solve(phi, rho, state=state, max_iter=max_iter, tol=tol)
 
     def cfl(self, dt, FUx, FUy, FUz):
         a = [np.ascontiguousarray(f, dtype=np.complex128) for f in (FUx, FUy, FUz)]
