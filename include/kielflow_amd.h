@@ -51,6 +51,7 @@ typedef struct kf_fft_s *kf_fft;
 typedef struct kf_poisson_s *kf_poisson;
 typedef struct kf_rhs_s *kf_rhs;
 typedef struct kf_rk4_s *kf_rk4;
+typedef struct kf_euler_s *kf_euler;
 
 /* axis_CoEqSt(l0, L, N) (axis_EqSt.hpp:15), axis_CoSiSt(l0, L, N, m)
  * (axis_SiSt.hpp), axis_CoHySt(l0, L, N, m) (axis_HySt.hpp) */
@@ -96,6 +97,11 @@ int kf_grid_destroy(kf_grid g);
 int kf_grid_dims(kf_grid g, int *nx, int *ny, int *nz, size_t *nreal, size_t *ncplx);
 /* axis::val_at / k_val_at / dS (axis_interface.hpp:10-12, axis.hpp:44) */
 int kf_axis_values(const kf_axis *a, int index, double *val, double *k, double *dS);
+/* axis::index_at (axis_EqSt.cpp:63-66, axis_HySt.cpp:121-147,
+ * axis_SiSt.cpp:98-113): the grid index of coordinate val (EqSt truncates,
+ * HySt and SiSt round; SiSt inverts its map by bisection to 1e-6, as the
+ * reference does); save_slice takes the z plane at index_at(0) */
+int kf_axis_index_at(const kf_axis *a, double val, int *index);
 
 /* ---------------------------------------------------------------- FFT
  * OP_FFT(grid) / OP_iFFT(grid) with operator()(in, out)
@@ -240,6 +246,51 @@ int kf_rk4_time(kf_rk4 k, double *t);
 int kf_rk4_stage_input(kf_ctx ctx, size_t n, double c, const double *y0, const double *k, double *out);
 int kf_rk4_combine(kf_ctx ctx, size_t n, double dt, const double *y0, const double *k1, const double *k2,
                    const double *k3, const double *k4, double *out);
+
+/* field_integrate::execute: the volume integral of a real field, with
+ * dV = prod_b S'_b(xi) L_b / N_b on stretched axes (used at frontend.cpp:
+ * 157-159; its header is missing from the reference). */
+int kf_field_integrate(kf_grid g, const double *f, double *result);
+
+/* The force decomposition of the reference backend (backend.cpp:271-654):
+ * the terms of d/dt u that rhs_standard.cpp accumulates, each as three real
+ * fields (x, y, z) in terms[(3 t + a) * Nreal] (may be NULL) with its volume
+ * integral in integrals[3 t + a] (may be NULL), t in the backend's order
+ *   0 DISSIPATION  -tau u            1 SPECTRAL_VISCOSITY  -SV u
+ *   2 E_INT  -d_a filt(phi)          3 DIFFUSION  -d_a ln n / theta
+ *   4 E_EXT  -M d_x (D23 u)          5 ADVECTION  -u . grad u
+ * from the spectral state and the real potential phi, with the parameters
+ * of the FP64 RHS r; and the grain's x-drag -integral n_d d_x phi dV into
+ * *drag (may be NULL).  Without penalization the six terms sum to the RHS.
+ * The terms follow rhs_standard's arithmetic, not the effect_* library
+ * bodies the backend calls (which differ from it). */
+#define KF_FORCE_TERMS 6
+int kf_force_decomposition(kf_rhs r, const double *FUx, const double *FUy, const double *FUz, const double *Fn,
+                           const double *phi, double *terms, double *integrals, double *drag);
+
+/* The stored form of a state, as kielflow writes and reads its HDF5 files
+ * (kielflow.cpp:156-182 on load, 290-327 before save_slice / save_all, and
+ * the iFFT of save_field_imag, IO.cpp:446-481): out = device double[5][Nreal]
+ * holding Ux, Uy, Uz = iFFT(FU), ni = exp(iFFT(F ln n)) and Ph = phi.
+ * The reference's FFT(exp(...)) then iFFT round trip is the identity and is
+ * not repeated.  Host files go through kielflow_io.h. */
+int kf_state_to_real(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *Fn,
+                     const double *phi, double *out);
+/* the same five fields on z plane k only (save_slice, IO.cpp:1175-1219, takes
+ * k = z_axis->index_at(0)): out = device double[5][Nx][Ny] */
+int kf_state_plane_z(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *Fn,
+                     const double *phi, int k, double *out);
+/* the inverse (load_field_imag, IO.cpp:484-504, and the conversion to
+ * ln n, kielflow.cpp:172-182): in = device double[5][Nreal] as above; phi
+ * may be NULL.  KF_ERR_ARG when some ni is not positive and finite. */
+int kf_state_from_real(kf_grid g, const double *in, double *FUx, double *FUy, double *FUz, double *Fn, double *phi);
+
+/* euler_method(interface_rhs&, dt) and euler_method::solve (euler_method.hpp:
+ * 27-31, euler_method.cpp:13-91): y = y + dt k per step, FP64 state */
+int kf_euler_create(kf_rhs r, double dt, kf_euler *out);
+int kf_euler_destroy(kf_euler e);
+int kf_euler_steps(kf_euler e, int n, double *FUx, double *FUy, double *FUz, double *Fn);
+int kf_euler_set_dt(kf_euler e, double dt);
 
 /* cfl_all(dt, FUx, FUy, FUz) (CFL.cpp:29-82) */
 int kf_cfl_all(kf_grid g, double dt, const double *FUx, const double *FUy, const double *FUz,

@@ -102,6 +102,16 @@ SIGNATURES = {
     "kf_rk4_set_dt": (C.c_int, [_vp, C.c_double]),
     "kf_rk4_time": (C.c_int, [_vp, _dp]),
     "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
+    "kf_field_integrate": (C.c_int, [_vp, _vp, _dp]),
+    "kf_axis_index_at": (C.c_int, [C.POINTER(Axis), C.c_double, _ip]),
+    "kf_state_to_real": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "kf_state_plane_z": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
+    "kf_state_from_real": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "kf_force_decomposition": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _dp, _dp]),
+    "kf_euler_create": (C.c_int, [_vp, C.c_double, C.POINTER(_vp)]),
+    "kf_euler_destroy": (C.c_int, [_vp]),
+    "kf_euler_steps": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "kf_euler_set_dt": (C.c_int, [_vp, C.c_double]),
     "kf_cfl_all_noalloc": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
     "kf_rk4_stage_input": (C.c_int, [_vp, C.c_size_t, C.c_double, _vp, _vp, _vp]),
     "kf_rk4_combine": (C.c_int, [_vp, C.c_size_t, C.c_double, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -167,6 +177,13 @@ def axis_values(ax: Axis, i: int):
     v, k, d = C.c_double(), C.c_double(), C.c_double()
     check(load().kf_axis_values(C.byref(ax), i, C.byref(v), C.byref(k), C.byref(d)))
     return v.value, k.value, d.value
+
+
+def axis_index_at(ax: Axis, val: float) -> int:
+    """axis::index_at: the grid index of coordinate val."""
+    i = C.c_int(0)
+    check(load().kf_axis_index_at(C.byref(ax), float(val), C.byref(i)))
+    return i.value
 
 
 def cell_volumes(axes):
@@ -342,6 +359,46 @@ class Grid:
         check(self.ctx.lib.kf_effect_penalization(self.h, self._r(mask, "mask"), eta, fix, int(fix_scaled_by_mask),
                                                   self._c(X, "input"), self._c(out, "out")))
 
+    def state_to_real(self, F, phi) -> np.ndarray:
+        """[Ux, Uy, Uz, ni (linear), Ph] as host double[5][Nx][Ny][Nz]: the
+        fields kielflow stores (kielflow.cpp:290-327, IO.cpp:446-481)."""
+        ptrs = _need4(F, self, "state")
+        out = self.ctx.empty((5,) + tuple(self.rshape), np.float64)
+        try:
+            check(self.ctx.lib.kf_state_to_real(self.h, *ptrs, self._r(phi, "phi"), out.ptr))
+            return out.get()
+        finally:
+            out.free()
+
+    def state_plane_z(self, F, phi, k: int) -> np.ndarray:
+        """The same five fields on z plane k: host double[5][Nx][Ny]."""
+        ptrs = _need4(F, self, "state")
+        out = self.ctx.empty((5, self.n[0], self.n[1]), np.float64)
+        try:
+            check(self.ctx.lib.kf_state_plane_z(self.h, *ptrs, self._r(phi, "phi"), int(k), out.ptr))
+            return out.get()
+        finally:
+            out.free()
+
+    def state_from_real(self, fields: np.ndarray, F, phi=None):
+        """The inverse: F = FFT(U), F ln n = FFT(log ni), phi = Ph."""
+        fields = np.ascontiguousarray(fields, np.float64)
+        if fields.shape != (5,) + tuple(self.rshape):
+            raise ValueError(f"fields: expected float64{(5,) + tuple(self.rshape)}, got {fields.shape}")
+        ptrs = _need4(F, self, "state")
+        d = self.ctx.to_device(fields)
+        try:
+            check(self.ctx.lib.kf_state_from_real(self.h, d.ptr, *ptrs,
+                                                  None if phi is None else self._r(phi, "phi")))
+        finally:
+            d.free()
+
+    def integrate(self, f: DeviceArray):
+        """field_integrate: the volume integral of a real field."""
+        out = C.c_double(0)
+        check(self.ctx.lib.kf_field_integrate(self.h, self._r(f, "field"), C.byref(out)))
+        return out.value
+
     def frontend_equilibrium(self, theta, nd, phi, ni, mode=POISSON_CASCADE, tol=1e-10, max_iterations=100):
         """The equilibrium start of the reference frontend (frontend.cpp:133-198,
         291-310): phi (in/out, the initial guess) and ni = exp(-theta phi),
@@ -480,6 +537,26 @@ class Rhs:
     def solve(self, t, F):
         check(self.grid.ctx.lib.kf_rhs_solve(self.h, t, *_need4(F, self.grid, "rhs state")))
 
+    FORCE_TERMS = ("DISSIPATION", "SPECTRAL_VISCOSITY", "E_INT", "DIFFUSION", "E_EXT", "ADVECTION")
+
+    def force_decomposition(self, F, phi, fields=True):
+        """(terms [6, 3, Nx, Ny, Nz] or None, integrals [6, 3], x-drag on the
+        grain) of the backend's force decomposition (backend.cpp:271-654)."""
+        g = self.grid
+        ptrs = _need4(F, g, "state")
+        nt = len(self.FORCE_TERMS)
+        terms = g.ctx.empty((nt, 3) + tuple(g.rshape), np.float64) if fields else None
+        integ = np.zeros(3 * nt, np.float64)
+        drag = C.c_double(0)
+        check(g.ctx.lib.kf_force_decomposition(self.h, *ptrs, _need(phi, g.rshape, np.float64, "phi"),
+                                               terms.ptr if fields else None,
+                                               integ.ctypes.data_as(_dp), C.byref(drag)))
+        host = None
+        if fields:
+            host = terms.get()
+            terms.free()
+        return host, integ.reshape(nt, 3), drag.value
+
     def force_general(self, on=True):
         check(self.grid.ctx.lib.kf_rhs_force_general(self.h, int(on)))
 
@@ -512,6 +589,27 @@ class Rk4:
         v = C.c_double(0)
         check(self.rhs.grid.ctx.lib.kf_rk4_time(self.h, C.byref(v)))
         return v.value
+
+
+class Euler:
+    """euler_method (euler_method.cpp:13-91): y = y + dt k per step."""
+
+    def __init__(self, rhs: Rhs, dt):
+        self.rhs = rhs
+        h = _vp()
+        check(rhs.grid.ctx.lib.kf_euler_create(rhs.h, dt, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.rhs.grid.ctx.lib.kf_euler_destroy(self.h)
+            self.h = None
+
+    def step(self, F, n=1):
+        check(self.rhs.grid.ctx.lib.kf_euler_steps(self.h, n, *_need4(F, self.rhs.grid, "euler state")))
+
+    def set_dt(self, dt):
+        check(self.rhs.grid.ctx.lib.kf_euler_set_dt(self.h, dt))
 
 
 class Adjoint:

@@ -49,8 +49,36 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_io(verbose)
     build_driver(verbose)
     return LIB
+
+
+IO_SRC = os.path.join(HERE, "host", "kf_io.cpp")
+IO_LIB = os.path.join(HERE, "libkielflow_io.so")
+HDF5_PREFIX = os.environ.get("KF_HDF5_PREFIX", "/opt/conda")
+
+
+def build_io(verbose: bool = False):
+    """libkielflow_io.so: the reference's HDF5 file layout over the HDF5 C
+    library (host code only).  Skipped, with a message, where no HDF5
+    headers are installed; the rest of the package does not depend on it."""
+    inc = os.path.join(os.path.dirname(HERE), "include")
+    h5inc = os.path.join(HDF5_PREFIX, "include")
+    h5lib = os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(h5inc, "hdf5.h")):
+        print(f"kielflow_amd: no hdf5.h under {h5inc}; libkielflow_io.so not built", file=sys.stderr)
+        return None
+    tmp = IO_LIB + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-Wextra", "-shared", "-fPIC", "-I", inc,
+           "-I", h5inc, IO_SRC, "-L", h5lib, "-lhdf5", f"-Wl,-rpath,{h5lib}", "-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"IO library build failed:\n{r.stderr}")
+    os.replace(tmp, IO_LIB)
+    if verbose:
+        print("built", IO_LIB)
+    return IO_LIB
 
 
 DRIVER_SRC = os.path.join(HERE, "host", "kf_driver.cpp")
@@ -59,10 +87,10 @@ DRIVER = os.path.join(HERE, "host", "kf_driver")
 
 def build_driver(verbose: bool = False) -> str:
     """The kielflow-style C++ driver over the reference-interface adaptors
-    (host code only: g++ against the C ABI of the library)."""
+    (host code only: g++ against the C ABIs of the two libraries)."""
     inc = os.path.join(os.path.dirname(HERE), "include")
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I", inc, DRIVER_SRC, "-L", HERE,
-           "-lkielflow_amd", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]
+           "-lkielflow_amd", "-lkielflow_io", "-Wl,-rpath,$ORIGIN/..", "-o", DRIVER]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"driver build failed:\n{r.stderr}")

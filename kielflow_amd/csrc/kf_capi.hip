@@ -65,6 +65,11 @@ struct kf_rk4_s {
     Rk4<double> *k64;
     Rk4<float> *k32;
 };
+struct kf_euler_s {
+    kf_rhs rhs;
+    Euler<double> *e64;
+    Euler<float> *e32;
+};
 
 template <class F>
 static int guard(F &&f)
@@ -297,6 +302,61 @@ int kf_axis_values(const kf_axis *a, int index, double *val, double *k, double *
         if (val) *val = axis_val_at(d, index);
         if (k) *k = axis_k_val_at(d, index);
         if (dS) *dS = axis_dS(d, index);
+        return KF_OK;
+    });
+}
+
+int kf_axis_index_at(const kf_axis *a, double val, int *index)
+{
+    return guard([&] {
+        KF_REQUIRE(a && index, "null argument");
+        KF_REQUIRE((a->type == KF_AXIS_EQST || a->type == KF_AXIS_HYST || a->type == KF_AXIS_SIST) && a->n > 0 &&
+                       a->L > 0,
+                   "bad axis");
+        KF_REQUIRE(a->type != KF_AXIS_HYST || a->m != 0., "a HySt axis needs m != 0");
+        const int N = a->n;
+        const double l0 = a->l0, L = a->L, m = a->m;
+        KF_REQUIRE(std::isfinite(val), "coordinate is not finite");
+        if (a->type == KF_AXIS_EQST) {
+            // axis_EqSt.cpp:63-66: the conversion to int truncates
+            *index = (int)(N * (val - l0) / L);
+        } else if (a->type == KF_AXIS_HYST) {
+            // axis_HySt.cpp:121-147: periodic shift, then the inverse map
+            KF_REQUIRE(std::fabs(val - l0) < 1.e6 * L, "coordinate too far outside the box");
+            double x = val;
+            int sh = 0;
+            while (x < l0) {
+                sh -= 1;
+                x += L;
+            }
+            while (x >= l0 + L) {
+                sh += 1;
+                x -= L;
+            }
+            x = (L / (2. * m)) * std::asinh(2. * x * std::sinh(m) / L);
+            *index = (int)std::nearbyint(N * (x - l0) / L) + N * sh;
+        } else {
+            // axis_SiSt.cpp:98-113: wrap into [l0, l0 + L), then bisect
+            // S(x) = x - m sin(2 pi x / L) for S(x) = val to 1e-6
+            // (axis_SiSt.cpp:6-13), round, and N maps to 0
+            KF_REQUIRE(std::fabs(val - l0) < 1.e6 * L, "coordinate too far outside the box");
+            double y = val;
+            while (y < l0) y += L;
+            while (y >= l0 + L) y -= L;
+            const double pi = std::acos(-1.);
+            auto S = [&](double x) { return x - m * std::sin(2. * pi * x / L); };
+            double lo = l0, hi = l0 + L, x = 0.5 * (lo + hi);
+            for (int it = 0; it < 200 && !(std::fabs(y - S(x)) < 1.e-6); ++it) {
+                if (S(x) < y)
+                    lo = x;
+                else
+                    hi = x;
+                x = 0.5 * (lo + hi);
+            }
+            int r = (int)std::nearbyint(N * (x - l0) / L);
+            if (r == N) r = 0;
+            *index = r;
+        }
         return KF_OK;
     });
 }
@@ -836,6 +896,185 @@ int kf_cfl_all_noalloc(kf_grid g, double dt, const double *FUx, const double *FU
             return KF_ERR_ARG;
         }
         *dt_new = cfl_all_into(*g->g, *g->p64, dt, F, noalloc_scratch(g->ctx->device), g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+int kf_field_integrate(kf_grid g, const double *f, double *result)
+{
+    return guard([&] {
+        KF_REQUIRE(g && f && result, "null argument");
+        set_dev(g->ctx);
+        *result = field_integrate(*g->g, f, nullptr, g->ctx->stream);
+        return KF_OK;
+    });
+}
+
+int kf_force_decomposition(kf_rhs r, const double *FUx, const double *FUy, const double *FUz, const double *Fn,
+                           const double *phi, double *terms, double *integrals, double *drag)
+{
+    return guard([&] {
+        KF_REQUIRE(r && FUx && FUy && FUz && Fn && phi, "null argument");
+        KF_REQUIRE(r->r64, "the force decomposition takes an FP64 RHS");
+        kf_grid gh = r->grid;
+        set_dev(gh->ctx);
+        Grid &g = *gh->g;
+        fft::Plan<double> &p = *gh->p64;
+        EffectWork &w = gh->work();
+        hipStream_t s = gh->ctx->stream;
+        const Params pa = r->r64->params();
+        const Switches sw = r->r64->switches();
+        const long nr = g.nr, nc = g.nc;
+        const double2 *FU[3] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz};
+        double2 *T[3] = {}, *Fphi = nullptr;
+        double *out = nullptr;
+        struct Free {
+            std::vector<void *> v;
+            ~Free()
+            {
+                for (void *q : v) (void)hipFree(q);
+            }
+        } fr;
+        for (auto &t : T) {
+            KF_HIP(hipMalloc(&t, sizeof(double2) * nc));
+            fr.v.push_back(t);
+        }
+        KF_HIP(hipMalloc(&Fphi, sizeof(double2) * nc));
+        fr.v.push_back(Fphi);
+        if (!terms) {
+            KF_HIP(hipMalloc(&out, sizeof(double) * nr));
+            fr.v.push_back(out);
+        }
+        fft::r2c<double>(p, phi, Fphi, s);
+        // the terms of d/dt u as rhs_standard.cpp accumulates them (each the
+        // negative of its part of B), in the reference backend's names:
+        // DISSIPATION, SPECTRAL_VISCOSITY, E_INT, DIFFUSION, E_EXT, ADVECTION
+        for (int t = 0; t < KF_FORCE_TERMS; ++t) {
+            for (auto q : T) KF_HIP(hipMemsetAsync(q, 0, sizeof(double2) * nc, s));
+            switch (t) {
+            case 0:
+                for (int a = 0; a < 3; ++a) effect_force_linear(g, -pa.tau, FU[a], T[a], s);
+                break;
+            case 1:
+                if (sw.sv_on)
+                    for (int a = 0; a < 3; ++a) effect_spectral_viscosity(g, FU[a], T[a], s);
+                break;
+            case 2: effect_force_E(g, p, w, Fphi, T, s); break;
+            case 3: effect_diffusion_u(g, p, w, pa.theta, (const double2 *)Fn, T, s); break;
+            case 4:
+                for (int a = 0; a < 3; ++a) effect_translation(g, p, w, pa.M, FU[a], T[a], s);
+                break;
+            default:
+                for (int a = 0; a < 3; ++a) effect_advection(g, p, w, FU, FU[a], T[a], s);
+            }
+            for (int a = 0; a < 3; ++a) {
+                double *o = terms ? terms + (size_t)(3 * t + a) * nr : out;
+                fft::c2r<double>(p, T[a], o, s);
+                if (integrals) integrals[3 * t + a] = field_integrate(g, o, nullptr, s);
+            }
+        }
+        if (drag) {
+            // the grain's x-drag, -integral of n_d d_x phi dV (the objective of
+            // the adjoint): d_x phi in T[0], real in a scratch field
+            op_partial_derivative(g, p, 0, Fphi, T[0], s);
+            double *dx = out;
+            if (!dx) {
+                KF_HIP(hipMalloc(&dx, sizeof(double) * nr));
+                fr.v.push_back(dx);
+            }
+            fft::c2r<double>(p, T[0], dx, s);
+            *drag = -field_integrate(g, dx, r->r64->nd(), s);
+        }
+        KF_HIP(hipStreamSynchronize(s));
+        return KF_OK;
+    });
+}
+
+int kf_state_to_real(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *Fn,
+                     const double *phi, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FUx && FUy && FUz && Fn && phi && out, "null argument");
+        set_dev(g->ctx);
+        const double2 *F[4] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz,
+                               (const double2 *)Fn};
+        state_to_real(*g->g, *g->p64, F, phi, out, g->ctx->stream);
+        KF_HIP(hipStreamSynchronize(g->ctx->stream));
+        return KF_OK;
+    });
+}
+
+int kf_state_plane_z(kf_grid g, const double *FUx, const double *FUy, const double *FUz, const double *Fn,
+                     const double *phi, int k, double *out)
+{
+    return guard([&] {
+        KF_REQUIRE(g && FUx && FUy && FUz && Fn && phi && out, "null argument");
+        KF_REQUIRE(k >= 0 && k < g->g->nz, "plane index outside [0, Nz)");
+        set_dev(g->ctx);
+        const double2 *F[4] = {(const double2 *)FUx, (const double2 *)FUy, (const double2 *)FUz,
+                               (const double2 *)Fn};
+        state_plane_z(*g->g, *g->p64, F, phi, k, out, g->ctx->stream);
+        KF_HIP(hipStreamSynchronize(g->ctx->stream));
+        return KF_OK;
+    });
+}
+
+int kf_state_from_real(kf_grid g, const double *in, double *FUx, double *FUy, double *FUz, double *Fn, double *phi)
+{
+    return guard([&] {
+        KF_REQUIRE(g && in && FUx && FUy && FUz && Fn, "null argument");
+        set_dev(g->ctx);
+        double2 *F[4] = {(double2 *)FUx, (double2 *)FUy, (double2 *)FUz, (double2 *)Fn};
+        KF_REQUIRE(state_from_real(*g->g, *g->p64, in, F, phi, g->ctx->stream),
+                   "the ion density must be positive and finite (it is stored linear and evolved as ln n)");
+        return KF_OK;
+    });
+}
+
+int kf_euler_create(kf_rhs r, double dt, kf_euler *out)
+{
+    return guard([&] {
+        KF_REQUIRE(r && out, "null argument");
+        set_dev(r->grid->ctx);
+        std::unique_ptr<kf_euler_s> e(new kf_euler_s());
+        e->rhs = r;
+        if (r->r64) e->e64 = new Euler<double>(*r->r64, dt);
+        else e->e32 = new Euler<float>(*r->r32, dt);
+        *out = e.release();
+        return KF_OK;
+    });
+}
+
+int kf_euler_destroy(kf_euler e)
+{
+    return guard([&] {
+        if (!e) return KF_OK;
+        (void)hipStreamSynchronize(e->rhs->grid->ctx->stream);
+        delete e->e64;
+        delete e->e32;
+        delete e;
+        return KF_OK;
+    });
+}
+
+int kf_euler_steps(kf_euler e, int n, double *FUx, double *FUy, double *FUz, double *Fn)
+{
+    return guard([&] {
+        KF_REQUIRE(e && FUx && FUy && FUz && Fn, "null argument");
+        double2 *y[4] = {(double2 *)FUx, (double2 *)FUy, (double2 *)FUz, (double2 *)Fn};
+        if (e->e64) e->e64->steps(y, n);
+        else e->e32->steps(y, n);
+        KF_HIP(hipGetLastError());
+        return KF_OK;
+    });
+}
+
+int kf_euler_set_dt(kf_euler e, double dt)
+{
+    return guard([&] {
+        KF_REQUIRE(e, "null integrator");
+        if (e->e64) e->e64->set_dt(dt);
+        else e->e32->set_dt(dt);
         return KF_OK;
     });
 }

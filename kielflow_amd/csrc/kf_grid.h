@@ -25,6 +25,9 @@ struct Grid {
     // x index only (rhs_standard.cpp:39-49)
     double *bar64 = nullptr;
     float *bar32 = nullptr;
+    // cell volumes per axis, S'(xi_i) L / N (their product is the volume
+    // element of field integrals on stretched grids)
+    double *dv[3] = {nullptr, nullptr, nullptr};
 
     // scratch of the stretched OP_partial_derivative and of cfl_all,
     // allocated on first use and kept with the grid: three real fields, the

@@ -134,6 +134,11 @@ Grid::Grid(const AxisDesc axes[3], hipStream_t s)
         bar64 = upload(b, s);
         bar32 = upload(std::vector<float>(b.begin(), b.end()), s);
     }
+    for (int d = 0; d < 3; ++d) {
+        std::vector<double> v(ax[d].n);
+        for (int i = 0; i < ax[d].n; ++i) v[i] = axis_dS(ax[d], i) * ax[d].L / ax[d].n;
+        dv[d] = upload(v, s);
+    }
     KF_HIP(hipStreamSynchronize(s));
 }
 
@@ -162,6 +167,7 @@ Grid::~Grid()
     (void)hipFree(imax);
     (void)hipFree(bar64);
     (void)hipFree(bar32);
+    for (int d = 0; d < 3; ++d) (void)hipFree(dv[d]);
 }
 
 }  // namespace kf

@@ -178,6 +178,27 @@ class Rk4 {
     double2 *acc_[4] = {};
 };
 
+// euler_method::solve (euler_method.cpp:13-91): y = y + dt k with an FP64
+// state, the RHS evaluated in precision R (its Poisson solves warm-started
+// from the extrapolation of the previous steps' potentials)
+template <typename R>
+class Euler {
+  public:
+    Euler(Rhs<R> &rhs, double dt);
+    ~Euler();
+    Euler(const Euler &) = delete;
+    Euler &operator=(const Euler &) = delete;
+    void steps(double2 *y[4], int n);
+    double dt() const { return dt_; }
+    void set_dt(double dt) { dt_ = dt; }
+    double t() const { return t_; }
+
+  private:
+    Rhs<R> &rhs_;
+    double dt_, t_ = 0.;
+    C2<R> *S_[4] = {};
+};
+
 // conversions and small field kernels shared with the C ABI
 void to_single(const double2 *in, float2 *out, long n, hipStream_t s);
 void to_double(const float2 *in, double2 *out, long n, hipStream_t s);
@@ -195,6 +216,21 @@ double cfl_all(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], hi
 // cfl_all_scratch_doubles(g) doubles (nothing is allocated)
 size_t cfl_all_scratch_doubles(const Grid &g);
 double cfl_all_into(Grid &g, fft::Plan<double> &p, double dt, const double2 *F[3], double *ws, hipStream_t s);
+// the volume integral of f [* w] over the box, dV = prod_b S'_b L_b / N_b
+// (field_integrate: used at frontend.cpp:157-159, its header is missing
+// from the reference); blocks of the grid's workspace hold the partial sums
+double field_integrate(Grid &g, const double *f, const double *w, hipStream_t s);
+// The stored form of a state (the fields of kielflow's HDF5 files,
+// kielflow.cpp:290-327): out = [Ux, Uy, Uz, n, Ph], 5 real fields, with
+// U = iFFT(FU), n = exp(iFFT(F ln n)) (linear density) and Ph = phi.
+void state_to_real(Grid &g, fft::Plan<double> &p, const double2 *F[4], const double *phi, double *out,
+                   hipStream_t s);
+// the same 5 fields on the z plane k only: out = double[5][nx][ny]
+void state_plane_z(Grid &g, fft::Plan<double> &p, const double2 *F[4], const double *phi, int k, double *out,
+                   hipStream_t s);
+// the inverse (kielflow.cpp:156-182): F = FFT(U), F ln n = FFT(log n),
+// phi = Ph (if phi is not null); false if some n is not positive and finite
+bool state_from_real(Grid &g, fft::Plan<double> &p, const double *in, double2 *F[4], double *phi, hipStream_t s);
 // n_i = exp(-theta phi) (frontend.cpp:186-189)
 void boltzmann_ions(const double *phi, double theta, double *ni, long n, hipStream_t s);
 // the RK4 arithmetic of RKO4.cpp for a right-hand side evaluated elsewhere:

@@ -666,6 +666,57 @@ void Rk4<R>::steps(double2 *y[4], int n)
     }
 }
 
+// euler_method.cpp:74-88: y = y + dt k
+template <typename R>
+__global__ void k_euler(Quad<R> q, long n, double dt)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    double2 *Y = const_cast<double2 *>(f == 0 ? q.y[0] : f == 1 ? q.y[1] : f == 2 ? q.y[2] : q.y[3]);
+    const C2<R> kk = q.S[f][i];
+    double2 v = Y[i];
+    v.x = v.x + dt * (double)kk.x;
+    v.y = v.y + dt * (double)kk.y;
+    Y[i] = v;
+}
+
+template <typename R>
+Euler<R>::Euler(Rhs<R> &rhs, double dt) : rhs_(rhs), dt_(dt)
+{
+    rhs.alloc_history();
+    for (int f = 0; f < 4; ++f) S_[f] = ralloc<C2<R>>(rhs.grid().nc);
+}
+
+template <typename R>
+Euler<R>::~Euler()
+{
+    for (int f = 0; f < 4; ++f) (void)hipFree(S_[f]);
+}
+
+template <typename R>
+void Euler<R>::steps(double2 *y[4], int n)
+{
+    hipStream_t s = rhs_.stream();
+    const long nc = rhs_.grid().nc;
+    Quad<R> q;
+    for (int f = 0; f < 4; ++f) {
+        q.y[f] = y[f];
+        q.S[f] = S_[f];
+        q.acc[f] = nullptr;
+    }
+    dim3 grid(nbk(nc), 4);
+    for (int it = 0; it < n; ++it) {
+        hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
+        rhs_.set_stage(0);
+        rhs_.eval(t_, S_);
+        rhs_.set_stage(-1);
+        hipLaunchKernelGGL(k_euler<R>, grid, dim3(256), 0, s, q, nc, dt_);
+        KF_HIP(hipGetLastError());
+        t_ += dt_;
+    }
+}
+
 // ------------------------------------------------------------------ OP_* on FP64 fields
 void op_partial_derivative(Grid &g, Plan<double> &p, int dir, const double2 *in, double2 *out,
                            hipStream_t s)
@@ -842,6 +893,113 @@ double cfl_all_into(Grid &g, Plan<double> &p, double dt, const double2 *F[3], do
     return .4 * dt / cmax;
 }
 
+// per-block sums of f [* w] dV (dV = dvx[i] dvy[j] dvz[k])
+__global__ void k_integrate(const double *__restrict__ f, const double *__restrict__ w,
+                            const double *__restrict__ dvx, const double *__restrict__ dvy,
+                            const double *__restrict__ dvz, int ny, int nz, long N, double *__restrict__ part)
+{
+    __shared__ double sh[256];
+    const long p = (long)blockIdx.x * 256 + threadIdx.x;
+    double v = 0.;
+    if (p < N) {
+        int i, j, k;
+        dec3c(p, ny, nz, i, j, k);
+        v = f[p] * (w ? w[p] : 1.) * (dvx[i] * dvy[j] * dvz[k]);
+    }
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+double field_integrate(Grid &g, const double *f, const double *w, hipStream_t s)
+{
+    const long nr = g.nr;
+    const unsigned nb = nbk(nr);
+    double *part = g.workspace() + 3 * nr + (g.nx + g.ny + g.nz);
+    hipLaunchKernelGGL(k_integrate, dim3(nb), dim3(256), 0, s, f, w, (const double *)g.dv[0],
+                       (const double *)g.dv[1], (const double *)g.dv[2], g.ny, g.nz, nr, part);
+    std::vector<double> h(nb);
+    KF_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+    double sum = 0.;
+    for (double v : h) sum += v;
+    return sum;
+}
+
+// ------------------------------------------------ state <-> stored fields
+
+__global__ void k_exp_inplace(double *a, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = exp(a[i]);
+}
+
+__global__ void k_fill_int(int *a, int n)
+{
+    const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (i < n) a[i] = 0;
+}
+
+// out = log(in); *bad |= 1 where in is not a positive finite number
+__global__ void k_log_checked(const double *__restrict__ in, double *__restrict__ out, int *bad, long n)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double v = in[i];
+    if (!(v > 0.) || !isfinite(v)) atomicOr(bad, 1);
+    out[i] = log(v);
+}
+
+// out[i][j] = f[i][j][k]
+__global__ void k_plane_z(const double *__restrict__ f, double *__restrict__ out, int ny, int nz, int k, long nxy)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p < nxy) out[p] = f[p * nz + k];
+}
+
+void state_to_real(Grid &g, Plan<double> &p, const double2 *F[4], const double *phi, double *out, hipStream_t s)
+{
+    const long nr = g.nr;
+    for (int a = 0; a < 4; ++a) c2r<double>(p, F[a], out + a * nr, s);
+    hipLaunchKernelGGL(k_exp_inplace, dim3(nbk(nr)), dim3(256), 0, s, out + 3 * nr, nr);
+    KF_HIP(hipMemcpyAsync(out + 4 * nr, phi, sizeof(double) * nr, hipMemcpyDeviceToDevice, s));
+}
+
+void state_plane_z(Grid &g, Plan<double> &p, const double2 *F[4], const double *phi, int k, double *out,
+                   hipStream_t s)
+{
+    const long nr = g.nr, nxy = (long)g.nx * g.ny;
+    double *tmp = g.workspace();
+    for (int a = 0; a < 4; ++a) {
+        c2r<double>(p, F[a], tmp, s);
+        hipLaunchKernelGGL(k_plane_z, dim3(nbk(nxy)), dim3(256), 0, s, (const double *)tmp, out + a * nxy, g.ny,
+                           g.nz, k, nxy);
+    }
+    hipLaunchKernelGGL(k_exp_inplace, dim3(nbk(nxy)), dim3(256), 0, s, out + 3 * nxy, nxy);
+    hipLaunchKernelGGL(k_plane_z, dim3(nbk(nxy)), dim3(256), 0, s, phi, out + 4 * nxy, g.ny, g.nz, k, nxy);
+    (void)nr;
+}
+
+bool state_from_real(Grid &g, Plan<double> &p, const double *in, double2 *F[4], double *phi, hipStream_t s)
+{
+    const long nr = g.nr;
+    double *tmp = g.workspace();
+    int *bad = (int *)(tmp + nr);
+    for (int a = 0; a < 3; ++a) r2c<double>(p, in + a * nr, F[a], s);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, s, bad, 1);
+    hipLaunchKernelGGL(k_log_checked, dim3(nbk(nr)), dim3(256), 0, s, in + 3 * nr, tmp, bad, nr);
+    r2c<double>(p, tmp, F[3], s);
+    if (phi) KF_HIP(hipMemcpyAsync(phi, in + 4 * nr, sizeof(double) * nr, hipMemcpyDeviceToDevice, s));
+    int h = 0;
+    KF_HIP(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, s));
+    KF_HIP(hipStreamSynchronize(s));
+    return h == 0;
+}
+
 // Boltzmann ions of the equilibrium start (frontend.cpp:186-189)
 __global__ void k_boltzmann_ions(const double *__restrict__ phi, double theta, double *__restrict__ ni, long n)
 {
@@ -887,5 +1045,7 @@ template class Rhs<float>;
 template class Rhs<double>;
 template class Rk4<float>;
 template class Rk4<double>;
+template class Euler<float>;
+template class Euler<double>;
 
 }  // namespace kf

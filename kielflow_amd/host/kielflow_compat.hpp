@@ -102,6 +102,13 @@ class axis {
     double val_at(const int &index) const { return values(index, 0); }
     double k_val_at(const int &index) const { return values(index, 1); }
     virtual double dS(const int &index) const { return values(index, 2); }
+    int index_at(const double &val) const
+    {
+        const kf_axis a = c();
+        int i = 0;
+        kf_compat::check(kf_axis_index_at(&a, val, &i));
+        return i;
+    }
     kf_axis c() const { return kf_axis{(int)type_id, N, l0, L, m}; }
 
   protected:
@@ -758,6 +765,54 @@ class rhs_standard : public interface_rhs {
     field_real &Phi, &nd, &my_solids;
     grid g_;
     kf_rhs r_ = nullptr;
+};
+
+// euler_method (euler_method.hpp:27-39): y = y + dt k per step.  With
+// kielflow_amd's rhs_standard the step runs in the library (kf_euler_*);
+// any other interface_rhs is evaluated through its solve() and the update
+// runs on the device (kf_rk4_stage_input with c = dt).
+class euler_method : public interface_ode_solver {
+  public:
+    explicit euler_method(interface_rhs &rhs, double dt = 0.1) : dt_(dt), my_rhs(rhs)
+    {
+        if (auto *r = dynamic_cast<rhs_standard *>(&rhs)) kf_compat::check(kf_euler_create(r->handle(), dt, &e_));
+    }
+    ~euler_method() override
+    {
+        if (e_) kf_euler_destroy(e_);
+    }
+    void solve(field_imag &FUx, field_imag &FUy, field_imag &FUz, field_imag &Fni) override
+    {
+        field_imag *y[4] = {&FUx, &FUy, &FUz, &Fni};
+        if (e_) {
+            auto &r = dynamic_cast<rhs_standard &>(my_rhs);
+            r.sync_parameters();
+            kf_compat::check(kf_euler_steps(e_, 1, (double *)FUx.upload(), (double *)FUy.upload(),
+                                            (double *)FUz.upload(), (double *)Fni.upload()));
+            for (field_imag *f : y) f->download();
+            r.potential().download();
+        } else {
+            std::vector<field_imag> k;
+            k.reserve(4);
+            for (int f = 0; f < 4; ++f) k.emplace_back(*y[f]);
+            my_rhs.solve(t_, k[0], k[1], k[2], k[3]);
+            for (int f = 0; f < 4; ++f) {
+                // y <- y + dt k, into y's device mirror (a separate buffer)
+                field_imag y0(*y[f]);
+                kf_compat::check(kf_rk4_stage_input(kf_compat::context(), 2 * (size_t)y[f]->N, dt_,
+                                                    (const double *)y0.upload(), (const double *)k[f].upload(),
+                                                    (double *)y[f]->dev()));
+                y[f]->download();
+            }
+        }
+        t_ += dt_;
+    }
+
+  private:
+    double dt_;
+    double t_ = 0.;
+    interface_rhs &my_rhs;
+    kf_euler e_ = nullptr;
 };
 
 class Runge_kutta_O4 : public interface_ode_solver {

@@ -1284,6 +1284,24 @@ int or_rk4_step(or_rhs *r, double dt, double *t_io, double *FUx, double *FUy, do
     return st;
 }
 
+/* euler_method::solve (euler_method.cpp:13-91) */
+int or_euler_step(or_rhs *r, double dt, double *t_io, double *FUx, double *FUy, double *FUz, double *Fn)
+{
+    size_t n2 = 2 * r->g->nc;
+    double *y[4] = {FUx, FUy, FUz, Fn};
+    double *k[4];
+    for (int f = 0; f < 4; ++f) {
+        k[f] = (double *)malloc(sizeof(double) * n2);
+        memcpy(k[f], y[f], sizeof(double) * n2);
+    }
+    int st = or_rhs_eval(r, *t_io, k[0], k[1], k[2], k[3]);
+    for (int f = 0; f < 4; ++f)
+        for (size_t i = 0; i < n2; ++i) y[f][i] = y[f][i] + dt * k[f][i];
+    *t_io += dt;
+    for (int f = 0; f < 4; ++f) free(k[f]);
+    return st;
+}
+
 static double get_delta(const or_axis *A, int i)
 {
     /* CFL.cpp:6-12 (the edge branch can never be taken) */

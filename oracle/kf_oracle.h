@@ -121,6 +121,9 @@ void or_rhs_get_phi(const or_rhs *r, double *phi);
 int  or_rhs_eval(or_rhs *r, double t, double *FUx, double *FUy, double *FUz, double *Fn);
 int  or_rhs_poisson_status(const or_rhs *r, double *last_resid, int *last_iters);
 
+/* ---- Euler (euler_method.cpp:13-91) ------------------------------------ */
+int or_euler_step(or_rhs *r, double dt, double *t_io, double *FUx, double *FUy, double *FUz, double *Fn);
+
 /* ---- RK4 (RKO4.cpp:50-248) -------------------------------------------- */
 int or_rk4_step(or_rhs *r, double dt, double *t_io,
                 double *FUx, double *FUy, double *FUz, double *Fn);

@@ -95,6 +95,8 @@ def lib():
         L.or_rhs_eval.argtypes = [vp, C.c_double, vp, vp, vp, vp]
         L.or_rhs_poisson_status.restype = C.c_int
         L.or_rhs_poisson_status.argtypes = [vp, dp, C.POINTER(C.c_int)]
+        L.or_euler_step.restype = C.c_int
+        L.or_euler_step.argtypes = [vp, C.c_double, dp, vp, vp, vp, vp]
         L.or_rk4_step.restype = C.c_int
         L.or_rk4_step.argtypes = [vp, C.c_double, dp, vp, vp, vp, vp]
         L.or_set_threads.argtypes = [C.c_int]
@@ -346,6 +348,12 @@ class Rhs:
         J = lib().or_objective_tl(self.h, _p(Fn), None if Tn is None else _p(Tn), _p(dV),
                                   None if d is None else _p(d), C.byref(dJ))
         return J, dJ.value
+
+    def euler(self, dt, t, state):
+        """euler_method::solve in place; returns (t_new, status)."""
+        tt = C.c_double(t)
+        st = lib().or_euler_step(self.h, dt, C.byref(tt), *[_p(a) for a in state])
+        return tt.value, st
 
     def rk4(self, dt, t, state):
         """In place on the 4 arrays of `state`; returns (t_new, status)."""

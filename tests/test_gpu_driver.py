@@ -101,3 +101,55 @@ def test_driver_stepper_over_a_foreign_rhs(ctx, tmp_path):
         assert rel_l2(got[a], y[a]) <= 1e-13, (a, rel_l2(got[a], y[a]))
     for o in (fft, rhs, ps, g):
         o.close()
+
+
+@pytest.mark.parametrize("wrapped", [False, True])
+def test_driver_euler_matches_oracle(ctx, oracle, tmp_path, wrapped):
+    """euler_method (euler_method.cpp:13-91) through the adaptor, on the
+    library's RHS and around a foreign one, against the C ABI's Euler steps
+    (kf_euler_steps) with the same Poisson tolerance, and those against the
+    oracle's Euler steps with a Poisson solve tight enough (1e-13) that the
+    residual tolerance does not show."""
+    import kielflow_amd as kf
+    n, steps, M, Q, dt = 16, 3, 1.0, -0.1, 1e-3
+    print("driver euler", wrapped, flush=True)
+    out = tmp_path / "state.bin"
+    r = subprocess.run([DRIVER, str(n), str(steps), str(out), "eqst", "newton", str(M), str(Q),
+                        "wrapped" if wrapped else "direct", "euler"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    nc = n * n * (n // 2 + 1)
+    got = [raw[2 * a * nc:2 * (a + 1) * nc].view(np.complex128).reshape(n, n, n // 2 + 1) for a in range(4)]
+    axes, oaxes = axes_for("eqst", n)
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=Q, a=grain_radius("eqst", n))
+    og = oracle.Grid(*oaxes)
+    orhs = oracle.Rhs(og, M=M, tau=0.1, theta=100.0, sv_on=False, pen_on=True, poisson_tol=1e-14, poisson_max=400)
+    orhs.set_fields(np.zeros(og.rshape), nd, mask)
+    ost = [np.zeros(og.cshape, complex) for _ in range(3)] + [og.r2c(ln_n)]
+    t = 0.0
+    for _ in range(steps):
+        t, s = orhs.euler(dt, t, ost)
+        assert s == 0
+    g = kf.Grid(ctx, axes)
+    fft = kf.FFT(g, kf.KF_FP64)
+    # the adaptor's path for a foreign RHS starts each Newton solve from another
+    # guess than the library's stepper, so the two agree to the Poisson
+    # residual tolerance (1e-10) rather than to rounding; Uy and Uz, driven
+    # only through the potential's transverse gradient, show it most
+    for tol, ref, bound in ((1e-10, got, 2e-9 if wrapped else 1e-12), (1e-13, ost, 1e-11)):
+        # the same steps through kf_euler_steps
+        ps = kf.Poisson(g, kf.KF_FP64, tol=tol, max_it=100)
+        rhs = kf.Rhs(g, ps, M=M, tau=0.1, theta=100.0, sv_on=False, pen_on=True)
+        phi = g.rfield()
+        rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+        F = [g.cfield() for _ in range(4)]
+        fft.r2c(ctx.to_device(ln_n), F[3])
+        eu = kf.Euler(rhs, dt)
+        eu.step(F, steps)
+        for a in range(4):
+            if np.abs(ref[a]).max() > 0:
+                assert rel_l2(F[a].get(), ref[a]) < bound, (tol, a, rel_l2(F[a].get(), ref[a]))
+        for o in (eu, rhs, ps):
+            o.close()
+    for o in (fft, g):
+        o.close()

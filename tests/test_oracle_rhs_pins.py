@@ -268,3 +268,25 @@ def test_cascade_ends_on_the_finest_discrete_solution(oracle):
                                tol=(0.01, 5e-4, 1e-4, 1e-13))
     assert st.converged == 1
     assert np.abs(phi - ref).max() < 1e-10 * np.abs(ref).max() + 1e-13
+
+
+def test_euler_amplification_of_a_linear_mode(oracle):
+    """euler_method (euler_method.cpp:74-88) on u_z = eps sin(kx), n = 1:
+    du_z/dt = -tau u_z - M d_x u_z, so one step multiplies each mode by 1 + z,
+    z = (-tau - i M k) dt."""
+    g = oracle.Grid.cube(N, L)
+    X, _, _ = _coords(oracle, g)
+    tau, M, dt = 0.1, 1.3, 0.01
+    r = oracle.Rhs(g, M=M, tau=tau, pen_on=False, sv_on=False, poisson_tol=1e-14)
+    r.set_fields(np.zeros(g.rshape), np.zeros(g.rshape), np.zeros(g.rshape))
+    st = _zero_state(g)
+    st[2] = g.r2c(1e-3 * np.sin(2 * K1 * X))
+    F0 = st[2].copy()
+    t, s = r.euler(dt, 0.0, st)
+    assert s == 0 and t == dt
+    kx = np.array([oracle.k_val_at(g.axes[0], i) for i in range(N)])[:, None, None]
+    amp = 1 + (-tau - 1j * M * kx) * dt * np.ones(g.cshape)
+    sel = np.abs(F0) > 1e-6
+    assert sel.sum() == 2
+    assert np.allclose(st[2][sel], amp[sel] * F0[sel], rtol=1e-14, atol=0)
+    assert max(np.abs(st[i]).max() for i in (0, 1, 3)) < 1e-15
