@@ -328,8 +328,9 @@ int kf_batch_set_fields(kf_batch b, int box, const double *nd, const double *mas
 /* record the graphs; Poisson solves inside run exactly poisson_iterations
  * Newton iterations (unconverged solves are counted as failures) with two
  * pre and two post sweeps per level.  The order of the potential warm start
- * (extrapolation in time) is recorded too: capture after at least 5
- * kf_batch_steps, when it is steady (quartic) */
+ * (extrapolation in time) is recorded too, so capture needs at least 5
+ * kf_batch_steps first, when it is steady (quartic): KF_ERR_ARG otherwise.
+ * A failure during a capture leaves no stream in capture mode. */
 int kf_batch_capture(kf_batch b, int poisson_iterations);
 int kf_batch_steps(kf_batch b, int nsteps);
 int kf_batch_synchronize(kf_batch b);

@@ -94,6 +94,14 @@ void Batch::steps(int n)
 void Batch::capture(int poisson_iterations)
 {
     sync();
+    // a graph keeps the extrapolation order and the buffers it was recorded
+    // with: the histories must be full (all lazy allocations done) first
+    for (Box *x : box_) {
+        const bool ready = x->r32 ? x->r32->history_ready() : x->r64->history_ready();
+        if (!x->exec && !ready)
+            throw Error("batch capture needs " + std::to_string(Rhs<double>::history_steps()) +
+                        " eager steps first (the potential histories are not full)");
+    }
     for (Box *x : box_) {
         if (x->exec) continue;
         x->ps->set_graph_iterations(poisson_iterations);
@@ -102,11 +110,23 @@ void Batch::capture(int poisson_iterations)
         x->ps->set_smoothing(2, 2, 24);
         hipGraph_t graph = nullptr;
         KF_HIP(hipStreamBeginCapture(x->s, hipStreamCaptureModeThreadLocal));
-        if (x->k32) x->k32->step(x->F);
-        else x->k64->step(x->F);
+        try {
+            if (x->k32) x->k32->step(x->F);
+            else x->k64->step(x->F);
+        } catch (...) {
+            // leave the stream out of capture mode whatever failed
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(x->s, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
         KF_HIP(hipStreamEndCapture(x->s, &graph));
-        KF_HIP(hipGraphInstantiate(&x->exec, graph, nullptr, nullptr, 0));
-        KF_HIP(hipGraphDestroy(graph));
+        hipError_t e = hipGraphInstantiate(&x->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            x->exec = nullptr;
+            KF_HIP(e);
+        }
     }
     // the capture only recorded the step: nothing advanced
     captured_ = true;

@@ -81,7 +81,11 @@ class Rhs {
     void set_capture_stages(Capture *c4) { cap_stages_ = c4; }
     // graph-safe history: rotate the per-stage potentials by copying
     // (fixed buffers) rather than by swapping pointers on the host
-    void set_graph_safe(bool v) { graph_safe_ = v; }
+    void set_graph_safe(bool v);
+    // every stage's history holds enough potentials for the full order of
+    // the extrapolation (a captured step keeps the order it was recorded with)
+    bool history_ready() const;
+    static constexpr int history_steps() { return 5; }
     // forget the per-stage potential history (the run jumped in time)
     void reset_history()
     {
@@ -105,6 +109,7 @@ class Rhs {
   private:
     void eval_fused(C2<R> *F[4]);
     void eval_general(C2<R> *F[4]);
+    void alloc_general();
     // rho from Fn (or from its staged x/y passes), then the Poisson solve
     void poisson_step(const C2<R> *Fn, const C2<R> *staged = nullptr);
     // poisson_step in two halves (work that does not read phi may go between)

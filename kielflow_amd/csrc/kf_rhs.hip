@@ -463,13 +463,36 @@ void Rhs<R>::deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accum
 }
 
 template <typename R>
+void Rhs<R>::alloc_general()
+{
+    if (F1_) return;
+    const long nc = g_.nc;
+    F1_ = ralloc<C2<R>>(nc);
+    for (auto &d : dF_) d = ralloc<C2<R>>(nc);
+}
+
+template <typename R>
+void Rhs<R>::set_graph_safe(bool v)
+{
+    graph_safe_ = v;
+    // a graph records buffer addresses: allocate the general path's
+    // buffers now where that path will run
+    if (v && (force_general_ || g_.stretched(0) || g_.stretched(1) || g_.stretched(2))) alloc_general();
+}
+
+template <typename R>
+bool Rhs<R>::history_ready() const
+{
+    for (int h : hc_)
+        if (h < extrap_order_ + 1) return false;
+    return true;
+}
+
+template <typename R>
 void Rhs<R>::eval_general(C2<R> *F[4])
 {
     const long nc = g_.nc;
-    if (!F1_) {
-        F1_ = ralloc<C2<R>>(nc);
-        for (auto &d : dF_) d = ralloc<C2<R>>(nc);
-    }
+    alloc_general();
     poisson_step(F[3]);
     Capture *cap = std::is_same<R, double>::value ? active_capture() : nullptr;
     const long nr = g_.nr;

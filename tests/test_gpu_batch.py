@@ -85,3 +85,25 @@ def test_batch_graph_replays_match_solo_runs(ctx):
             err = rel_l2(F[a].get(), ref)
             assert err <= 1e-5, (b, a, err)
     B.close()
+
+
+def test_batch_capture_needs_full_histories(ctx):
+    """kf_batch_capture refuses to record a step before the potential
+    histories are full (the graph would keep a lower extrapolation order)."""
+    import kielflow_amd as kf
+    axes = kf.cube(16, 16.0)
+    B = kf.Batch(ctx, [(axes, kf.api.Parameters(1.0, 0.1, 100.0, 0.0, 0.0))], precision=kf.KF_MIXED, nstreams=1)
+    ln_n, nd, mask = kf.setup_host(axes, seed=3, amp=1e-3, Q=-0.1, a=2.0)
+    B.set_fields(0, ctx.to_device(nd), ctx.to_device(mask))
+    B.steps(2)
+    B.sync()
+    with pytest.raises(kf.KfError, match="eager steps"):
+        B.capture(6)
+    B.steps(3)
+    B.sync()
+    B.capture(6)
+    B.steps(2)
+    B.sync()
+    assert B.poisson_failures() == 0
+    B.close()
+
