@@ -372,6 +372,13 @@ int kf_adjoint_run(kf_adjoint a, double *const y[4], int nsteps, int k, const do
  * checkpoint segment's fit, each segment is recomputed once and kept; else
  * every step is recomputed from the segment's stored states */
 int kf_adjoint_set_capture_budget(kf_adjoint a, size_t bytes);
+/* in the segments mode, the forward sweep also keeps every stage's converged
+ * potential (one real field per stage, where half the free HBM holds them)
+ * and the recomputation starts each Poisson solve from it: the solve ends at
+ * its first check and the recomputed segment reproduces the forward sweep's
+ * numbers.  On by default; keep = 0 recomputes the solves from the
+ * checkpointed potential. */
+int kf_adjoint_set_keep_potentials(kf_adjoint a, int keep);
 /* bytes of the kept intermediates of one RK4 step (four stages) */
 int kf_adjoint_step_capture_bytes(kf_adjoint a, size_t *bytes);
 /* of the last kf_adjoint_run: how it got its intermediates (KF_ADJ_*), and

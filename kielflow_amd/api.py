@@ -136,6 +136,7 @@ SIGNATURES = {
     "kf_batch_synchronize": (C.c_int, [_vp]),
     "kf_batch_poisson_failures": (C.c_int, [_vp, _ip]),
     "kf_adjoint_set_capture_budget": (C.c_int, [_vp, C.c_size_t]),
+    "kf_adjoint_set_keep_potentials": (C.c_int, [_vp, C.c_int]),
     "kf_adjoint_step_capture_bytes": (C.c_int, [_vp, C.POINTER(C.c_size_t)]),
     "kf_adjoint_run_info": (C.c_int, [_vp, _ip, C.POINTER(C.c_long)]),
     "kf_profile_enable": (C.c_int, [C.c_int]),
@@ -670,6 +671,9 @@ class Adjoint:
 
     def set_capture_budget(self, nbytes):
         check(self.lib.kf_adjoint_set_capture_budget(self.h, int(nbytes)))
+
+    def set_keep_potentials(self, keep=True):
+        check(self.lib.kf_adjoint_set_keep_potentials(self.h, int(bool(keep))))
 
     def step_capture_bytes(self):
         """Bytes of the kept intermediates of one RK4 step."""

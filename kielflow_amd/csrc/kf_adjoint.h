@@ -52,6 +52,9 @@ class Adjoint {
 
     // capture budget (bytes) of the stage intermediates kept in HBM
     void set_capture_budget(size_t b) { capture_budget_ = b; }
+    // segments mode: keep every stage's converged potential in the forward
+    // sweep and start the recomputation's solves from them (default on)
+    void set_keep_potentials(bool v) { keep_potentials_ = v; }
     // bytes of the intermediates of one RK4 step (four stages)
     size_t step_capture_bytes() const;
     // how the last run got its intermediates: RUN_RECOMPUTE (every step
@@ -78,6 +81,7 @@ class Adjoint {
     std::vector<Capture> pool_;      // per-stage intermediates of kept steps
     std::vector<void *> slabs_;      // the pool's allocations
     size_t capture_budget_ = (size_t)64 << 30;
+    bool keep_potentials_ = true;
     int run_mode_ = RUN_RECOMPUTE;
     double *lam_[4] = {};            // transposed Poisson solutions per stage (warm start)
     bool lam_valid_ = false;

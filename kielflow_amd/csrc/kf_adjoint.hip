@@ -686,6 +686,21 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
         seg.resize((size_t)k * 4);
         for (auto &p : seg) p = (double2 *)alloc(sizeof(double2) * nc);
     }
+    // With segments recomputed, the forward sweep also keeps every stage's
+    // converged potential (one real field per stage, about 1/21 of a stage's
+    // intermediates) where half the free HBM holds them: the recomputation
+    // then starts each solve from it, and the solve ends at its first check.
+    std::vector<double *> phis;
+    if (keep && !whole) {
+        size_t fr = 0, tot = 0;
+        KF_HIP(hipMemGetInfo(&fr, &tot));
+        const size_t need = (size_t)nsteps * 4 * sizeof(double) * nr;
+        if (keep_potentials_ && need <= fr / 2) {
+            double *slab = (double *)alloc(need);
+            phis.resize((size_t)nsteps * 4);
+            for (size_t q = 0; q < phis.size(); ++q) phis[q] = slab + q * nr;
+        }
+    }
     rk_->set_dt(dt_);
     rhs_.reset_history();
     for (int n = 0; n < nsteps; ++n) {
@@ -697,8 +712,10 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
                 KF_HIP(hipMemcpyAsync(ck[(size_t)c * 4 + a], y[a], sizeof(double2) * nc, hipMemcpyDeviceToDevice, s_));
             KF_HIP(hipMemcpyAsync(ckphi[c], rhs_.phi(), sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
         }
+        if (!phis.empty()) rhs_.set_phi_sink(&phis[(size_t)n * 4]);
         rk_->step(y);
     }
+    rhs_.set_phi_sink(nullptr);
     rhs_.set_capture_stages(nullptr);
     const double2 *yc[4] = {y[0], y[1], y[2], y[3]};
     const double J = objective(yc, dV);
@@ -720,8 +737,10 @@ double Adjoint::run(double2 *y[4], int nsteps, int k, const double *dV, const do
                                       hipMemcpyDeviceToDevice, s_));
             for (int n = n0; n < n1; ++n) {
                 rhs_.set_capture_stages(&pool_[(size_t)(n - n0) * 4]);
+                if (!phis.empty()) rhs_.set_phi_source(&phis[(size_t)n * 4]);
                 rk_->step(yscratch_);
             }
+            rhs_.set_phi_source(nullptr);
             rhs_.set_capture_stages(nullptr);
             for (int n = n1 - 1; n >= n0; --n) reverse_step(&pool_[(size_t)(n - n0) * 4], yb);
         } else {

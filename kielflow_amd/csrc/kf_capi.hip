@@ -1239,6 +1239,15 @@ int kf_adjoint_set_capture_budget(kf_adjoint a, size_t bytes)
     });
 }
 
+int kf_adjoint_set_keep_potentials(kf_adjoint a, int keep)
+{
+    return guard([&] {
+        KF_REQUIRE(a, "null adjoint");
+        a->a->set_keep_potentials(keep != 0);
+        return KF_OK;
+    });
+}
+
 int kf_adjoint_run_info(kf_adjoint a, int *mode, long *linear_iterations)
 {
     return guard([&] {

@@ -136,6 +136,10 @@ class Poisson {
     int solve(double *phi, const double *rho64, const float *rho32, hipStream_t s, bool sync,
               bool from_guess = false);
     double *guess_buffer() { return alt64_; }
+    // the next Newton-MG solve starts from a potential that converged for the
+    // same right side: enqueue no correction ahead of its first check (which
+    // ends it), and keep the expected count of the other solves
+    void next_guess_converged() { known_next_ = true; }
     // solve() in two halves, so that work that does not read phi can be
     // enqueued between them (Newton-MG; other modes solve in solve_begin):
     // solve_begin enqueues the corrections the last solve needed, solve_end
@@ -199,7 +203,7 @@ class Poisson {
         const double *rho64 = nullptr;
         const float *rho32 = nullptr;
         hipStream_t s = nullptr;
-        bool from_guess = false, graph = false, active = false;
+        bool from_guess = false, graph = false, active = false, known = false;
         int it = 0, max_it = 0;
     } run_;
     // BiCGStab on J x = b (level-0 b holds the right side on entry), right
@@ -243,6 +247,7 @@ class Poisson {
     double *corr64_ = nullptr;    // FP64 scratch (mixed defect correction)
     double *alt64_ = nullptr;     // second potential buffer of the fused Newton update
     int expect_ = 1;              // corrections the last solve needed (host sync point of the next)
+    bool known_next_ = false;     // the next solve's guess is a converged potential of the same rho
     double ion_theta_ = 0.;       // Boltzmann-ion theta (0: none)
     bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
     int kiters_ = 2;

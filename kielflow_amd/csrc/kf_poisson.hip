@@ -1756,9 +1756,11 @@ void Poisson::newton_begin(double *phi, const double *rho64, const float *rho32,
     r.max_it = r.graph ? graph_it_ : max_it_;
     r.it = 0;
     r.active = true;
+    r.known = known_next_ && !r.graph;
+    known_next_ = false;
     hipLaunchKernelGGL(k_fill<int>, dim3(1), dim3(8), 0, s, d_ctl_, 8L, 0, (const int *)nullptr);
     newton_check<R>(0);
-    const int upto = r.graph ? r.max_it : std::min(std::max(1, expect_), r.max_it);
+    const int upto = r.graph ? r.max_it : (r.known ? 0 : std::min(std::max(1, expect_), r.max_it));
     while (r.it < upto) {
         newton_correct<R>();
         newton_check<R>(++r.it);
@@ -1785,7 +1787,7 @@ void Poisson::newton_end()
             newton_check<R>(++r.it);
         }
         // ctl[1]: the iteration of the check that ended the solve
-        expect_ = std::max(1, h_ctl_[4 * (r.it & 1) + 1]);
+        if (!r.known) expect_ = std::max(1, h_ctl_[4 * (r.it & 1) + 1]);
     }
     // result in phi: in place (A = phi) copy when the newest iterate is in B
     // (ctl[3] == 1); from a guess (B = phi) copy when it is in A (ctl[3] == 0)

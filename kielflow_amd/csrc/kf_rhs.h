@@ -86,6 +86,14 @@ class Rhs {
     // the extrapolation (a captured step keeps the order it was recorded with)
     bool history_ready() const;
     static constexpr int history_steps() { return 5; }
+    // keep the converged potential of RK stage s in p4[s] after each solve
+    // (null: off); phi_cur_ is that potential, in phi_ or a history slot
+    void set_phi_sink(double *const *p4) { sink_ = p4; }
+    // start the solve of RK stage s from p4[s] instead of the extrapolated
+    // history (null: off).  A recomputation of a step from the same state
+    // with the potentials its first run kept converges at the first check and
+    // reproduces that run's numbers.
+    void set_phi_source(const double *const *p4) { src_ = p4; }
     // forget the per-stage potential history (the run jumped in time)
     void reset_history()
     {
@@ -158,6 +166,8 @@ class Rhs {
     double *ph_[4][6] = {};  // per RK stage: the last six converged potentials, newest last
     int hc_[4] = {0, 0, 0, 0};
     int extrap_order_ = 4;   // highest order of the extrapolation in time
+    double *const *sink_ = nullptr;
+    const double *const *src_ = nullptr;
 };
 
 // RK4 with an FP64 state and accumulator; stages evaluated in precision R.

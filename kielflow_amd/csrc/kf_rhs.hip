@@ -301,6 +301,15 @@ void Rhs<R>::poisson_begin(const C2<R> *Fn, const C2<R> *staged)
     slot_ = nullptr;
     const double *rho64 = std::is_same<R, double>::value ? (const double *)rho_ : (rho64_on_ ? rho64_ : nullptr);
     const float *rho32 = std::is_same<R, double>::value || rho64_on_ ? nullptr : (const float *)rho_;
+    if (src_ && stage_ >= 0 && !graph_safe_) {
+        // a kept potential of this stage (a recomputation of a step whose
+        // solve converged before): it passes the first convergence check
+        KF_HIP(hipMemcpyAsync(ps_.guess_buffer(), src_[stage_], sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        slot_ = ph_[stage_][0];
+        ps_.next_guess_converged();
+        ps_.solve_begin(slot_, rho64, rho32, s_, true);
+        return;
+    }
     if (stage_ >= 0 && hc_[stage_] >= 2) {
         double **h = ph_[stage_];
         const int order = std::min(hc_[stage_] - 1, extrap_order_);
@@ -346,6 +355,7 @@ void Rhs<R>::poisson_end()
         KF_HIP(hipMemcpyAsync(h[HIST - 1], phi_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
     }
     hc_[stage_]++;
+    if (sink_) KF_HIP(hipMemcpyAsync(sink_[stage_], phi_cur_, sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
 }
 
 template <typename R>

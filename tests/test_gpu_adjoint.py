@@ -196,6 +196,18 @@ def test_checkpointed_run_gradients(ctx):
         assert abs(J2 - J) <= 1e-9 * abs(J)
         assert abs(dM2 - dM) <= 1e-9 * abs(dM) and abs(dQ2 - dQ) <= 1e-9 * abs(dQ)
         assert np.abs(dFn02 - dFn0).max() <= 1e-9 * np.abs(dFn0).max()
+    # segments whose recomputation starts each solve from the potential the
+    # forward sweep kept reproduce that sweep: the kept trajectory's numbers
+    adj.set_capture_budget(k * adj.step_capture_bytes())
+    for keep, tol in ((True, 1e-12), (False, 1e-9)):
+        adj.set_keep_potentials(keep)
+        J3, dM3, dQ3, dFn03 = run_adj()
+        assert adj.run_info()[0] == kf.api.KF_ADJ_SEGMENTS
+        print(f"segments keep={keep}: dM {dM3:.15e}/{dM:.15e}", flush=True)
+        assert abs(J3 - J) <= tol * abs(J)
+        assert abs(dM3 - dM) <= tol * abs(dM) and abs(dQ3 - dQ) <= tol * abs(dQ)
+        assert np.abs(dFn03 - dFn0).max() <= tol * np.abs(dFn0).max()
+    adj.set_keep_potentials(True)
     J0 = run_J()
     assert abs(J - J0) <= 1e-8 * abs(J0)
     hM, hQ = 1e-3, 1e-4
