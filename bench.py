@@ -42,6 +42,10 @@ def parse():
     p.add_argument("--no-adjoint", action="store_true", help="skip the adjoint cost leg")
     p.add_argument("--adjoint-steps", type=int, default=200)
     p.add_argument("--no-batch", action="store_true", help="skip the batched-boxes leg")
+    p.add_argument("--no-config1", action="store_true", help="skip the 128^3 SiSt FP64 leg")
+    p.add_argument("--config1-steps", type=int, default=20)
+    p.add_argument("--only", choices=("adjoint", "config1", "batch"), default=None,
+                   help="run only this leg and print its JSON (for profiling)")
     p.add_argument("--smoothing", type=str, default="",
                    help="Poisson smoothing sweeps 'pre,post,coarse' (library default when empty)")
     p.add_argument("--long", type=int, default=0,
@@ -239,6 +243,49 @@ def adjoint_cost(kf, ctx, n=64, steps=200, k=20, m=3.0):
                                 "dJ_dQ_rel_diff": abs(dQ2 - dQ) / max(abs(dQ), 1e-300)}}
 
 
+def config1_rate(kf, ctx, n=128, steps=20, warm=2):
+    """FP64 RK4 steps/s on the 128^3 sine-stretched box (S'(0) = 1/3),
+    M = 1.2, Q = -0.2, 1e-3 density noise, spectral viscosity and
+    penalization on, Poisson to 1e-10, dt from cfl_all before every step."""
+    m = L_BOX * (1 - 1 / 3) / (2 * np.pi)
+    axes = [kf.axis(kf.SIST, n, -0.5 * L_BOX, L_BOX, m) for _ in range(3)]
+    x0 = kf.api.axis_values(axes[0], n // 2)[0]
+    x1 = kf.api.axis_values(axes[0], n // 2 + 1)[0]
+    ln_n, nd, mask = kf.setup_host(axes, seed=0, amp=1e-3, Q=-0.2, a=2.0 * (x1 - x0))
+    g = kf.Grid(ctx, axes)
+    ps = kf.Poisson(g, kf.KF_FP64, tol=1e-10, max_it=80)
+    rhs = kf.Rhs(g, ps, M=1.2, tau=0.1, theta=100.0, sv_on=True, pen_on=True)
+    phi = g.rfield()
+    rhs.set_fields(phi, ctx.to_device(nd), ctx.to_device(mask))
+    fft = kf.FFT(g, kf.KF_FP64)
+    F = [g.cfield() for _ in range(4)]
+    fft.r2c(ctx.to_device(ln_n), F[3])
+    rk = kf.Rk4(rhs, 1e-3)
+    dt = 1e-3
+
+    def step():
+        nonlocal dt
+        dt = g.cfl(dt, F[0], F[1], F[2])
+        rk.set_dt(dt)
+        rk.step(F, 1)
+
+    for _ in range(warm):
+        step()
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    ctx.sync()
+    t = time.perf_counter() - t0
+    fails = rhs.poisson_failures()
+    for o in (rk, fft, rhs, ps):
+        o.close()
+    g.close()
+    return {"workload": f"{n}^3 SiSt (S'(0) = 1/3) FP64 RK4, spectral viscosity and penalization on, "
+                        "dt from cfl_all every step", "steps_per_s": round(steps / t, 3),
+            "ms_per_step": round(1e3 * t / steps, 3), "steps": steps, "last_dt": dt, "poisson_failures": fails}
+
+
 def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=4, check_boxes=4):
     """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
     M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
@@ -400,6 +447,16 @@ def main():
         long_run(kf, args.n, args.long)
         return
     import kielflow_amd as kf
+    if args.only:
+        ctx = kf.Context(0)
+        if args.only == "adjoint":
+            out = adjoint_cost(kf, ctx, steps=args.adjoint_steps)
+        elif args.only == "config1":
+            out = config1_rate(kf, ctx, steps=args.config1_steps)
+        else:
+            out = batch_sweep(kf, ctx, steps=args.batch_steps)
+        print(json.dumps(out))
+        return
     n = args.n
     axes = kf.cube(n, L_BOX)
     a = 2.0 * L_BOX / n
@@ -476,6 +533,8 @@ def main():
 
     if not args.no_adjoint:
         extra["adjoint"] = adjoint_cost(kf, ctx, steps=args.adjoint_steps)
+    if not args.no_config1:
+        extra["config1"] = config1_rate(kf, ctx, steps=args.config1_steps)
     if not args.no_batch:
         extra["batch"] = batch_sweep(kf, ctx, steps=args.batch_steps)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_n, n)
