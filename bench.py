@@ -289,8 +289,7 @@ def config1_rate(kf, ctx, n=128, steps=20, warm=2):
 def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=4, check_boxes=4):
     """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
     M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
-    precision, the batch step captured as one graph with a branch per stream
-    (`nstreams` streams).
+    precision, one captured graph per box and step over `nstreams` streams.
     Error: the first box of each size against an FP64 solo run."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([32, 48, 64, 96], nbox)

@@ -313,9 +313,9 @@ int kf_cfl_all_noalloc(kf_grid g, double dt, const double *FUx, const double *FU
 /* ---------------------------------------------------------------- batched boxes
  * Parameter sweeps over many independent boxes of uneven sizes (no
  * counterpart in the reference, which runs one box per process).  Each box
- * is a complete solver on one of nstreams streams; after kf_batch_capture
- * one RK4 step of every box is one HIP graph (a branch per stream), and
- * kf_batch_steps launches it once per step.  precision: KF_FP64 or KF_MIXED. */
+ * is a complete solver; after kf_batch_capture one RK4 step of every box is
+ * a HIP graph, and kf_batch_steps replays them round-robin over nstreams
+ * streams.  precision: KF_FP64 or KF_MIXED. */
 typedef struct kf_batch_s *kf_batch;
 int kf_batch_create(kf_ctx ctx, int nbox, const kf_axis *axes /* 3 per box */, const kf_parameters *params,
                     const kf_rhs_switches *sw, int precision, double dt, double poisson_tol, int nstreams,

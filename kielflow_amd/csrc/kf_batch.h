@@ -1,5 +1,5 @@
-// kf_batch.h -- independent boxes of uneven sizes advanced together: the RK4
-// step of every box, spread over a few streams, captured as one HIP graph.
+// kf_batch.h -- independent boxes of uneven sizes advanced together, one
+// captured HIP graph per box and RK4 step, boxes spread over a few streams.
 #pragma once
 
 #include <vector>
@@ -30,10 +30,9 @@ class Batch {
     Grid &grid(int b) { return *box_.at(b)->g; }
     // n_d and mask of box b (device, FP64; copied)
     void set_fields(int b, const double *nd, const double *mask);
-    // record one RK4 step of every box as one graph (the boxes' streams are
-    // branches of it); later steps replay it.  The Poisson solves of the
-    // graph run exactly poisson_iterations Newton iterations (early exit on
-    // the device once converged).
+    // record one RK4 step of every box as a graph; later steps replay it.
+    // The Poisson solves of the graph run exactly poisson_iterations
+    // Newton iterations (early exit on the device once converged).
     void capture(int poisson_iterations);
     bool captured() const { return captured_; }
     // n steps of every box (enqueued; call sync() to wait)
@@ -53,14 +52,13 @@ class Batch {
         Rk4<double> *k64 = nullptr;
         double2 *F[4] = {};
         double *phi = nullptr;
+        hipGraphExec_t exec = nullptr;
     };
     void step_box(Box &x);
     int precision_;
     double dt_;
     bool captured_ = false;
-    hipGraphExec_t exec_ = nullptr;
     std::vector<hipStream_t> streams_;
-    std::vector<hipEvent_t> events_;  // fork, then one join per stream
     std::vector<Box *> box_;
 };
 

@@ -2,12 +2,10 @@
 // (parameter sweeps).  Every box is a complete solver (grid, FFT plan,
 // Newton-MG Poisson, RHS, RK4) on one of a few HIP streams.  After a few
 // eager steps fill the per-stage potential histories, one RK4 step of every
-// box is captured into ONE HIP graph: the first stream forks to the others
-// through an event, each box's step is recorded on its stream (a branch of
-// the graph), and the other streams join back before the capture ends.  A
-// batch step is then one graph launch; the small boxes' kernels overlap
-// across the branches and no host launch loop remains.  Nothing is shared
-// between boxes, so a box's result does not depend on the batch it runs in.
+// box is captured into a HIP graph; a batch step is then one graph launch
+// per box, round-robin over the streams, so the small boxes' kernels overlap
+// across streams and no host launch loop remains.  Nothing is shared between
+// boxes, so a box's result does not depend on the batch it runs in.
 #include "kf_batch.h"
 
 #include <algorithm>
@@ -22,8 +20,6 @@ Batch::Batch(const std::vector<BatchBox> &boxes, const Switches &sw, int precisi
     nstreams = std::max(1, std::min(nstreams, 16));
     streams_.resize(nstreams);
     for (auto &s : streams_) KF_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    events_.resize(nstreams + 1);
-    for (auto &e : events_) KF_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     for (size_t b = 0; b < boxes.size(); ++b) {
         Box *x = new Box();
         x->s = streams_[b % streams_.size()];
@@ -56,8 +52,8 @@ Batch::Batch(const std::vector<BatchBox> &boxes, const Switches &sw, int precisi
 Batch::~Batch()
 {
     sync();
-    if (exec_) (void)hipGraphExecDestroy(exec_);
     for (Box *x : box_) {
+        if (x->exec) (void)hipGraphExecDestroy(x->exec);
         delete x->k32;
         delete x->k64;
         delete x->r32;
@@ -68,7 +64,6 @@ Batch::~Batch()
         (void)hipFree(x->phi);
         delete x;
     }
-    for (auto e : events_) (void)hipEventDestroy(e);
     for (auto s : streams_) (void)hipStreamDestroy(s);
 }
 
@@ -82,72 +77,56 @@ void Batch::set_fields(int b, const double *nd, const double *mask)
 
 void Batch::step_box(Box &x)
 {
+    if (x.exec) {
+        KF_HIP(hipGraphLaunch(x.exec, x.s));
+        return;
+    }
     if (x.k32) x.k32->step(x.F);
     else x.k64->step(x.F);
 }
 
 void Batch::steps(int n)
 {
-    for (int i = 0; i < n; ++i) {
-        if (exec_) {
-            KF_HIP(hipGraphLaunch(exec_, streams_[0]));
-            continue;
-        }
+    for (int i = 0; i < n; ++i)
         for (Box *x : box_) step_box(*x);
-    }
 }
 
 void Batch::capture(int poisson_iterations)
 {
-    if (exec_) return;
     sync();
     // a graph keeps the extrapolation order and the buffers it was recorded
     // with: the histories must be full (all lazy allocations done) first
     for (Box *x : box_) {
         const bool ready = x->r32 ? x->r32->history_ready() : x->r64->history_ready();
-        if (!ready)
+        if (!x->exec && !ready)
             throw Error("batch capture needs " + std::to_string(Rhs<double>::history_steps()) +
                         " eager steps first (the potential histories are not full)");
     }
     for (Box *x : box_) {
+        if (x->exec) continue;
         x->ps->set_graph_iterations(poisson_iterations);
         // a captured solve runs a fixed number of Newton iterations, so
         // extra sweeps per iteration save nothing: two pre and two post
         x->ps->set_smoothing(2, 2, 24);
-    }
-    const int ns = (int)streams_.size();
-    hipStream_t origin = streams_[0];
-    // join every other stream back into the origin (also on failure, so
-    // that the capture can end and no stream stays in capture mode)
-    auto join = [&]() {
-        for (int k = 1; k < ns; ++k) {
-            KF_HIP(hipEventRecord(events_[k], streams_[k]));
-            KF_HIP(hipStreamWaitEvent(origin, events_[k], 0));
-        }
-    };
-    hipGraph_t graph = nullptr;
-    KF_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
-    try {
-        KF_HIP(hipEventRecord(events_[0], origin));
-        for (int k = 1; k < ns; ++k) KF_HIP(hipStreamWaitEvent(streams_[k], events_[0], 0));
-        for (Box *x : box_) step_box(*x);
-        join();
-    } catch (...) {
+        hipGraph_t graph = nullptr;
+        KF_HIP(hipStreamBeginCapture(x->s, hipStreamCaptureModeThreadLocal));
         try {
-            join();
+            if (x->k32) x->k32->step(x->F);
+            else x->k64->step(x->F);
         } catch (...) {
+            // leave the stream out of capture mode whatever failed
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(x->s, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
         }
-        hipGraph_t partial = nullptr;
-        (void)hipStreamEndCapture(origin, &partial);
-        if (partial) (void)hipGraphDestroy(partial);
-        throw;
-    }
-    KF_HIP(hipStreamEndCapture(origin, &graph));
-    hipError_t e = hipGraphInstantiate(&exec_, graph, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(graph);
-    if (e != hipSuccess) {
-        exec_ = nullptr;
-        KF_HIP(e);
+        KF_HIP(hipStreamEndCapture(x->s, &graph));
+        hipError_t e = hipGraphInstantiate(&x->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            x->exec = nullptr;
+            KF_HIP(e);
+        }
     }
     // the capture only recorded the step: nothing advanced
     captured_ = true;
