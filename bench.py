@@ -46,6 +46,13 @@ def parse():
     p.add_argument("--config1-steps", type=int, default=20)
     p.add_argument("--only", choices=("adjoint", "config1", "batch"), default=None,
                    help="run only this leg and print its JSON (for profiling)")
+    p.add_argument("--summarize-trace", nargs=3, metavar=("DIR", "TITLE", "COMMAND"),
+                   help="print the text summary of a rocprofv3 --kernel-trace --stats csv run in DIR and exit")
+    p.add_argument("--step-marker", default=None, help="kernel name fragment launched --per times per step")
+    p.add_argument("--per", type=int, default=1)
+    p.add_argument("--last", type=int, default=None, help="steps at the end of the trace to average over")
+    p.add_argument("--summarize-pmc", nargs=6, metavar=("FETCH", "WRITE", "CF", "CW", "TITLE", "COMMANDS"),
+                   help="print the HBM traffic summary of four --pmc runs (COMMANDS separated by ';') and exit")
     p.add_argument("--smoothing", type=str, default="",
                    help="Poisson smoothing sweeps 'pre,post,coarse' (library default when empty)")
     p.add_argument("--long", type=int, default=0,
@@ -425,6 +432,96 @@ def hbm_traffic(kernel, functor, grid=None):
     return round(tot / cnt * 1e6), "profiles/hbm_traffic_256_mixed.txt (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
 
 
+def _csv_rows(d, suffix):
+    import csv
+    import glob
+    paths = sorted(glob.glob(os.path.join(d, "**", "*" + suffix), recursive=True))
+    if not paths:
+        raise SystemExit(f"no *{suffix} under {d}")
+    rows = []
+    for pth in paths:
+        with open(pth) as f:
+            rows.extend(csv.DictReader(f))
+    return rows
+
+
+def summarize_trace(d, title, command, marker=None, per=1, last=None, top=30):
+    """Text summary of a `rocprofv3 --kernel-trace --stats --output-format
+    csv -d D` run: the tool's per-kernel statistics (top entries), and, with
+    a step marker (a kernel name fragment launched `per` times per step), the
+    per-step wall time, kernel time, launches and idle time over the last
+    `last` marked steps of the trace."""
+    stats = _csv_rows(d, "kernel_stats.csv")
+    tot_ns = sum(float(r["TotalDurationNs"]) for r in stats)
+    ndisp = sum(int(r["Calls"]) for r in stats)
+    out = [title, "", "Command (from the repo root):", "  " + command, "",
+           f"total kernel time {tot_ns / 1e6:.2f} ms over {ndisp} dispatches", ""]
+    if marker:
+        tr = sorted(_csv_rows(d, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
+        ends = [i for i, r in enumerate(tr) if marker in r["Kernel_Name"]][per - 1::per]
+        nst = len(ends)
+        k = min(last or nst - 1, nst - 1)
+        if k >= 1:
+            sel = tr[ends[nst - 1 - k] + 1:ends[nst - 1] + 1]
+            t0, t1 = int(sel[0]["Start_Timestamp"]), int(sel[-1]["End_Timestamp"])
+            busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sel)
+            out += [f"steps in the trace: {nst} (marker '{marker}', {per} per step); the last {k}:",
+                    f"  wall {(t1 - t0) / 1e6 / k:.3f} ms/step, kernels {busy / 1e6 / k:.3f} ms/step, "
+                    f"idle {(t1 - t0 - busy) / 1e6 / k:.3f} ms/step, {len(sel) / k:.1f} launches/step", ""]
+    out.append(f"{'%':>6} {'total ms':>9} {'calls':>6} {'avg us':>9}  kernel")
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
+        name = r["Name"].replace("void ", "").replace("kf::fft::", "").replace("kf::", "")
+        out.append(f"{100 * float(r['TotalDurationNs']) / tot_ns:6.2f} {float(r['TotalDurationNs']) / 1e6:9.3f} "
+                   f"{int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:9.2f}  {name[:110]}")
+    return "\n".join(out) + "\n"
+
+
+def summarize_pmc(fetch_dir, write_dir, cf_dir, cw_dir, title, commands, min_mb=20.0):
+    """HBM bytes per dispatch from two counter runs (FETCH_SIZE, WRITE_SIZE,
+    KiB per dispatch), each scaled by 2^30 bytes over the same counter on the
+    library's 1 GiB copy kernel (k_copy16); one line per kernel and grid
+    size, in the format hbm_traffic() reads."""
+    import collections
+
+    def per_kernel(d, counter):
+        acc = collections.defaultdict(lambda: [0.0, 0])
+        for r in _csv_rows(d, "counter_collection.csv"):
+            if r.get("Counter_Name") != counter:
+                continue
+            grid = r.get("Grid_Size") or r.get("Grid_Size_X")
+            a = acc[(r["Kernel_Name"], grid)]
+            a[0] += float(r["Counter_Value"])
+            a[1] += 1
+        return acc
+
+    def calib(d, counter):
+        vals = [v[0] / v[1] for (k, _), v in per_kernel(d, counter).items() if "k_copy16" in k]
+        if not vals:
+            raise SystemExit(f"no k_copy16 dispatch in {d}")
+        return vals[0]
+
+    cf, cw = calib(cf_dir, "FETCH_SIZE"), calib(cw_dir, "WRITE_SIZE")
+    ff, fw = (1 << 30) / (cf * 1024), (1 << 30) / (cw * 1024)
+    fe, wr = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    out = [title, "", "Commands (from the repo root; each counter pass is its own run):"]
+    out += ["  " + c for c in commands]
+    out += ["", f"Calibration: the library's copy kernel k_copy16 over 1 GiB reads FETCH_SIZE = {cf:.0f} KiB and",
+            f"WRITE_SIZE = {cw:.0f} KiB per dispatch; factors applied below: fetch x {ff:.4f}, write x {fw:.4f}.", "",
+            f"{'kernel':<82}{'grid':>9} {'n':>4} {'fetch MB':>9} {'write MB':>9} {'total MB':>9}"]
+    lines = []
+    for key, (fsum, fn) in fe.items():
+        wsum, wn = wr.get(key, [0.0, 0])
+        fmb = fsum / fn * 1024 * ff / 1e6
+        wmb = (wsum / wn * 1024 * fw / 1e6) if wn else 0.0
+        if fmb + wmb < min_mb:
+            continue
+        name = key[0].replace("void ", "").replace("kf::fft::", "").replace("kf::", "")
+        name = name if len(name) <= 79 else name[:76] + "..."
+        lines.append((fmb + wmb, f"{name:<82}{key[1]:>9} {fn:>4} {fmb:9.1f} {wmb:9.1f} {fmb + wmb:9.1f}"))
+    out += [ln for _, ln in sorted(lines, key=lambda t: -t[0])]
+    return "\n".join(out) + "\n"
+
+
 def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
@@ -441,6 +538,14 @@ def main():
         SMOOTHING = tuple(int(v) for v in args.smoothing.split(","))
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.summarize_trace:
+        d, title, cmd = args.summarize_trace
+        sys.stdout.write(summarize_trace(d, title, cmd, args.step_marker, args.per, args.last))
+        return
+    if args.summarize_pmc:
+        f, w, cf, cw, title, cmds = args.summarize_pmc
+        sys.stdout.write(summarize_pmc(f, w, cf, cw, title, [c.strip() for c in cmds.split(";")]))
         return
     if args.long:
         import kielflow_amd as kf
