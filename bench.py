@@ -51,6 +51,7 @@ def parse():
     p.add_argument("--step-marker", default=None, help="kernel name fragment launched --per times per step")
     p.add_argument("--per", type=int, default=1)
     p.add_argument("--last", type=int, default=None, help="steps at the end of the trace to average over")
+    p.add_argument("--first", type=int, default=None, help="average over the --last steps after this marked step")
     p.add_argument("--summarize-pmc", nargs=6, metavar=("FETCH", "WRITE", "CF", "CW", "TITLE", "COMMANDS"),
                    help="print the HBM traffic summary of four --pmc runs (COMMANDS separated by ';') and exit")
     p.add_argument("--smoothing", type=str, default="",
@@ -445,34 +446,52 @@ def _csv_rows(d, suffix):
     return rows
 
 
-def summarize_trace(d, title, command, marker=None, per=1, last=None, top=30):
+def summarize_trace(d, title, command, marker=None, per=1, last=None, top=30, first=None):
     """Text summary of a `rocprofv3 --kernel-trace --stats --output-format
     csv -d D` run: the tool's per-kernel statistics (top entries), and, with
     a step marker (a kernel name fragment launched `per` times per step), the
-    per-step wall time, kernel time, launches and idle time over the last
-    `last` marked steps of the trace."""
+    per-step wall time, kernel time, launches and idle time over `last`
+    marked steps: the last ones of the trace, or those after step `first`."""
     stats = _csv_rows(d, "kernel_stats.csv")
     tot_ns = sum(float(r["TotalDurationNs"]) for r in stats)
     ndisp = sum(int(r["Calls"]) for r in stats)
     out = [title, "", "Command (from the repo root):", "  " + command, "",
            f"total kernel time {tot_ns / 1e6:.2f} ms over {ndisp} dispatches", ""]
+    k = 0
     if marker:
         tr = sorted(_csv_rows(d, "kernel_trace.csv"), key=lambda r: int(r["Start_Timestamp"]))
         ends = [i for i, r in enumerate(tr) if marker in r["Kernel_Name"]][per - 1::per]
         nst = len(ends)
         k = min(last or nst - 1, nst - 1)
+        lo = nst - 1 - k if first is None else min(first, nst - 1 - k)
         if k >= 1:
-            sel = tr[ends[nst - 1 - k] + 1:ends[nst - 1] + 1]
+            sel = tr[ends[lo] + 1:ends[lo + k] + 1]
             t0, t1 = int(sel[0]["Start_Timestamp"]), int(sel[-1]["End_Timestamp"])
             busy = sum(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]) for r in sel)
-            out += [f"steps in the trace: {nst} (marker '{marker}', {per} per step); the last {k}:",
+            which = f"steps {lo + 2}..{lo + k + 1}" if first is not None else f"the last {k}"
+            out += [f"steps in the trace: {nst} (marker '{marker}', {per} per step); {which}:",
                     f"  wall {(t1 - t0) / 1e6 / k:.3f} ms/step, kernels {busy / 1e6 / k:.3f} ms/step, "
                     f"idle {(t1 - t0 - busy) / 1e6 / k:.3f} ms/step, {len(sel) / k:.1f} launches/step", ""]
+    def short(n):
+        return n.replace("void ", "").replace("kf::fft::", "").replace("kf::", "")[:110]
+
+    if marker and k >= 1:
+        import collections
+        agg = collections.defaultdict(lambda: [0, 0])
+        for r in sel:
+            a = agg[r["Kernel_Name"]]
+            a[0] += int(r["End_Timestamp"]) - int(r["Start_Timestamp"])
+            a[1] += 1
+        tot = sum(v[0] for v in agg.values())
+        out.append(f"per step over those {k} steps:")
+        out.append(f"{'%':>6} {'ms/step':>9} {'calls/step':>10} {'avg us':>9}  kernel")
+        for name, (ns, c) in sorted(agg.items(), key=lambda kv: -kv[1][0])[:top]:
+            out.append(f"{100 * ns / tot:6.2f} {ns / 1e6 / k:9.3f} {c / k:10.1f} {ns / c / 1e3:9.2f}  {short(name)}")
+        return "\n".join(out) + "\n"
     out.append(f"{'%':>6} {'total ms':>9} {'calls':>6} {'avg us':>9}  kernel")
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"]))[:top]:
-        name = r["Name"].replace("void ", "").replace("kf::fft::", "").replace("kf::", "")
         out.append(f"{100 * float(r['TotalDurationNs']) / tot_ns:6.2f} {float(r['TotalDurationNs']) / 1e6:9.3f} "
-                   f"{int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:9.2f}  {name[:110]}")
+                   f"{int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:9.2f}  {short(r['Name'])}")
     return "\n".join(out) + "\n"
 
 
@@ -541,7 +560,7 @@ def main():
         return
     if args.summarize_trace:
         d, title, cmd = args.summarize_trace
-        sys.stdout.write(summarize_trace(d, title, cmd, args.step_marker, args.per, args.last))
+        sys.stdout.write(summarize_trace(d, title, cmd, args.step_marker, args.per, args.last, first=args.first))
         return
     if args.summarize_pmc:
         f, w, cf, cw, title, cmds = args.summarize_pmc

@@ -465,7 +465,7 @@ class Poisson:
     def set_max_iterations(self, m):
         check(self.grid.ctx.lib.kf_poisson_set_max_iterations(self.h, m))
 
-    def set_smoothing(self, pre, post, coarse=24):
+    def set_smoothing(self, pre, post, coarse=8):
         check(self.grid.ctx.lib.kf_poisson_set_smoothing(self.h, pre, post, coarse))
 
     def set_ion_boltzmann(self, theta):
