@@ -60,6 +60,8 @@ def parse():
                    help="only run the mixed and FP64 steppers side by side for this many steps, printing the "
                         "per-field rel-L2 every 100 steps, and exit")
     p.add_argument("--batch-steps", type=int, default=100)
+    p.add_argument("--batch-eager", action="store_true",
+                   help="batch leg without graphs (host-launched steps), for kernel traces")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run the plain device copy (1 GiB, 5 times) that the HBM byte counters are "
                         "calibrated against, and exit")
@@ -294,10 +296,12 @@ def config1_rate(kf, ctx, n=128, steps=20, warm=2):
             "ms_per_step": round(1e3 * t / steps, 3), "steps": steps, "last_dt": dt, "poisson_failures": fails}
 
 
-def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=4, check_boxes=4):
+def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=4, check_boxes=4,
+                eager=False):
     """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
     M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
-    precision, one captured graph per box and step over `nstreams` streams.
+    precision, one captured graph per box and step over `nstreams` streams
+    (eager: no graphs, every step launched from the host, for kernel traces).
     Error: the first box of each size against an FP64 solo run."""
     rng = np.random.default_rng(seed)
     sizes = rng.choice([32, 48, 64, 96], nbox)
@@ -322,7 +326,8 @@ def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson
         g.close()
     B.steps(warm)
     B.sync()
-    B.capture(poisson_iterations)
+    if not eager:
+        B.capture(poisson_iterations)
     ctx.sync()
     t0 = time.perf_counter()
     B.steps(steps)
@@ -354,7 +359,8 @@ def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson
     B.close()
     pts = float(sum(int(n) ** 3 for n in sizes))
     return {"workload": f"{nbox} boxes, sizes uniform in {{32,48,64,96}}^3 (seed {seed}), M in [0.5,2], "
-                        f"Q in [-0.5,-0.05], mixed precision, {steps} graph-replayed RK4 steps",
+                        f"Q in [-0.5,-0.05], mixed precision, {steps} "
+                        f"{'eager' if eager else 'graph-replayed'} RK4 steps",
             "box_steps_per_s": round(nbox * steps / t, 2), "batch_steps_per_s": round(steps / t, 3),
             "grid_point_steps_per_s": pts * steps / t, "sizes": {str(n): int((sizes == n).sum()) for n in
                                                                 (32, 48, 64, 96)},
@@ -578,7 +584,7 @@ def main():
         elif args.only == "config1":
             out = config1_rate(kf, ctx, steps=args.config1_steps)
         else:
-            out = batch_sweep(kf, ctx, steps=args.batch_steps)
+            out = batch_sweep(kf, ctx, steps=args.batch_steps, eager=args.batch_eager)
         print(json.dumps(out))
         return
     n = args.n
