@@ -30,7 +30,7 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--steps", type=int, default=100)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--n", type=int, default=256)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -389,15 +389,18 @@ def long_run(kf, n, steps, every=100):
     done = 0
     while done < steps:
         k = min(every, steps - done)
-        for r in runs.values():
+        ms = {}
+        for name, r in runs.items():
             ctx.sync()
             t0 = time.perf_counter()
             r["rk"].step(r["F"], k)
             ctx.sync()
-            r["t"] += time.perf_counter() - t0
+            dt_ = time.perf_counter() - t0
+            r["t"] += dt_
+            ms[name] = 1e3 * dt_ / k
         done += k
         a, b = runs["mixed"], runs["fp64"]
-        row = {"step": done}
+        row = {"step": done, "mixed_ms_per_step": round(ms["mixed"], 3), "fp64_ms_per_step": round(ms["fp64"], 3)}
         for i, f in enumerate(("FUx", "FUy", "FUz", "ln_n")):
             row[f] = rel_l2(a["F"][i].get(), b["F"][i].get())
         row["phi"] = rel_l2(a["phi"].get(), b["phi"].get())
@@ -406,7 +409,8 @@ def long_run(kf, n, steps, every=100):
     out = {"workload": f"{n}^3 equidistant box, M=1.5, Q=-0.3, dt=1e-3, {steps} RK4 steps",
            "mixed_steps_per_s": steps / runs["mixed"]["t"], "fp64_steps_per_s": steps / runs["fp64"]["t"],
            "poisson_failures": {k: r["rhs"].poisson_failures() for k, r in runs.items()},
-           "max_rel_l2": max(max(v for kk, v in r.items() if kk != "step") for r in rows)}
+           "max_rel_l2": max(max(v for kk, v in r.items() if kk in ("FUx", "FUy", "FUz", "ln_n", "phi"))
+                             for r in rows)}
     print(json.dumps(out), flush=True)
 
 
@@ -626,24 +630,29 @@ def main():
     nc = n * n * (n // 2 + 1)
     C, Rb = 8 * nc, 4 * n ** 3  # bytes of one FP32 complex / real field
     # The dominant kernel family: the z passes of the inverse transforms
-    # (half spectrum -> real rows, FP32), timed together.  One RHS evaluation
-    # runs 8 of them; with their fused loads and stores they move 24 C + 28 R
-    # bytes:
-    #   rho: C + n_d (FP64) in, ln n + rho (FP64) out              = C + 5R
+    # (half spectrum -> real rows), timed together.  One RHS evaluation runs
+    # 8 of them; with their fused loads and stores they move 25 C + 28 R
+    # bytes (C, R: one FP32 complex / real field; an FP64 field is twice):
+    #   rho (FP64 transform of the FP64 stage input): 2C + n_d (2R) in,
+    #   ln n (R) and rho (2R) out                                  = 2C + 5R
     #   velocities: C in, U_b out, 3 times                         = 3 (C + R)
     #   per field (k_zpen): the three staged derivative rows and the
     #   linear terms' rows in (4C), U_0..2, f (U_a or ln n) and the
     #   mask in (5R), the blend's forward z transform out (C), 4 times
     #                                                              = 4 (5C + 5R)
-    roof = roofline(kf.api.PROF_FFT_ZC2R, (24 * C + 28 * Rb) / 8,
+    roof = roofline(kf.api.PROF_FFT_ZC2R, (25 * C + 28 * Rb) / 8,
                     "FFT z passes of the inverse transforms (k_zc2r, k_zpen: products, penalization blend and "
-                    "forward z transform fused; FP32, Nz = 256)",
-                    (("k_zc2r<float, 128", "k_zpen<float, 128"), None))
-    # the FFT y passes (strided, FP32, length 256).  Per RHS evaluation: 4
-    # that read a staged field once and write it and its i k_y multiple (3 C),
-    # and 13 that read and write one complex field (2 C): 38 C over 17
-    roof_y = roofline(kf.api.PROF_FFT_Y, 38 * C / 17, "FFT y passes (k_strided, k_strided_ik; FP32, length 256)",
-                      (("k_strided<float, 256", "k_strided_ik<float, 256"), "LoadMasked"))
+                    "forward z transform fused; FP32, and the FP64 one of rho; Nz = 256)",
+                    (("k_zc2r<float, 128", "k_zpen<float, 128",
+                      "k_zc2r<double, 128, 16, RowLoadC<double>, ZStoreK<double, 6>"), None))
+    # the FFT y passes (strided, length 256).  Per RHS evaluation: 4 FP32
+    # ones that read a staged field once and write it and its i k_y multiple
+    # (3 C), 13 FP32 ones that read and write one complex field (2 C), and
+    # the FP64 inverse pass of rho's transform (4 C): 42 C over 18
+    roof_y = roofline(kf.api.PROF_FFT_Y, 42 * C / 18, "FFT y passes (k_strided, k_strided_ik; FP32, and the FP64 "
+                      "one of rho; length 256)",
+                      (("k_strided<float, 256", "k_strided_ik<float, 256",
+                        "k_strided<double, 256, 8, 1, LoadMasked<double>"), "LoadMasked"))
     # the finest-level red-black half sweep of the Poisson multigrid: the
     # other colour's x, this colour's b and e in, this colour's x out,
     # 4 * (N/2) * 4 bytes (FP32)

@@ -170,7 +170,7 @@ int kf_poisson_destroy(kf_poisson p);
 int kf_poisson_set_tolerance(kf_poisson p, double tol);
 int kf_poisson_set_max_iterations(kf_poisson p, int iters);
 /* multigrid smoothing of the Newton-MG solver: pre/post red-black sweeps per
- * level and sweeps on the coarsest level (defaults 2, 2, 8 in FP64, 3, 3,
+ * level and sweeps on the coarsest level (defaults 2, 2, 8 in FP64, 2, 3,
  * 8 in mixed precision) */
 int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse);
 /* solver_poisson_multigrid::set_level_control (solver_poisson_multigrid.cpp:

@@ -43,6 +43,7 @@ struct RkEpilogue {
     double2 *acc[4];
     double c;
     int final_, first, next;
+    double2 *s64n;  // the next stage input of ln n in FP64 (or null)
 };
 
 // RHS in compute precision R (double: FP64 mode; float: mixed mode).
@@ -86,6 +87,13 @@ class Rhs {
     // the extrapolation (a captured step keeps the order it was recorded with)
     bool history_ready() const;
     static constexpr int history_steps() { return 5; }
+    // Mixed precision: the FP64 spectrum of ln n that the next evaluation's
+    // rho = exp(ln n) + n_d is formed from (an FP64 inverse transform), in
+    // place of its FP32 stage input (null: off).  The FP32 stage input's
+    // rounding otherwise reaches rho and, through the potential history, the
+    // Newton residual of later solves.  Set per stage by the RK4 stepper.
+    void set_rho_source(const double2 *Fn64) { rho_src_ = Fn64; }
+    bool wants_rho_source() const { return plan64_ != nullptr; }
     // keep the converged potential of RK stage s in p4[s] after each solve
     // (null: off); phi_cur_ is that potential, in phi_ or a history slot
     void set_phi_sink(double *const *p4) { sink_ = p4; }
@@ -168,6 +176,8 @@ class Rhs {
     int extrap_order_ = 4;   // highest order of the extrapolation in time
     double *const *sink_ = nullptr;
     const double *const *src_ = nullptr;
+    const double2 *rho_src_ = nullptr;
+    fft::Plan<double> *plan64_ = nullptr;  // mixed, with rho in FP64: the FP64 transform of rho_src_
 };
 
 // RK4 with an FP64 state and accumulator; stages evaluated in precision R.
@@ -191,6 +201,7 @@ class Rk4 {
     double dt_, t_ = 0.;
     C2<R> *S_[4] = {};
     double2 *acc_[4] = {};
+    double2 *S64n_ = nullptr;  // FP64 stage input of ln n, where the RHS takes it
 };
 
 // euler_method::solve (euler_method.cpp:13-91): y = y + dt k with an FP64

@@ -171,6 +171,12 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
             rho64_ = ralloc<double>(nr);
             nd64_ = ralloc<double>(nr);
             KF_HIP(hipMemsetAsync(nd64_, 0, sizeof(double) * nr, s));
+            // KF_RHO_STATE=0: rho from the FP32 stage input even under RK4
+            const char *e2 = std::getenv("KF_RHO_STATE");
+            if (!(e2 && std::atoi(e2) == 0)) {
+                plan64_ = new fft::Plan<double>();
+                plan_init<double>(*plan64_, g.nx, g.ny, g.nz, s);
+            }
         }
     }
     if (!g.any_stretched())
@@ -185,6 +191,10 @@ template <typename R>
 Rhs<R>::~Rhs()
 {
     plan_free<R>(plan_);
+    if (plan64_) {
+        plan_free<double>(*plan64_);
+        delete plan64_;
+    }
     (void)hipFree(nd_);
     (void)hipFree(mask_);
     for (int b = 0; b < 3; ++b) (void)hipFree(U_[b]);
@@ -287,14 +297,32 @@ void Rhs<R>::poisson_step(const C2<R> *Fn, const C2<R> *staged)
 template <typename R>
 void Rhs<R>::poisson_begin(const C2<R> *Fn, const C2<R> *staged)
 {
-    ZStore<R> z = zs<R>(rho64_on_ ? ZK_RHO64 : ZK_RHO, plan_);
-    z.o0 = L_;
-    z.o1 = rho_;
-    z.a0 = nd_;
-    z.od = rho64_;
-    z.ad = nd64_;
-    if (staged) z_staged<R>(g_, plan_, staged, false, z, s_);
-    else c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
+    if constexpr (!std::is_same<R, double>::value) {
+        if (rho_src_ && plan64_) {
+            // ln n and rho from the FP64 stage input (rhs_standard.cpp:100-108:
+            // D23, inverse transform, exp, + n_d), ln n then rounded to FP32
+            fft::Plan<double> &p = *plan64_;
+            ZStore<double> z = zs<double>(ZK_RHOF, p);
+            z.of = (float *)L_;
+            z.od = rho64_;
+            z.ad = nd64_;
+            c2r_xy<double>(g_, p, rho_src_, 1, s_);
+            launch_zc2r<double>(p.nz / 2, (long)p.nx * p.ny, p.tw_m, p.tw_n, RowLoadC<double>{p.scratch, p.nzs},
+                                ZStoreK<double, ZK_RHOF>{z}, s_);
+            staged = nullptr;
+            Fn = nullptr;
+        }
+    }
+    if (Fn || staged) {
+        ZStore<R> z = zs<R>(rho64_on_ ? ZK_RHO64 : ZK_RHO, plan_);
+        z.o0 = L_;
+        z.o1 = rho_;
+        z.a0 = nd_;
+        z.od = rho64_;
+        z.ad = nd64_;
+        if (staged) z_staged<R>(g_, plan_, staged, false, z, s_);
+        else c2r_spec<R>(g_, plan_, Fn, 1, -1, z, s_);
+    }
     if (!phi_) throw Error("RHS potential field not set");
     const long nr = g_.nr;
     phi_cur_ = phi_;
@@ -447,6 +475,7 @@ void Rhs<R>::eval_fused(C2<R> *F[4])
             xo.rc = rk_->c;
             xo.rfirst = rk_->first;
             xo.rnext = rk_->next;
+            xo.r64 = (a == 3 && !rk_->final_) ? rk_->s64n : nullptr;
         }
         r2c_yx<R>(g_, plan_, xo, s_);
     }
@@ -581,9 +610,10 @@ __global__ void k_rk_init(Quad<R> q, long n)
     q.S[f][i] = mk<C2<R>>((R)v.x, (R)v.y);
 }
 
-// after stage s: acc = (s==1 ? 1*k : acc + 2*k); S = y0 + c*k
+// after stage s: acc = (s==1 ? 1*k : acc + 2*k); S = y0 + c*k (and, where
+// s64n is set, field 3's S also in FP64)
 template <typename R>
-__global__ void k_rk_stage(Quad<R> q, long n, double c, int first)
+__global__ void k_rk_stage(Quad<R> q, long n, double c, int first, double2 *s64n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int f = blockIdx.y;
@@ -601,7 +631,9 @@ __global__ void k_rk_stage(Quad<R> q, long n, double c, int first)
     }
     q.acc[f][i] = a;
     const double2 y0 = q.y[f][i];
-    q.S[f][i] = mk<C2<R>>((R)(y0.x + c * kx), (R)(y0.y + c * ky));
+    const double sx = y0.x + c * kx, sy = y0.y + c * ky;
+    q.S[f][i] = mk<C2<R>>((R)sx, (R)sy);
+    if (s64n && f == 3) s64n[i] = make_double2(sx, sy);
 }
 
 // y = y0 + (dt/6) (acc + 1*k4)   (RKO4.cpp:225-238); with next, also the
@@ -634,6 +666,7 @@ Rk4<R>::Rk4(Rhs<R> &rhs, double dt) : rhs_(rhs), dt_(dt)
         S_[f] = ralloc<C2<R>>(nc);
         acc_[f] = ralloc<double2>(nc);
     }
+    if (rhs.wants_rho_source()) S64n_ = ralloc<double2>(nc);
 }
 
 template <typename R>
@@ -643,6 +676,7 @@ Rk4<R>::~Rk4()
         (void)hipFree(S_[f]);
         (void)hipFree(acc_[f]);
     }
+    (void)hipFree(S64n_);
 }
 
 // RKO4.cpp:50-248 with an FP64 running sum k1 + 2 k2 + 2 k3
@@ -674,6 +708,7 @@ void Rk4<R>::steps(double2 *y[4], int n)
         e.y[f] = y[f];
         e.acc[f] = acc_[f];
     }
+    e.s64n = S64n_;
     for (int it = 0; it < n; ++it) {
         if (it == 0) hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
         const double tt[4] = {t_, t_ + 0.5 * dt_, t_ + 0.5 * dt_, t_ + 1.0 * dt_};
@@ -684,12 +719,15 @@ void Rk4<R>::steps(double2 *y[4], int n)
             e.first = st == 0;
             e.next = it + 1 < n;
             rhs_.set_rk_epilogue(&e);
+            // the first stage's input is the FP64 state itself
+            if (S64n_) rhs_.set_rho_source(st == 0 ? (const double2 *)y[3] : (const double2 *)S64n_);
             rhs_.eval(tt[st], S_);
+            rhs_.set_rho_source(nullptr);
             const bool fused = rhs_.rk_epilogue_done();
             rhs_.set_rk_epilogue(nullptr);
             if (fused) continue;
             if (st < 3)
-                hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0);
+                hipLaunchKernelGGL(k_rk_stage<R>, grid, dim3(256), 0, s, q, nc, c[st], st == 0 ? 1 : 0, S64n_);
             else
                 hipLaunchKernelGGL(k_rk_final<R>, grid, dim3(256), 0, s, q, nc, dt_ / 6., it + 1 < n ? 1 : 0);
         }

@@ -9,7 +9,7 @@ namespace kf {
 
 using namespace fft;
 
-enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN, ZK_RHO64 };
+enum { ZK_WRITE = 0, ZK_PROF, ZK_RHO, ZK_PROD, ZK_PEN, ZK_RHO64, ZK_RHOF };
 enum { XK_WRITE = 0, XK_ACC, XK_NEG_D23, XK_NEG_ACC, XK_F1, XK_LIN, XK_ADJ, XK_RK_STAGE, XK_RK_FINAL };
 
 // ------------------------------------------------------------------ spectral factors
@@ -138,8 +138,9 @@ struct ZStore {
     int flag;
     const R *prof;
     int pdir;
-    double *od;         // ZK_RHO64: rho in FP64
-    const double *ad;   // ZK_RHO64: n_d in FP64
+    double *od;         // ZK_RHO64, ZK_RHOF: rho in FP64
+    const double *ad;   // ZK_RHO64, ZK_RHOF: n_d in FP64
+    float *of;          // ZK_RHOF: ln n rounded to FP32
     __device__ __forceinline__ void operator()(long row, int n, C2<R> v) const { apply<-1>(row, n, v); }
     // K >= 0: the kind fixed at compile time (ZStoreK), so only its code is built
     template <int K>
@@ -176,6 +177,15 @@ struct ZStore {
             // FP64 Newton residual sees no rounding of rho ~ 1 to R
             const double2 d = *reinterpret_cast<const double2 *>(ad + base);
             *reinterpret_cast<C *>(o0 + base) = mk<C>(x0, x1);
+            *reinterpret_cast<double2 *>(od + base) = make_double2(exp((double)x0) + d.x, exp((double)x1) + d.y);
+            break;
+        }
+        case ZK_RHOF: {
+            // an FP64 transform of the FP64 stage input: ln n to FP32 for the
+            // rest of the mixed RHS, rho = exp(ln n) + n_d in FP64 from the
+            // unrounded ln n
+            const double2 d = *reinterpret_cast<const double2 *>(ad + base);
+            *reinterpret_cast<float2 *>(of + base) = make_float2((float)x0, (float)x1);
             *reinterpret_cast<double2 *>(od + base) = make_double2(exp((double)x0) + d.x, exp((double)x1) + d.y);
             break;
         }
@@ -224,6 +234,9 @@ struct XStore {
     double2 *ry, *racc;
     double rc;
     int rfirst, rnext;
+    // XK_RK_STAGE: the next stage input also in FP64 (before its rounding
+    // to R), where set
+    double2 *r64;
     // linear terms of velocity component a (rhs_standard.cpp:119-140,
     // 219-247, 279-310) from its spectrum u; adds i k_a [D23] u to ln
     __device__ __forceinline__ C2<R> lin_u(C2<R> u, R fac, C2<R> F1, bool kp, R kx, R ka, C2<R> &ln) const
@@ -289,7 +302,9 @@ struct XStore {
             }
             racc[idx] = a;
             const double2 y0 = ry[idx];
-            o0[idx] = mk<C>((R)(y0.x + rc * kx), (R)(y0.y + rc * ky));
+            const double sx = y0.x + rc * kx, sy = y0.y + rc * ky;
+            o0[idx] = mk<C>((R)sx, (R)sy);
+            if (r64) r64[idx] = make_double2(sx, sy);
             break;
         }
         case XK_RK_FINAL: {
