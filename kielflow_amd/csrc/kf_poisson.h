@@ -255,7 +255,7 @@ class Poisson {
     int kiters_ = 2;
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)
     double *d_scal_ = nullptr;    // device scalars of BiCGStab
-    hipEvent_t ev_[2] = {};       // convergence-flag copies of the look-ahead loop
+    hipEvent_t ev_[2] = {};       // behind the convergence-flag copies of the last two checks
     double *lsx_ = nullptr, *lsr_ = nullptr, *lsb_ = nullptr;  // linear solve work
     double *lsT_ = nullptr;  // right-hand side of the transposed solve
     int lin_it_ = 0;
