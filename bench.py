@@ -99,12 +99,15 @@ def run_precision(kf, ctx, g, axes, prec, nd_d, mask_d, Fn0, steps, warmup, prof
     rk = kf.Rk4(rhs, 1e-3)
     rk.step(F, warmup)
     ctx.sync()
+    n0, c0 = ps.newton_counts()
     t0 = time.perf_counter()
     rk.step(F, steps)
     ctx.sync()
     t1 = time.perf_counter()
+    n1, c1 = ps.newton_counts()
     out = {"steps_per_s": steps / (t1 - t0), "ms_per_step": 1e3 * (t1 - t0) / steps,
-           "poisson_failures": rhs.poisson_failures()}
+           "poisson_failures": rhs.poisson_failures(),
+           "newton_corrections_per_solve": round((c1 - c0) / max(n1 - n0, 1), 3)}
     # state after warmup + timed steps (compared across precisions)
     out["state"] = [f.get() for f in F]
     out["phi"] = phi.get()
@@ -667,6 +670,8 @@ def main():
         extra.update({"fp64_steps_per_s": round(fp64["steps_per_s"], 3),
                       "fp64_ms_per_step": round(fp64["ms_per_step"], 3),
                       "mixed_over_fp64": round(mixed["steps_per_s"] / fp64["steps_per_s"], 3),
+                      "newton_corrections_per_solve": {"mixed": mixed["newton_corrections_per_solve"],
+                                                       "fp64": fp64["newton_corrections_per_solve"]},
                       "rel_l2_mixed_vs_fp64_after_steps": errs,
                       "poisson_failures_fp64": fp64["poisson_failures"]})
 

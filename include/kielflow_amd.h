@@ -182,6 +182,11 @@ int kf_poisson_set_smoothing(kf_poisson p, int pre, int post, int coarse);
  * on it.  Default: kielflow.cpp:207-217's {-3, 1, 1, 1}, {0, -1, -1, -1},
  * {1e-2, 5e-4, 1e-4, 1e-4}. */
 int kf_poisson_set_cascade(kf_poisson p, int n, const int *level_change, const int *steps, const double *tol);
+/* Newton-MG solves this solver has finished under the host's loop since its
+ * creation (those of kf_rhs_solve / kf_rk4_steps included, captured ones
+ * not), and the Newton corrections they took: a diagnostic with no
+ * counterpart in the reference, whose NLJ reports nothing. */
+int kf_poisson_newton_counts(kf_poisson p, long *solves, long *corrections);
 /* Boltzmann ions in the equation, as the reference frontend builds the
  * solver (solver_poisson_jacobi_nlin.cpp:192-195, __FRONTEND__):
  * L_h phi = exp(phi) - exp(-theta phi) - rho; theta = 0 turns them off.

@@ -103,6 +103,7 @@ SIGNATURES = {
     "kf_rk4_time": (C.c_int, [_vp, _dp]),
     "kf_cfl_all": (C.c_int, [_vp, C.c_double, _vp, _vp, _vp, _dp]),
     "kf_field_integrate": (C.c_int, [_vp, _vp, _dp]),
+    "kf_poisson_newton_counts": (C.c_int, [_vp, C.POINTER(C.c_long), C.POINTER(C.c_long)]),
     "kf_axis_index_at": (C.c_int, [C.POINTER(Axis), C.c_double, _ip]),
     "kf_state_to_real": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "kf_state_plane_z": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int, _vp]),
@@ -464,6 +465,12 @@ class Poisson:
 
     def set_max_iterations(self, m):
         check(self.grid.ctx.lib.kf_poisson_set_max_iterations(self.h, m))
+
+    def newton_counts(self):
+        """(solves, Newton corrections) finished under the host loop so far."""
+        a, b = C.c_long(0), C.c_long(0)
+        check(self.grid.ctx.lib.kf_poisson_newton_counts(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def set_smoothing(self, pre, post, coarse=8):
         check(self.grid.ctx.lib.kf_poisson_set_smoothing(self.h, pre, post, coarse))

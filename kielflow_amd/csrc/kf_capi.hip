@@ -798,6 +798,16 @@ int kf_rhs_poisson_failures(kf_rhs r, int *count)
     });
 }
 
+int kf_poisson_newton_counts(kf_poisson p, long *solves, long *corrections)
+{
+    return guard([&] {
+        KF_REQUIRE(p && solves && corrections, "null argument");
+        *solves = p->p->solves_counted();
+        *corrections = p->p->corrections_counted();
+        return KF_OK;
+    });
+}
+
 int kf_rhs_force_general(kf_rhs r, int on)
 {
     return guard([&] {

@@ -140,6 +140,10 @@ class Poisson {
     // same right side: enqueue no correction ahead of its first check (which
     // ends it), and keep the expected count of the other solves
     void next_guess_converged() { known_next_ = true; }
+    // Newton-MG solves finished with the host loop since creation, and the
+    // corrections they took (captured solves are not counted)
+    long solves_counted() const { return n_solves_; }
+    long corrections_counted() const { return n_corr_; }
     // solve() in two halves, so that work that does not read phi can be
     // enqueued between them (Newton-MG; other modes solve in solve_begin):
     // solve_begin enqueues the corrections the last solve needed, solve_end
@@ -250,6 +254,7 @@ class Poisson {
     double *alt64_ = nullptr;     // second potential buffer of the fused Newton update
     int expect_ = 1;              // corrections the last solve needed (host sync point of the next)
     bool known_next_ = false;     // the next solve's guess is a converged potential of the same rho
+    long n_solves_ = 0, n_corr_ = 0;
     double ion_theta_ = 0.;       // Boltzmann-ion theta (0: none)
     bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
     int kiters_ = 2;

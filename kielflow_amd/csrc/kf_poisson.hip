@@ -1791,6 +1791,8 @@ void Poisson::newton_end()
         }
         // ctl[1]: the iteration of the check that ended the solve
         if (!r.known) expect_ = std::max(1, h_ctl_[4 * (r.it & 1) + 1]);
+        n_solves_ += 1;
+        n_corr_ += h_ctl_[4 * (r.it & 1) + 1];
     }
     // result in phi: in place (A = phi) copy when the newest iterate is in B
     // (ctl[3] == 1); from a guess (B = phi) copy when it is in A (ctl[3] == 0)
