@@ -299,7 +299,7 @@ def config1_rate(kf, ctx, n=128, steps=20, warm=2):
             "ms_per_step": round(1e3 * t / steps, 3), "steps": steps, "last_dt": dt, "poisson_failures": fails}
 
 
-def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=4, check_boxes=4,
+def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=3, check_boxes=4,
                 eager=False):
     """Parameter sweep of independent boxes: sizes uniform in {32, 48, 64, 96}^3,
     M uniform in [0.5, 2], Q uniform in [-0.5, -0.05], 1e-3 noise, mixed
