@@ -780,7 +780,10 @@ void Euler<R>::steps(double2 *y[4], int n)
     for (int it = 0; it < n; ++it) {
         hipLaunchKernelGGL(k_rk_init<R>, grid, dim3(256), 0, s, q, nc);
         rhs_.set_stage(0);
+        // the stage input is the FP64 state itself
+        if (rhs_.wants_rho_source()) rhs_.set_rho_source((const double2 *)y[3]);
         rhs_.eval(t_, S_);
+        rhs_.set_rho_source(nullptr);
         rhs_.set_stage(-1);
         hipLaunchKernelGGL(k_euler<R>, grid, dim3(256), 0, s, q, nc, dt_);
         KF_HIP(hipGetLastError());
