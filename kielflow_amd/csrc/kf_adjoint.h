@@ -82,6 +82,10 @@ class Adjoint {
     std::vector<void *> slabs_;      // the pool's allocations
     size_t capture_budget_ = (size_t)64 << 30;
     bool keep_potentials_ = true;
+    // the C2R, dS_b multiply and R2C of a stretched-axis transposed
+    // derivative in one z pass (KF_PROF_UNFUSED: separate z passes, the same
+    // numbers)
+    bool prof_fused_ = true;
     int run_mode_ = RUN_RECOMPUTE;
     double *lam_[4] = {};            // transposed Poisson solutions per stage (warm start)
     bool lam_valid_ = false;

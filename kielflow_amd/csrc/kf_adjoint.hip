@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "kf_rhs_impl.h"
@@ -237,6 +238,7 @@ static T *dalloc(long n)
 Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rhs.plan()), s_(rhs.stream()), dt_(dt)
 {
     const long nr = g_.nr, nc = g_.nc;
+    prof_fused_ = std::getenv("KF_PROF_UNFUSED") == nullptr;
     for (auto &c : cap_) {
         for (auto &u : c.U) u = dalloc<double>(nr);
         c.L = dalloc<double>(nr);
@@ -326,29 +328,29 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
                            acc);
         return;
     }
-    ZStore<double> z = zs<double>(ZK_PROF, p_);
-    z.o0 = rt_;
-    z.prof = g_.prof<double>(b);
-    z.pdir = b;
     XLoadSpec<double> ld = xload<double>(Y, g_.view<double>(), 0, -1);
     ld.w = 1;
-    {
-        GridView<double> gv = g_.view<double>();
-        const StridedAddr ay = saddr_y(p_);
-        launch_strided<double, +1>(p_.nx, scratch_cols_x(p_), p_.tw_x, pad_cols(ld, p_), scratch_store_x(p_.scratch, p_),
-                                   s_, prof::FFT_X, p_.vx);
-        launch_strided<double, +1>(p_.ny, scratch_cols_y(p_), p_.tw_y, LoadMasked<double>{p_.scratch, ay, (unsigned)p_.nzc},
+    const StridedAddr ay = saddr_y(p_);
+    launch_strided<double, +1>(p_.nx, scratch_cols_x(p_), p_.tw_x, pad_cols(ld, p_), scratch_store_x(p_.scratch, p_),
+                               s_, prof::FFT_X, p_.vx);
+    launch_strided<double, +1>(p_.ny, scratch_cols_y(p_), p_.tw_y, LoadMasked<double>{p_.scratch, ay, (unsigned)p_.nzc},
                                StoreMasked<double>{p_.scratch, ay, (unsigned)p_.nzc}, s_, prof::FFT_Y, p_.vy);
-        launch_zc2r<double>(p_.nz / 2, (long)p_.nx * p_.ny, p_.tw_m, p_.tw_n, RowLoadC<double>{p_.scratch, p_.nzs}, z,
-                            s_);
-        (void)gv;
-    }
     XStore<double> x = xs<double>(XK_ADJ, g_);
     x.o0 = out;
     x.d23 = 1;
     x.dir = b;
     x.acc = acc;
     x.coef = coef * (double)g_.nr;
+    if (prof_fused_) {
+        zprof<double>(p_, g_.prof<double>(b), b, s_);
+        r2c_yx<double>(g_, p_, x, s_);
+        return;
+    }
+    ZStore<double> z = zs<double>(ZK_PROF, p_);
+    z.o0 = rt_;
+    z.prof = g_.prof<double>(b);
+    z.pdir = b;
+    launch_zc2r<double>(p_.nz / 2, (long)p_.nx * p_.ny, p_.tw_m, p_.tw_n, RowLoadC<double>{p_.scratch, p_.nzs}, z, s_);
     r2c_store<double, double>(g_, p_, rt_, x, s_);
 }
 

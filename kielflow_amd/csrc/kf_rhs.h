@@ -145,6 +145,10 @@ class Rhs {
     // environment variable KF_PEN_UNFUSED selects the separate passes, which
     // compute the same numbers)
     bool pen_fused_ = true;
+    // the metric multiply of a stretched-axis derivative fused into one z
+    // pass (KF_PROF_UNFUSED selects the separate C2R and R2C z passes, which
+    // compute the same numbers)
+    bool prof_fused_ = true;
     const RkEpilogue *rk_ = nullptr;
     bool rk_done_ = false;
     double *phi_ = nullptr;     // caller-owned

@@ -153,6 +153,7 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     // KF_EXTRAP_ORDER=1..5 caps the order of the potential extrapolation
     if (const char *env = std::getenv("KF_EXTRAP_ORDER")) extrap_order_ = std::max(1, std::min(5, std::atoi(env)));
     pen_fused_ = std::getenv("KF_PEN_UNFUSED") == nullptr;
+    prof_fused_ = std::getenv("KF_PROF_UNFUSED") == nullptr;
     const long nr = g.nr, nc = g.nc;
     nd_ = ralloc<R>(nr);
     mask_ = ralloc<R>(nr);
@@ -491,13 +492,21 @@ void Rhs<R>::deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accum
                            accumulate);
         return;
     }
+    XStore<R> x = xs<R>(accumulate ? XK_ACC : XK_WRITE, g_);
+    x.o0 = out;
+    if (prof_fused_) {
+        // inverse x and y passes, then one z pass for C2R, 1/S' and R2C,
+        // then the forward y and x passes
+        c2r_xy<R>(g_, plan_, X, 1, s_, b);
+        zprof<R>(plan_, g_.prof<R>(b), b, s_);
+        r2c_yx<R>(g_, plan_, x, s_);
+        return;
+    }
     ZStore<R> z = zs<R>(ZK_PROF, plan_);
     z.o0 = rtmp_;
     z.prof = g_.prof<R>(b);
     z.pdir = b;
     c2r_spec<R>(g_, plan_, X, 1, b, z, s_);
-    XStore<R> x = xs<R>(accumulate ? XK_ACC : XK_WRITE, g_);
-    x.o0 = out;
     r2c_store<R, R>(g_, plan_, rtmp_, x, s_);
 }
 

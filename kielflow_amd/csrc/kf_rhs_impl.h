@@ -738,6 +738,124 @@ k_zpen(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long ro
     }
 }
 
+// first-pass store of k_zprof: the real pair t of the row, times the
+// stretched axis' 1/S' exactly as the ZK_PROF z store multiplies it, kept in
+// the thread's slots
+template <typename R>
+struct ProfMul {
+    C2<R> *acc;
+    int *pos;
+    const R *prof;
+    int pdir, ny;
+    long row;
+    R scale;
+    bool live;
+    __device__ __forceinline__ void st(int t, C2<R> v, int slot) const
+    {
+        if (!live) return;
+        const R x0 = v.x * scale, x1 = v.y * scale;
+        R p0, p1;
+        if (pdir == 2) {
+            p0 = prof[2 * t];
+            p1 = prof[2 * t + 1];
+        } else {
+            const unsigned rr = (unsigned)row, yy = (unsigned)ny, qq = rr / yy;
+            const int id = (pdir == 1) ? (int)(rr - qq * yy) : (int)qq;
+            p0 = p1 = prof[id];
+        }
+        acc[slot] = mk<C2<R>>(x0 * p0, x1 * p1);
+        pos[slot] = t;
+    }
+};
+
+// The metric multiply of a derivative along a stretched axis in one z pass
+// (OP_partial_derivative.cpp:104-136): per row, the C2R of the x/y-
+// transformed spectrum (the scratch, rows of nzs), times 1/S' of the axis,
+// into LDS, and its R2C back over the row in place.  Same arithmetic as the
+// ZK_PROF z pass and k_zr2c one after another, without the real field in
+// between.
+template <typename R, int M, int RB>
+__global__ void __launch_bounds__(tpt_for(M) * RB)
+k_zprof(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, int ny, int nzs,
+        C2<R> *rowsbuf, const R *prof, int pdir, R scale)
+{
+    using C = C2<R>;
+    using Buf = RowBuf<C, M>;
+    constexpr int TPT = tpt_for(M);
+    constexpr int WG = TPT * RB;
+    constexpr int E = out_slots(M, TPT);
+    __shared__ C lds[RB * Buf::SC];
+    __shared__ C tw[M];
+    __shared__ C twN[M + 1];
+    const int tid = threadIdx.x;
+    const long r0 = (long)xcd_block(blockIdx.x, gridDim.x) * RB;
+    for (int m = tid; m < M; m += WG) tw[m] = twM_g[m];
+    for (int m = tid; m <= M; m += WG) twN[m] = twN_g[m];
+    __syncthreads();
+    const int c = tid / TPT, tt = tid % TPT;
+    const long row = r0 + c;
+    const bool live = row < rows;
+    const Buf A{lds, c};
+    C acc[E];
+    int pos[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) pos[e] = -1;
+    const RowLoadC<R> ld{rowsbuf, nzs};
+    const ProfMul<R> pm{acc, pos, prof, pdir, ny, row, scale, live};
+    if constexpr (merge_in_regs(M)) {
+        C z[M / TPT];
+        merge_regs<C, M>(ld, row, live, tt, twN, z);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z}, pm, A, A);
+    } else {
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ld, row, live, twN}, pm, A, A);
+    }
+    // the product row into LDS (pairs 2t, 2t+1 at t), then its forward
+    // Nz/2-point transform in place
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; ++e)
+        if (pos[e] >= 0) A.st(pos[e], live ? acc[e] : mk<C>((R)0, (R)0));
+    __syncthreads();
+    fft_io<C, M, TPT, -1, false, true, true>(tw, tt, A, A, A, A);
+    __syncthreads();
+    // split, as k_zr2c
+    for (int e = tid; e < RB * (M + 1); e += WG) {
+        const int cc = e / (M + 1), k = e % (M + 1);
+        const long rr = r0 + cc;
+        if (rr < rows) {
+            const Buf Z{lds, cc};
+            C zk = Z.ld(k == M ? 0 : k);
+            C zm = cconj(Z.ld(k == 0 ? 0 : M - k));
+            C ev = mk<C>((R)0.5 * (zk.x + zm.x), (R)0.5 * (zk.y + zm.y));
+            C od = mk<C>((R)0.5 * (zk.x - zm.x), (R)0.5 * (zk.y - zm.y));
+            C wod = cmul(twN[k], od);
+            rowsbuf[rr * nzs + k] = mk<C>(ev.x + wod.y, ev.y - wod.x);
+        }
+    }
+}
+
+// k_zprof over all rows of the plan's scratch
+template <typename R>
+inline void zprof(Plan<R> &p, const R *prof, int pdir, hipStream_t s)
+{
+    prof::Scope ps_(prof::FFT_ZC2R, s);
+    const long rows = (long)p.nx * p.ny;
+    const R scale = (R)(1.0 / (double)p.nr);
+    switch (p.nz / 2) {
+#define KF_CASE(NN)                                                                                            \
+    case NN: {                                                                                                 \
+        constexpr int RB = cb_for<R>(NN);                                                                      \
+        hipLaunchKernelGGL((k_zprof<R, NN, RB>), dim3((unsigned)((rows + RB - 1) / RB)), dim3(tpt_for(NN) * RB), \
+                           0, s, p.tw_m, p.tw_n, rows, p.ny, p.nzs, p.scratch, prof, pdir, scale);             \
+        break;                                                                                                 \
+    }
+        KF_FFT_LENGTHS(KF_CASE)
+#undef KF_CASE
+    default:
+        throw Error("unsupported z length " + std::to_string(p.nz));
+    }
+}
+
 // k_zpen over all rows: lin (scratch layout, after the inverse x and y
 // passes of the linear terms) is replaced by the forward z transform of the
 // blend
@@ -768,11 +886,11 @@ inline void zpen(Grid &g, Plan<R> &p, const C2<R> *Hx, const C2<R> *Hy, const C2
 // the x and y passes of an inverse transform of src (2/3 rule at the load
 // when d23) into the plan's scratch, for a z pass of the caller's own
 template <typename R>
-inline void c2r_xy(Grid &g, Plan<R> &p, const C2<R> *src, int d23, hipStream_t s)
+inline void c2r_xy(Grid &g, Plan<R> &p, const C2<R> *src, int d23, hipStream_t s, int dir = -1)
 {
     GridView<R> gv = g.view<R>();
     const StridedAddr ay = saddr_y(p);
-    with_xload<R>(xload<R>(src, gv, d23, -1), [&](const auto &ld) {
+    with_xload<R>(xload<R>(src, gv, d23, dir), [&](const auto &ld) {
         launch_strided<R, +1>(p.nx, scratch_cols_x(p), p.tw_x, pad_cols(ld, p), scratch_store_x(p.scratch, p), s,
                               prof::FFT_X, p.vx);
     });

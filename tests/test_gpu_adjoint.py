@@ -80,6 +80,47 @@ def test_rhs_adjoint_dot_product(ctx, kind, n, pen, sv):
     assert abs(dn - _ip(dnd, ndbar)) <= 1e-6 * abs(dn), (dn, _ip(dnd, ndbar))
 
 
+def _rhs_adjoint_host(ctx, kind, n, unfused):
+    import os
+    import kielflow_amd as kf
+    old = os.environ.get("KF_PROF_UNFUSED")
+    if unfused:
+        os.environ["KF_PROF_UNFUSED"] = "1"
+    else:
+        os.environ.pop("KF_PROF_UNFUSED", None)
+    try:
+        g, ps, rhs, phi, S, nd, mask, og = _setup(ctx, kind, n, True, True)
+        rng = np.random.default_rng(5)
+        w = [rng.standard_normal(og.cshape) + 1j * rng.standard_normal(og.cshape) for _ in range(4)]
+        adj = kf.Adjoint(rhs, 1e-3)
+        adj.reset()
+        Fb = [g.cfield() for _ in range(4)]
+        adj.rhs_adjoint([ctx.to_device(s) for s in S], [ctx.to_device(x) for x in w], Fb)
+        ndbar = g.rfield()
+        Mbar = adj.accumulators(ndbar)
+        out = [f.get() for f in Fb] + [ndbar.get()], Mbar
+        for o in (adj, rhs, ps, g):
+            o.close()
+        return out
+    finally:
+        if old is None:
+            os.environ.pop("KF_PROF_UNFUSED", None)
+        else:
+            os.environ["KF_PROF_UNFUSED"] = old
+
+
+@pytest.mark.parametrize("kind,n", [("sist", 16), ("hyst", 24)])
+def test_fused_metric_pass_in_the_adjoint_gives_identical_results(ctx, kind, n):
+    """The transposed stretched-axis derivatives with C2R, dS_b multiply and
+    R2C in one z pass give the adjoint RHS of the separate z passes
+    (KF_PROF_UNFUSED) bit for bit."""
+    print("adjoint fused metric pass", kind, n, flush=True)
+    (a, ma), (b, mb) = _rhs_adjoint_host(ctx, kind, n, False), _rhs_adjoint_host(ctx, kind, n, True)
+    assert ma == mb
+    for i, (x, y) in enumerate(zip(a, b)):
+        assert np.array_equal(x, y), (i, float(np.abs(x - y).max()))
+
+
 def _step_host(ctx, rhs, y, dt=1e-3):
     import kielflow_amd as kf
     d = [ctx.to_device(s) for s in y]

@@ -74,6 +74,20 @@ def test_fused_penalization_pass_gives_identical_results(ctx, n, prec):
         assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
 
 
+@pytest.mark.parametrize("kind,n,prec", [("sist", 16, "fp64"), ("sist", 24, "mixed"), ("hyst", 32, "fp64"),
+                                         ("hyst", 48, "mixed"), ("sist", 64, "fp64")])
+def test_fused_metric_pass_gives_identical_results(ctx, kind, n, prec):
+    """A derivative along a stretched axis with its C2R, 1/S' multiply and
+    R2C in one z pass gives the numbers of the separate z passes
+    (KF_PROF_UNFUSED) bit for bit, on radix-2 and radix-3 lengths."""
+    print("fused metric pass", kind, n, prec, flush=True)
+    a = _eval(ctx, kind, n, prec, False, None, env="KF_PROF_UNFUSED")
+    b = _eval(ctx, kind, n, prec, False, "1", env="KF_PROF_UNFUSED")
+    names = ["FUx", "FUy", "FUz", "Fn", "phi"]
+    for name, x, y in zip(names, a, b):
+        assert np.array_equal(x, y), (name, float(np.abs(x - y).max()))
+
+
 def _rk_run(ctx, n, prec, steps, unfused):
     import kielflow_amd as kf
     old = os.environ.get("KF_PEN_UNFUSED")
