@@ -89,8 +89,10 @@ class Adjoint {
     int run_mode_ = RUN_RECOMPUTE;
     double *lam_[4] = {};            // transposed Poisson solutions per stage (warm start)
     bool lam_valid_ = false;
-    double *lamh_[4][3] = {};        // per stage: the last three solutions, newest last
+    static constexpr int LAM_H = 5;  // transposed solutions kept per stage for the warm start
+    double *lamh_[4][LAM_H] = {};    // per stage: the last LAM_H solutions, newest last
     int lam_count_[4] = {0, 0, 0, 0};
+    int lam_order_ = 3;              // extrapolation order of the warm start (KF_ADJ_EXTRAP, 0..4)
     long lin_its_ = 0;
     double2 *Fb_[4] = {}, *linb_[4] = {}, *F1b_ = nullptr, *Yt_ = nullptr, *kb_[4][4] = {};
     double *advb_[4] = {}, *Ub_[3] = {}, *Lb_ = nullptr, *rt_ = nullptr, *Pb_ = nullptr;

@@ -206,23 +206,23 @@ __global__ void k_mul3(double *__restrict__ out, const double *__restrict__ a, c
 }
 
 // x = 3 (h2 - h1) + h0 (order 2) or 2 h2 - h1 (order 1)
-__global__ void k_lam_extrap(double *__restrict__ x, const double *__restrict__ h2, const double *__restrict__ h1,
-                             const double *__restrict__ h0, int order, long n)
+// the polynomial of degree `order` (1..4) through the newest order + 1 of
+// the equally spaced solutions h[4] (newest), h[3], ..., at the next point:
+// binomial weights with alternating signs
+struct LamHist {
+    const double *h[5];
+};
+__global__ void k_lam_extrap(double *__restrict__ x, LamHist hs, int order, long n)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    x[i] = order == 2 ? 3. * (h2[i] - h1[i]) + h0[i] : 2. * h2[i] - h1[i];
-}
-
-// h0 <- h1 <- h2 <- x
-__global__ void k_lam_shift(double *__restrict__ h0, double *__restrict__ h1, double *__restrict__ h2,
-                            const double *__restrict__ x, long n)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    h0[i] = h1[i];
-    h1[i] = h2[i];
-    h2[i] = x[i];
+    const double a = hs.h[4][i], b = hs.h[3][i];
+    switch (order) {
+    case 1: x[i] = 2. * a - b; break;
+    case 2: x[i] = 3. * (a - b) + hs.h[2][i]; break;
+    case 3: x[i] = 4. * (a + hs.h[2][i]) - 6. * b - hs.h[1][i]; break;
+    default: x[i] = 5. * (a - hs.h[1][i]) - 10. * (b - hs.h[2][i]) + hs.h[0][i]; break;
+    }
 }
 
 // ------------------------------------------------------------------ class
@@ -239,6 +239,7 @@ Adjoint::Adjoint(Rhs<double> &rhs, double dt) : rhs_(rhs), g_(rhs.grid()), p_(rh
 {
     const long nr = g_.nr, nc = g_.nc;
     prof_fused_ = std::getenv("KF_PROF_UNFUSED") == nullptr;
+    if (const char *e = std::getenv("KF_ADJ_EXTRAP")) lam_order_ = std::max(0, std::min(4, std::atoi(e)));
     for (auto &c : cap_) {
         for (auto &u : c.U) u = dalloc<double>(nr);
         c.L = dalloc<double>(nr);
@@ -342,7 +343,7 @@ void Adjoint::deriv_T(const double2 *Y, int b, double2 *out, int acc, double coe
     x.acc = acc;
     x.coef = coef * (double)g_.nr;
     if (prof_fused_) {
-        zprof<double>(p_, g_.prof<double>(b), b, s_);
+        zprof<double>(p_, nullptr, nullptr, g_.prof<double>(b), b, s_);
         r2c_yx<double>(g_, p_, x, s_);
         return;
     }
@@ -517,15 +518,21 @@ void Adjoint::reverse_step(const Capture caps[4], double2 *yb[4])
         // of this stage's last transposed Poisson solutions
         const long nr = g_.nr;
         double **h = lamh_[st];
-        if (lam_count_[st] >= 2)
-            hipLaunchKernelGGL(k_lam_extrap, dim3(nbk(nr)), dim3(256), 0, s_, lam_[st], (const double *)h[2],
-                               (const double *)h[1], (const double *)h[0], lam_count_[st] >= 3 ? 2 : 1, nr);
+        const int order = std::min(lam_order_, lam_count_[st] - 1);
+        if (order >= 1) {
+            LamHist hs;
+            for (int q = 0; q < LAM_H; ++q) hs.h[q] = h[q];
+            hipLaunchKernelGGL(k_lam_extrap, dim3(nbk(nr)), dim3(256), 0, s_, lam_[st], hs, order, nr);
+        }
         rhs_adjoint_core(this, rhs_, g_, p_, s_, caps[st], ob, Fb_, linb_, advb_, Ub_, Lb_, F1b_, Yt_, Pb_, phib_,
                          lam_[st], ndb_, Mb_, part_, ltol_, lam_valid_, &Adjoint::deriv_T);
         lin_its_ += rhs_.poisson().last_linear_iterations();
-        hipLaunchKernelGGL(k_lam_shift, dim3(nbk(nr)), dim3(256), 0, s_, h[0], h[1], h[2], (const double *)lam_[st],
-                           nr);
-        if (lam_count_[st] < 3) ++lam_count_[st];
+        // the solution into the oldest slot, which becomes the newest
+        double *old = h[0];
+        for (int q = 0; q + 1 < LAM_H; ++q) h[q] = h[q + 1];
+        h[LAM_H - 1] = old;
+        KF_HIP(hipMemcpyAsync(old, lam_[st], sizeof(double) * nr, hipMemcpyDeviceToDevice, s_));
+        if (lam_count_[st] < LAM_H) ++lam_count_[st];
         for (int a = 0; a < 4; ++a) {
             hipLaunchKernelGGL(k_axpy2, dim3(nb), dim3(256), 0, s_, yb[a], (const double2 *)Fb_[a], 1.0, nc);
             if (st > 0)

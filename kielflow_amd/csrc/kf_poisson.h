@@ -264,6 +264,7 @@ class Poisson {
     double *lsx_ = nullptr, *lsr_ = nullptr, *lsb_ = nullptr;  // linear solve work
     double *lsT_ = nullptr;  // right-hand side of the transposed solve
     int lin_it_ = 0;
+    int lin_expect_ = 1;          // corrections the last linear solve needed
     double *vol_[3] = {};         // per-axis cell volumes (H[i+1]+H[i])/2
     NljState nlj_;
     hipStream_t s0_;

@@ -2128,9 +2128,11 @@ __global__ void k_vscale(const double *__restrict__ in, double *__restrict__ out
 }
 
 __global__ void __launch_bounds__(RED_BLOCK)
-k_absmax2(const double *__restrict__ a, const double *__restrict__ b, long n, double *__restrict__ part)
+k_absmax2(const double *__restrict__ a, const double *__restrict__ b, long n, double *__restrict__ part,
+          const int *ctl)
 {
     __shared__ double sh[RED_BLOCK / 64];
+    if (ctl && ctl[0]) return;
     double ma = 0., mb = 0.;
     for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < n; p += (long)gridDim.x * RED_BLOCK) {
         ma = fmax(ma, fabs(a[p]));
@@ -2144,10 +2146,33 @@ k_absmax2(const double *__restrict__ a, const double *__restrict__ b, long n, do
     }
 }
 
-__global__ void k_axpy64(double *__restrict__ y, const double *__restrict__ x, long n)
+__global__ void k_axpy64(double *__restrict__ y, const double *__restrict__ x, long n, const int *ctl)
 {
+    if (ctl && ctl[0]) return;
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) y[i] += x[i];
+}
+
+// the stopping test of the linear defect correction on the device: stop
+// (ctl[0] = 1) when max|r| <= tol max|b| or after max_it corrections;
+// ctl[1]: corrections done, stat[0]: max|r| / max|b|
+__global__ void __launch_bounds__(RED_BLOCK)
+k_lin_check(const double *__restrict__ part, int nblk, int it, int max_it, double tol, int *ctl, double *stat)
+{
+    __shared__ double sh[RED_BLOCK / 64];
+    if (ctl[0]) return;
+    double rm = 0., bm = 0.;
+    for (int b = threadIdx.x; b < nblk; b += RED_BLOCK) {
+        rm = fmax(rm, part[2 * b]);
+        bm = fmax(bm, part[2 * b + 1]);
+    }
+    rm = block_max(rm, sh);
+    bm = block_max(bm, sh);
+    if (threadIdx.x == 0) {
+        stat[0] = rm / (bm > 0. ? bm : 1.);
+        ctl[1] = it;
+        if (rm <= tol * bm || it >= max_it) ctl[0] = 1;
+    }
 }
 
 void Poisson::set_linearization(const double *phi, hipStream_t s)
@@ -2173,29 +2198,36 @@ int Poisson::linear_solve(double *x, const double *b, double rtol, int max_it, h
     if (guess) KF_HIP(hipMemcpyAsync(lsx_, x, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     else hipLaunchKernelGGL(k_fill<double>, dim3(nb), dim3(256), 0, s, lsx_, N, 0., (const int *)nullptr);
     LevelView<double> v = view<double>(0);
-    std::vector<double> part(2 * (size_t)nbr);
-    int it = 0;
-    for (;; ++it) {
+    const int *ctl = d_ctl_;
+    // residual and stopping test after `it` corrections; every kernel of a
+    // correction launched after the test has stopped returns at once, so
+    // the corrections the last solve needed are launched without waiting
+    // for the host, and the result is that of a test after each one
+    auto check = [&](int it) {
         hipLaunchKernelGGL(k_lin_resid<double>, dim3(nb), dim3(256), 0, s, v, (const double *)lsx_, b,
-                           (const double *)L0.e64, lsr_, (const int *)nullptr);
-        hipLaunchKernelGGL(k_absmax2, dim3(nbr), dim3(RED_BLOCK), 0, s, (const double *)lsr_, b, N, d_red_);
-        KF_HIP(hipMemcpyAsync(part.data(), d_red_, sizeof(double) * 2 * nbr, hipMemcpyDeviceToHost, s));
+                           (const double *)L0.e64, lsr_, ctl);
+        hipLaunchKernelGGL(k_absmax2, dim3(nbr), dim3(RED_BLOCK), 0, s, (const double *)lsr_, b, N, d_red_, ctl);
+        hipLaunchKernelGGL(k_lin_check, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nbr, it, max_it,
+                           rtol, d_ctl_, d_stat_);
+    };
+    check(0);
+    int it = 0, upto = std::min(std::max(1, lin_expect_), max_it);
+    for (;;) {
+        for (; it < upto; ++it) {
+            hipLaunchKernelGGL(k_copy<double>, dim3(nb), dim3(256), 0, s, L0.b64, (const double *)lsr_, N, ctl);
+            if (krylov_) bicgstab<double>(kiters_, s);
+            else vcycle<double>(0, s, true);
+            hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, lsx_, (const double *)L0.x64, N, ctl);
+            check(it + 1);
+        }
+        KF_HIP(hipMemcpyAsync(h_ctl_, d_ctl_, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        KF_HIP(hipMemcpyAsync(h_stat_, d_stat_, sizeof(double), hipMemcpyDeviceToHost, s));
         KF_HIP(hipStreamSynchronize(s));
-        double rm = 0., bm = 0.;
-        for (unsigned q = 0; q < nbr; ++q) {
-            rm = std::max(rm, part[2 * q]);
-            bm = std::max(bm, part[2 * q + 1]);
-        }
-        h_stat_[0] = rm / (bm > 0. ? bm : 1.);
-        if (rm <= rtol * bm || it >= max_it) break;
-        KF_HIP(hipMemcpyAsync(L0.b64, lsr_, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-        if (krylov_) {
-            bicgstab<double>(kiters_, s);
-        } else {
-            vcycle<double>(0, s, true);
-        }
-        hipLaunchKernelGGL(k_axpy64, dim3(nb), dim3(256), 0, s, lsx_, (const double *)L0.x64, N);
+        if (h_ctl_[0] || it >= max_it) break;
+        upto = it + 1;
     }
+    it = h_ctl_[1];
+    lin_expect_ = it;
     KF_HIP(hipMemcpyAsync(x, lsx_, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
     lin_it_ = it;
     return (h_stat_[0] <= rtol) ? it : -1;

@@ -133,6 +133,9 @@ class Rhs {
     void poisson_end();
     void output(C2<R> *F[4]);
     void deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accumulate);
+    // the three derivatives of D23 X on a grid stretched along every axis,
+    // from one staging of the inverse x and y passes
+    void derivs_shared(const C2<R> *X, C2<R> *const out[3], int accumulate);
 
     Grid &g_;
     Poisson &ps_;
@@ -149,6 +152,10 @@ class Rhs {
     // pass (KF_PROF_UNFUSED selects the separate C2R and R2C z passes, which
     // compute the same numbers)
     bool prof_fused_ = true;
+    // on grids stretched along every axis, the three derivatives of a field
+    // share its inverse x and y passes (KF_XY_UNSHARED: one x/y/z round
+    // trip per derivative; the two orders round differently)
+    bool xy_shared_ = true;
     const RkEpilogue *rk_ = nullptr;
     bool rk_done_ = false;
     double *phi_ = nullptr;     // caller-owned

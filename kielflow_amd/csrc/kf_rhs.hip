@@ -154,6 +154,8 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     if (const char *env = std::getenv("KF_EXTRAP_ORDER")) extrap_order_ = std::max(1, std::min(5, std::atoi(env)));
     pen_fused_ = std::getenv("KF_PEN_UNFUSED") == nullptr;
     prof_fused_ = std::getenv("KF_PROF_UNFUSED") == nullptr;
+    xy_shared_ = prof_fused_ && std::getenv("KF_XY_UNSHARED") == nullptr && g.stretched(0) && g.stretched(1) &&
+                 g.stretched(2);
     const long nr = g.nr, nc = g.nc;
     nd_ = ralloc<R>(nr);
     mask_ = ralloc<R>(nr);
@@ -183,6 +185,8 @@ Rhs<R>::Rhs(Grid &g, Poisson &p, const Params &pa, const Switches &sw, hipStream
     if (!g.any_stretched())
         for (auto &f : H_)
             for (auto &h : f) h = ralloc<C2<R>>((long)plan_.nx * plan_.ny * plan_.nzs);
+    if (xy_shared_)
+        for (auto &h : H_[0]) h = ralloc<C2<R>>((long)plan_.nx * plan_.ny * plan_.nzs);
     KF_HIP(hipMemsetAsync(nd_, 0, sizeof(R) * nr, s));
     KF_HIP(hipMemsetAsync(mask_, 0, sizeof(R) * nr, s));
     KF_HIP(hipStreamSynchronize(s));
@@ -498,7 +502,7 @@ void Rhs<R>::deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accum
         // inverse x and y passes, then one z pass for C2R, 1/S' and R2C,
         // then the forward y and x passes
         c2r_xy<R>(g_, plan_, X, 1, s_, b);
-        zprof<R>(plan_, g_.prof<R>(b), b, s_);
+        zprof<R>(plan_, nullptr, nullptr, g_.prof<R>(b), b, s_);
         r2c_yx<R>(g_, plan_, x, s_);
         return;
     }
@@ -508,6 +512,23 @@ void Rhs<R>::deriv_into(const C2<R> *X, int d23_in, int b, C2<R> *out, int accum
     z.pdir = b;
     c2r_spec<R>(g_, plan_, X, 1, b, z, s_);
     r2c_store<R, R>(g_, plan_, rtmp_, x, s_);
+}
+
+// d_b D23 X for b = 0, 1, 2 (OP_partial_derivative.cpp:104-136), the
+// inverse x and y passes staged once (stage_xy: i k_x at the x pass, i k_y
+// at the y pass, i k_z at the z load), then per axis one z pass for C2R,
+// 1/S'_b and R2C, and the forward y and x passes into out[b]
+template <typename R>
+void Rhs<R>::derivs_shared(const C2<R> *X, C2<R> *const out[3], int accumulate)
+{
+    stage_xy<R>(g_, plan_, X, H_[0][0], H_[0][1], H_[0][2], s_);
+    C2<R> *const src[3] = {H_[0][1], H_[0][2], H_[0][0]};
+    for (int b = 0; b < 3; ++b) {
+        XStore<R> x = xs<R>(accumulate ? XK_ACC : XK_WRITE, g_);
+        x.o0 = out[b];
+        zprof<R>(plan_, src[b], b == 2 ? g_.view<R>().k[2] : nullptr, g_.prof<R>(b), b, s_);
+        r2c_yx<R>(g_, plan_, x, s_, src[b]);
+    }
 }
 
 template <typename R>
@@ -556,9 +577,20 @@ void Rhs<R>::eval_general(C2<R> *F[4])
     hipLaunchKernelGGL(k_lin_init<R>, dim3(nbk(nc)), dim3(256), 0, s_, g_.view<R>(), (const C2<R> *)F[0],
                        (const C2<R> *)F[1], (const C2<R> *)F[2], (const C2<R> *)F[3], lin_[0], lin_[1],
                        lin_[2], lin_[3], (R)pa_.tau, sw_.sv_on);
-    for (int a = 0; a < 3; ++a) deriv_into(F1_, 0, a, lin_[a], 1);
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 3; ++b) deriv_into(F[a], 1, b, dF_[3 * a + b], 0);
+    if (xy_shared_) {
+        // the stretched derivatives apply the 2/3 rule to their input, F1
+        // included (deriv_into)
+        C2<R> *const o1[3] = {lin_[0], lin_[1], lin_[2]};
+        derivs_shared(F1_, o1, 1);
+        for (int a = 0; a < 4; ++a) {
+            C2<R> *const o[3] = {dF_[3 * a], dF_[3 * a + 1], dF_[3 * a + 2]};
+            derivs_shared(F[a], o, 0);
+        }
+    } else {
+        for (int a = 0; a < 3; ++a) deriv_into(F1_, 0, a, lin_[a], 1);
+        for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 3; ++b) deriv_into(F[a], 1, b, dF_[3 * a + b], 0);
+    }
     hipLaunchKernelGGL(k_lin_combine<R>, dim3(nbk(nc)), dim3(256), 0, s_, nc, lin_[0], lin_[1], lin_[2],
                        lin_[3], (const C2<R> *)dF_[0], (const C2<R> *)dF_[3], (const C2<R> *)dF_[6],
                        (const C2<R> *)dF_[9], (const C2<R> *)dF_[0], (const C2<R> *)dF_[4],

@@ -2,6 +2,8 @@
 // transform helpers built from them (shared by the RHS and its adjoint).
 #pragma once
 
+#include <type_traits>
+
 #include "kf_fft.h"
 #include "kf_grid.h"
 
@@ -774,10 +776,10 @@ struct ProfMul {
 // into LDS, and its R2C back over the row in place.  Same arithmetic as the
 // ZK_PROF z pass and k_zr2c one after another, without the real field in
 // between.
-template <typename R, int M, int RB>
+template <typename R, int M, int RB, typename L>
 __global__ void __launch_bounds__(tpt_for(M) * RB)
 k_zprof(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long rows, int ny, int nzs,
-        C2<R> *rowsbuf, const R *prof, int pdir, R scale)
+        C2<R> *rowsbuf, L ld, const R *prof, int pdir, R scale)
 {
     using C = C2<R>;
     using Buf = RowBuf<C, M>;
@@ -800,14 +802,13 @@ k_zprof(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long r
     int pos[E];
 #pragma unroll
     for (int e = 0; e < E; ++e) pos[e] = -1;
-    const RowLoadC<R> ld{rowsbuf, nzs};
     const ProfMul<R> pm{acc, pos, prof, pdir, ny, row, scale, live};
     if constexpr (merge_in_regs(M)) {
         C z[M / TPT];
         merge_regs<C, M>(ld, row, live, tt, twN, z);
         fft_io<C, M, TPT, +1, false, false>(tw, tt, SrcRegs<C>{z}, pm, A, A);
     } else {
-        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, RowLoadC<R>>{ld, row, live, twN}, pm, A, A);
+        fft_io<C, M, TPT, +1, false, false>(tw, tt, MergeLoad<C, M, L>{ld, row, live, twN}, pm, A, A);
     }
     // the product row into LDS (pairs 2t, 2t+1 at t), then its forward
     // Nz/2-point transform in place
@@ -834,26 +835,34 @@ k_zprof(const C2<R> *__restrict__ twM_g, const C2<R> *__restrict__ twN_g, long r
     }
 }
 
-// k_zprof over all rows of the plan's scratch
+// k_zprof over all rows of buf (scratch layout; the plan's scratch when
+// null), with i k_z applied at the load when kz is set
 template <typename R>
-inline void zprof(Plan<R> &p, const R *prof, int pdir, hipStream_t s)
+inline void zprof(Plan<R> &p, C2<R> *buf, const R *kz, const R *prof, int pdir, hipStream_t s)
 {
     prof::Scope ps_(prof::FFT_ZC2R, s);
     const long rows = (long)p.nx * p.ny;
     const R scale = (R)(1.0 / (double)p.nr);
-    switch (p.nz / 2) {
-#define KF_CASE(NN)                                                                                            \
-    case NN: {                                                                                                 \
-        constexpr int RB = cb_for<R>(NN);                                                                      \
-        hipLaunchKernelGGL((k_zprof<R, NN, RB>), dim3((unsigned)((rows + RB - 1) / RB)), dim3(tpt_for(NN) * RB), \
-                           0, s, p.tw_m, p.tw_n, rows, p.ny, p.nzs, p.scratch, prof, pdir, scale);             \
-        break;                                                                                                 \
+    C2<R> *b = buf ? buf : p.scratch;
+    auto go = [&](const auto &ld) {
+        using L = std::decay_t<decltype(ld)>;
+        switch (p.nz / 2) {
+#define KF_CASE(NN)                                                                                              \
+    case NN: {                                                                                                   \
+        constexpr int RB = cb_for<R>(NN);                                                                        \
+        hipLaunchKernelGGL((k_zprof<R, NN, RB, L>), dim3((unsigned)((rows + RB - 1) / RB)),                      \
+                           dim3(tpt_for(NN) * RB), 0, s, p.tw_m, p.tw_n, rows, p.ny, p.nzs, b, ld, prof, pdir,  \
+                           scale);                                                                               \
+        break;                                                                                                   \
     }
-        KF_FFT_LENGTHS(KF_CASE)
+            KF_FFT_LENGTHS(KF_CASE)
 #undef KF_CASE
-    default:
-        throw Error("unsupported z length " + std::to_string(p.nz));
-    }
+        default:
+            throw Error("unsupported z length " + std::to_string(p.nz));
+        }
+    };
+    if (kz) go(RowLoadCK<R>{b, p.nzs, kz});
+    else go(RowLoadC<R>{b, p.nzs});
 }
 
 // k_zpen over all rows: lin (scratch layout, after the inverse x and y
@@ -898,16 +907,18 @@ inline void c2r_xy(Grid &g, Plan<R> &p, const C2<R> *src, int d23, hipStream_t s
                           StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
 }
 
-// the y and x passes of a forward transform from the plan's scratch (after
-// a z pass of the caller's own) into an XStore
+// the y and x passes of a forward transform from buf (scratch layout; the
+// plan's scratch when null), after a z pass of the caller's own, into an
+// XStore; the y pass runs in place on buf
 template <typename R>
-inline void r2c_yx(Grid &g, Plan<R> &p, const XStore<R> &st, hipStream_t s)
+inline void r2c_yx(Grid &g, Plan<R> &p, const XStore<R> &st, hipStream_t s, C2<R> *buf = nullptr)
 {
     const StridedAddr ax = saddr_x(p), ay = saddr_y(p);
-    launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{p.scratch, ay, (unsigned)p.nzc},
-                          StoreMasked<R>{p.scratch, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
+    C2<R> *b = buf ? buf : p.scratch;
+    launch_strided<R, -1>(p.ny, scratch_cols_y(p), p.tw_y, LoadMasked<R>{b, ay, (unsigned)p.nzc},
+                          StoreMasked<R>{b, ay, (unsigned)p.nzc}, s, prof::FFT_Y, p.vy);
     with_xstore<R>(st, [&](const auto &x) {
-        launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{p.scratch, ax}, x, s, prof::FFT_X, p.vx);
+        launch_strided<R, -1>(p.nx, (long)p.ny * p.nzc, p.tw_x, LoadPlain<R>{b, ax}, x, s, prof::FFT_X, p.vx);
     });
 }
 
