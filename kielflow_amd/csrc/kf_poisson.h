@@ -234,6 +234,7 @@ class Poisson {
     // sweeps on the coarsest level: 8 (measured: 24 gave the same Newton
     // counts and cost 4-6% of a stretched-grid step; 4 cost Newton iterations)
     int nu1_ = 2, nu2_ = 2, nuc_ = 8, inner_ = 1;
+    long coarse_n_ = 4096;        // levels of at most this many points run in the one-workgroup tail (KF_COARSE_N)
     int graph_it_ = 0;
     // the finest level's FP32 x, b, e are stored colour by colour
     bool split0_ = false;
