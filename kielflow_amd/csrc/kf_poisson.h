@@ -235,6 +235,7 @@ class Poisson {
     // counts and cost 4-6% of a stretched-grid step; 4 cost Newton iterations)
     int nu1_ = 2, nu2_ = 2, nuc_ = 8, inner_ = 1;
     long coarse_n_ = 4096;        // levels of at most this many points run in the one-workgroup tail (KF_COARSE_N)
+    bool coarse_wave_ = true;     // one wavefront smooths the coarsest level of the tail (KF_COARSE_WAVE=0: all)
     int graph_it_ = 0;
     // the finest level's FP32 x, b, e are stored colour by colour
     bool split0_ = false;

@@ -953,7 +953,56 @@ struct CoarseSet {
     R *x[COARSE_MAXL], *b[COARSE_MAXL], *e[COARSE_MAXL], *r[COARSE_MAXL];
     int nu1, nu2, nuc;
     int zero0;  // start the first level from x = 0
+    int wave;   // coarsest level of at most 64 points in LDS: one wavefront smooths it
 };
+
+// wavefront-scope barrier: the LDS writes of every lane are visible to the
+// wavefront's later reads
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// blk_smooth on a level of at most 64 points whose x is in LDS, run by the
+// first wavefront alone (call with threadIdx.x < 64): one point per lane,
+// b and e held in registers, wavefront barriers between the phases.  Same
+// updates in the same colour order as blk_smooth.
+template <typename R>
+__device__ void wave_smooth(const LevelView<R> &L, R *x, const R *b, const R *e, int sweeps)
+{
+    const int nx = L.n[0], ny = L.n[1], nz = L.n[2];
+    const bool rb = !(nx & 1) && !(ny & 1) && !(nz & 1);
+    const int p = threadIdx.x;
+    const bool act = p < L.N;
+    int i = 0, j = 0, k = 0;
+    if (act) dec3(p, ny, nz, i, j, k);
+    const R bp = act ? b[p] : (R)0, ep = act ? e[p] : (R)0;
+    const int col = (i + j + k) & 1;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        if (rb) {
+            for (int c = 0; c < 2; ++c) {
+                if (act && col == c) {
+                    R A, D;
+                    stencil<R>(L, x, i, j, k, A, D);
+                    x[p] = (A - bp) / (D + ep);
+                }
+                wave_sync();
+            }
+        } else {
+            R v = (R)0;
+            if (act) {
+                R A, D;
+                stencil<R>(L, x, i, j, k, A, D);
+                v = x[p] + (R)0.8 * ((A - bp) / (D + ep) - x[p]);
+            }
+            wave_sync();
+            if (act) x[p] = v;
+            wave_sync();
+        }
+    }
+}
 
 template <typename R>
 __device__ void blk_smooth(const LevelView<R> &L, R *x, const R *b, const R *e, R *r, int sweeps)
@@ -1063,7 +1112,12 @@ __global__ void __launch_bounds__(COARSE_BLOCK) k_coarse_vcycle(CoarseSet<R> cs,
         }
         __syncthreads();
     }
-    blk_smooth<R>(cs.L[nl - 1], CX(nl - 1), CB(nl - 1), cs.e[nl - 1], CR(nl - 1), cs.nuc);
+    if (cs.wave && cs.L[nl - 1].N <= 64 && soff[nl - 1] >= 0) {
+        if (tid < 64) wave_smooth<R>(cs.L[nl - 1], CX(nl - 1), CB(nl - 1), cs.e[nl - 1], cs.nuc);
+        __syncthreads();
+    } else {
+        blk_smooth<R>(cs.L[nl - 1], CX(nl - 1), CB(nl - 1), cs.e[nl - 1], CR(nl - 1), cs.nuc);
+    }
     for (int l = nl - 2; l >= 0; --l) {
         const LevelView<R> &L = cs.L[l], &C = cs.L[l + 1];
         const int cx = C.n[0], cy = C.n[1], cz = C.n[2];
@@ -1368,6 +1422,7 @@ Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed
     // run and the same at 256^3 EqSt)
     coarse_n_ = ax_desc_uniform() ? COARSE_N : 512;
     if (const char *e = std::getenv("KF_COARSE_N")) coarse_n_ = std::max(8L, std::min(COARSE_N, std::atol(e)));
+    coarse_wave_ = std::getenv("KF_COARSE_WAVE") == nullptr || std::atoi(std::getenv("KF_COARSE_WAVE")) != 0;
     int n[3] = {ax[0].n, ax[1].n, ax[2].n};
     for (int l = 0; l < 16; ++l) {
         Level L;
@@ -1590,6 +1645,7 @@ CoarseSet<R> Poisson::coarse_set(int l0) const
     cs.nu1 = nu1_;
     cs.nu2 = nu2_;
     cs.nuc = nuc_;
+    cs.wave = coarse_wave_ ? 1 : 0;
     return cs;
 }
 

@@ -55,6 +55,30 @@ def test_newton_mg_mixed_precision(ctx, oracle):
     assert np.abs(a.get() - b.get()).max() < 1e-8 * np.abs(a.get()).max()
 
 
+@pytest.mark.parametrize("kind,n,prec", [("eqst", 32, "fp64"), ("eqst", 48, "mixed"), ("sist", 32, "fp64"),
+                                         ("hyst", 24, "fp64"), ("eqst", 64, "mixed")])
+def test_wavefront_coarsest_smoothing_gives_identical_results(ctx, monkeypatch, kind, n, prec):
+    """The coarsest level (4^3 red-black, 3^3 damped Jacobi) smoothed by one
+    wavefront gives the solve of the whole workgroup (KF_COARSE_WAVE=0) bit
+    for bit, iteration count included."""
+    import kielflow_amd as kf
+    print("coarsest wavefront", kind, n, prec, flush=True)
+    axes, _, rho = _problem(kind, n)
+    out = []
+    for wave in ("1", "0"):
+        monkeypatch.setenv("KF_COARSE_WAVE", wave)
+        g = kf.Grid(ctx, axes)
+        p = kf.Poisson(g, kf.KF_FP64 if prec == "fp64" else kf.KF_MIXED, tol=1e-12 if prec == "fp64" else 1e-10,
+                       max_it=60)
+        dphi = g.rfield()
+        it, res, code = p.solve(dphi, ctx.to_device(rho))
+        out.append((it, res, dphi.get()))
+        p.close()
+        g.close()
+    assert out[0][0] == out[1][0] and out[0][1] == out[1][1]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
 def test_not_converged_is_an_error(ctx):
     import kielflow_amd as kf
     axes, oaxes, rho = _problem("eqst", 16)
