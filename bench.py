@@ -285,18 +285,21 @@ def config1_rate(kf, ctx, n=128, steps=20, warm=2):
     for _ in range(warm):
         step()
     ctx.sync()
+    n0, c0 = ps.newton_counts()
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
     ctx.sync()
     t = time.perf_counter() - t0
+    n1, c1 = ps.newton_counts()
     fails = rhs.poisson_failures()
     for o in (rk, fft, rhs, ps):
         o.close()
     g.close()
     return {"workload": f"{n}^3 SiSt (S'(0) = 1/3) FP64 RK4, spectral viscosity and penalization on, "
                         "dt from cfl_all every step", "steps_per_s": round(steps / t, 3),
-            "ms_per_step": round(1e3 * t / steps, 3), "steps": steps, "last_dt": dt, "poisson_failures": fails}
+            "ms_per_step": round(1e3 * t / steps, 3), "steps": steps, "last_dt": dt, "poisson_failures": fails,
+            "newton_corrections_per_solve": round((c1 - c0) / max(1, n1 - n0), 3)}
 
 
 def batch_sweep(kf, ctx, nbox=64, steps=100, warm=5, seed=0, nstreams=4, poisson_iterations=3, check_boxes=4,
