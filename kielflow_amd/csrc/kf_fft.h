@@ -144,8 +144,12 @@ k_strided(const C2<R> *__restrict__ tw_g, long Q, LD ld, ST_ st)
 // the position along it).  The inputs of the first pass stay in registers,
 // which roughly doubles the registers of a single pass; with one LDS buffer
 // (no ping-pong) the narrow column block is bounded by registers alone.
+// FP32: at least 6 waves per SIMD (80 VGPRs, no spills; the compiler's own
+// choice is 81-82, which allows 5: the 256-point passes then take 65 and
+// 44.6 us against 68.9 and 48.9).  FP64 is left to the compiler (the same
+// bound spills at lengths 12 and 24).
 template <typename R, int N, int CB, int DIR, class LD, class STA, class STB>
-__global__ void __launch_bounds__(tpt_for(N) * CB)
+__global__ void __launch_bounds__(tpt_for(N) * CB, sizeof(R) == 4 ? 6 : 1)
 k_strided_ik(const C2<R> *__restrict__ tw_g, long Q, LD ld, STA sta, STB stb, const R *__restrict__ kax)
 {
     using C = C2<R>;

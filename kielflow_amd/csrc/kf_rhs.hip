@@ -841,15 +841,13 @@ void op_partial_derivative(Grid &g, Plan<double> &p, int dir, const double2 *in,
                            0, 0);
         return;
     }
-    double *tmp = g.workspace();
-    ZStore<double> z = zs<double>(ZK_PROF, p);
-    z.o0 = tmp;
-    z.prof = g.prof<double>(dir);
-    z.pdir = dir;
-    c2r_spec<double>(g, p, in, 1, dir, z, s);
+    // inverse x and y passes with D23 and i k_dir at the x load, one z pass
+    // for C2R, 1/S' and R2C, the forward y and x passes
     XStore<double> x = xs<double>(XK_WRITE, g);
     x.o0 = out;
-    r2c_store<double, double>(g, p, tmp, x, s);
+    c2r_xy<double>(g, p, in, 1, s, dir);
+    zprof<double>(p, nullptr, nullptr, g.prof<double>(dir), dir, s);
+    r2c_yx<double>(g, p, x, s);
 }
 
 void op_dealias23(Grid &g, double2 *inout, hipStream_t s)
