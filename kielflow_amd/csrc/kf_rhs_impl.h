@@ -492,11 +492,9 @@ inline void stage_xy(Grid &g, Plan<R> &p, const C2<R> *F, C2<R> *H, C2<R> *Hx, C
     const StridedAddr ay = saddr_y(p);
     const long qx = scratch_cols_x(p), qy = scratch_cols_y(p);
     const unsigned nv = (unsigned)p.nzc;
-    // one load of F: D23 F -> H and i k_x D23 F -> Hx; in FP32 with the
-    // narrow column block, which fits more workgroups per CU (measured: 8%
-    // faster for this pass, slower for the y pass below)
+    // one load of F: D23 F -> H and i k_x D23 F -> Hx
     launch_strided_ik<R, +1>(p.nx, qx, p.tw_x, pad_cols(XLoadMask<R, 1>{F, gv}, p), scratch_store_x(H, p),
-                             scratch_store_x(Hx, p), gv.k[0], s, prof::FFT_X, sizeof(R) == 4 ? 1 : p.vx);
+                             scratch_store_x(Hx, p), gv.k[0], s, prof::FFT_X, p.vx);
     // one load of H: H in place and i k_y H -> Hy
     launch_strided_ik<R, +1>(p.ny, qy, p.tw_y, LoadMasked<R>{H, ay, nv}, StoreMasked<R>{H, ay, nv},
                              StoreMasked<R>{Hy, ay, nv}, gv.k[1], s, prof::FFT_Y, p.vy);
