@@ -260,6 +260,12 @@ class Poisson {
     double ion_theta_ = 0.;       // Boltzmann-ion theta (0: none)
     bool krylov_ = false;         // BiCGStab acceleration (stretched grids)
     int kiters_ = 2;
+    // the preconditioning V-cycle reads its right side from, and writes its
+    // solution to, the Krylov vectors themselves (finest level launched on
+    // its own); KF_PRECOND_COPY=1 copies them through the level's x and b
+    bool precond_direct_ = true;
+    void *x0o_ = nullptr;         // finest-level x and b of the current V-cycle, when set
+    const void *b0o_ = nullptr;
     void *kv_[10] = {};           // BiCGStab vectors (precision of the sweeps)
     double *d_scal_ = nullptr;    // device scalars of BiCGStab
     hipEvent_t ev_[2] = {};       // behind the convergence-flag copies of the last two checks

@@ -1423,6 +1423,7 @@ Poisson::Poisson(const AxisDesc ax[3], bool mixed, hipStream_t s) : mixed_(mixed
     coarse_n_ = ax_desc_uniform() ? COARSE_N : 512;
     if (const char *e = std::getenv("KF_COARSE_N")) coarse_n_ = std::max(8L, std::min(COARSE_N, std::atol(e)));
     coarse_wave_ = std::getenv("KF_COARSE_WAVE") == nullptr || std::atoi(std::getenv("KF_COARSE_WAVE")) != 0;
+    precond_direct_ = std::getenv("KF_PRECOND_COPY") == nullptr || std::atoi(std::getenv("KF_PRECOND_COPY")) == 0;
     int n[3] = {ax[0].n, ax[1].n, ax[2].n};
     for (int l = 0; l < 16; ++l) {
         Level L;
@@ -1563,6 +1564,7 @@ void Poisson::zero_level(int l, hipStream_t s)
     R *x;
     if constexpr (std::is_same<R, double>::value) x = L.x64;
     else x = L.x32;
+    if (l == 0 && x0o_) x = (R *)x0o_;
     hipLaunchKernelGGL(k_fill<R>, dim3(nblocks(L.N, 256)), dim3(256), 0, s, x, L.N, (R)0, (const int *)d_ctl_);
 }
 
@@ -1574,6 +1576,10 @@ void Poisson::smooth(int l, int sweeps, hipStream_t s, bool zero)
     R *x, *b, *e, *r;
     if constexpr (std::is_same<R, double>::value) { x = L.x64; b = L.b64; e = L.e64; r = L.r64; }
     else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; }
+    if (l == 0 && x0o_) {
+        x = (R *)x0o_;
+        b = (R *)b0o_;  // read only
+    }
     const bool rb = (L.n[0] % 2 == 0) && (L.n[1] % 2 == 0) && (L.n[2] % 2 == 0);
     const bool sp = split_level<R>(l);
     if (zero && (!rb || sweeps < 1)) {
@@ -1702,6 +1708,10 @@ void Poisson::vcycle(int l, hipStream_t s, bool zero)
     R *x, *b, *e, *r, *cx, *cb;
     if constexpr (std::is_same<R, double>::value) { x = L.x64; b = L.b64; e = L.e64; r = L.r64; cx = C.x64; cb = C.b64; }
     else { x = L.x32; b = L.b32; e = L.e32; r = L.r32; cx = C.x32; cb = C.b32; }
+    if (l == 0 && x0o_) {
+        x = (R *)x0o_;
+        b = (R *)b0o_;  // read only
+    }
     smooth<R>(l, nu1_, s, zero);
     LevelView<R> v = view<R>(l);
     const bool sp = split_level<R>(l);
@@ -1899,7 +1909,19 @@ void Poisson::bicgstab(int iters, hipStream_t s)
     hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, v, N, (R)0, ctl);
     hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 0, ctl);
     (void)bytes;
+    // a finest level launched on its own (not inside the one-workgroup
+    // tail, whose pointers are its levels') reads and writes the Krylov
+    // vectors in place of its b and x
+    const bool direct = precond_direct_ && first_coarse() > 0 && !split_level<R>(0);
     auto precond = [&](const R *in, R *out) {
+        if (direct) {
+            x0o_ = out;
+            b0o_ = in;
+            vcycle<R>(0, s, true);
+            x0o_ = nullptr;
+            b0o_ = nullptr;
+            return;
+        }
         hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, b0, in, N, ctl);
         vcycle<R>(0, s, true);
         hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, out, (const R *)x0, N, ctl);
