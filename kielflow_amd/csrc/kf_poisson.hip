@@ -1897,22 +1897,21 @@ void Poisson::bicgstab(int iters, hipStream_t s)
     R *x0, *b0, *e0;
     if constexpr (std::is_same<R, double>::value) { x0 = L0.x64; b0 = L0.b64; e0 = L0.e64; }
     else { x0 = L0.x32; b0 = L0.b32; e0 = L0.e32; }
-    R *X = (R *)kv_[0], *r = (R *)kv_[1], *rh = (R *)kv_[2], *p = (R *)kv_[3], *v = (R *)kv_[4];
-    R *sv = (R *)kv_[5], *t = (R *)kv_[6], *ph = (R *)kv_[7], *shv = (R *)kv_[8];
+    // a finest level launched on its own (not inside the one-workgroup
+    // tail, whose pointers are its levels') reads and writes the Krylov
+    // vectors in place of its b and x; its own b and x then hold the
+    // residual and the iterate (b is consumed, x is the result)
+    const bool direct = precond_direct_ && first_coarse() > 0 && !split_level<R>(0);
+    R *X = direct ? x0 : (R *)kv_[0], *r = direct ? b0 : (R *)kv_[1], *rh = (R *)kv_[2], *p = (R *)kv_[3];
+    R *v = (R *)kv_[4], *sv = (R *)kv_[5], *t = (R *)kv_[6], *ph = (R *)kv_[7], *shv = (R *)kv_[8];
     LevelView<R> v0 = view<R>(0);
-    const size_t bytes = sizeof(R) * N;
     // r = b, rh = b, x = 0, p = v = 0
-    hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, r, (const R *)b0, N, ctl);
+    if (!direct) hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, r, (const R *)b0, N, ctl);
     hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, rh, (const R *)b0, N, ctl);
     hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, X, N, (R)0, ctl);
     hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, p, N, (R)0, ctl);
     hipLaunchKernelGGL(k_fill<R>, dim3(nb), dim3(256), 0, s, v, N, (R)0, ctl);
     hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 0, ctl);
-    (void)bytes;
-    // a finest level launched on its own (not inside the one-workgroup
-    // tail, whose pointers are its levels') reads and writes the Krylov
-    // vectors in place of its b and x
-    const bool direct = precond_direct_ && first_coarse() > 0 && !split_level<R>(0);
     auto precond = [&](const R *in, R *out) {
         if (direct) {
             x0o_ = out;
@@ -1945,7 +1944,7 @@ void Poisson::bicgstab(int iters, hipStream_t s)
         hipLaunchKernelGGL(k_bicg_xr<R>, dim3(nb), dim3(256), 0, s, X, r, (const R *)ph, (const R *)shv,
                            (const R *)sv, (const R *)t, N, (const double *)d_scal_, ctl);
     }
-    hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, x0, (const R *)X, N, ctl);
+    if (!direct) hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, x0, (const R *)X, N, ctl);
 }
 
 void Poisson::set_ion_boltzmann(double theta)
