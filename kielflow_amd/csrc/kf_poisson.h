@@ -215,7 +215,9 @@ class Poisson {
     template <typename R>
     void bicgstab(int iters, hipStream_t s);
     template <typename R>
-    void dot(const R *a, const R *b, int slot, hipStream_t s);
+    // (a, b) into scalar slot (and (a, c) into slot + 1 when c is set), then
+    // the BiCGStab scalar update op when op >= 0
+    void dot(const R *a, const R *b, int slot, hipStream_t s, int op = -1, const R *c = nullptr);
     template <typename R>
     CoarseSet<R> coarse_set(int l0) const;
     int first_coarse() const;

@@ -1192,24 +1192,36 @@ k_dot_part(const R *__restrict__ a, const R *__restrict__ b, long n, double *__r
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
+// (a, b) and (a, c) in one pass over a: the per-block partials of each in
+// the order k_dot_part forms them, (a, b) at part[0, nb), (a, c) at
+// part[nb, 2 nb)
+template <typename R>
 __global__ void __launch_bounds__(RED_BLOCK)
-k_dot_final(const double *__restrict__ part, int nb, double *scal, int slot, const int *ctl)
+k_dot2_part(const R *__restrict__ a, const R *__restrict__ b, const R *__restrict__ c, long n,
+            double *__restrict__ part, const int *ctl)
 {
     __shared__ double sh[RED_BLOCK / 64];
     if (ctl[0]) return;
-    double acc = 0.;
-    for (int b = threadIdx.x; b < nb; b += RED_BLOCK) acc += part[b];
-    acc = block_sum(acc, sh);
-    if (threadIdx.x == 0) scal[slot] = acc;
+    double ab = 0., ac = 0.;
+    for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < n; p += (long)gridDim.x * RED_BLOCK) {
+        const double x = (double)a[p];
+        ab += x * (double)b[p];
+        ac += x * (double)c[p];
+    }
+    ab = block_sum(ab, sh);
+    ac = block_sum(ac, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ab;
+        part[gridDim.x + blockIdx.x] = ac;
+    }
 }
 
 __device__ __forceinline__ double safe_div(double a, double b) { return b != 0. ? a / b : 0.; }
 
 // op 0: init (rho=alpha=omega=1); op 1: beta = (rho_new/rho)(alpha/omega), rho = rho_new;
 // op 2: alpha = rho/(rh,v); op 3: omega = (t,s)/(t,t)
-__global__ void k_scal(double *scal, int op, const int *ctl)
+__device__ void scal_apply(double *scal, int op)
 {
-    if (ctl[0] || threadIdx.x != 0) return;
     if (op == 0) {
         scal[0] = 1.;
         scal[1] = 1.;
@@ -1222,6 +1234,29 @@ __global__ void k_scal(double *scal, int op, const int *ctl)
     } else {
         scal[2] = safe_div(scal[6], scal[7]);
     }
+}
+
+__global__ void k_scal(double *scal, int op, const int *ctl)
+{
+    if (ctl[0] || threadIdx.x != 0) return;
+    scal_apply(scal, op);
+}
+
+// the sums of nd partial sets of nb (set q at part[q nb, (q+1) nb)) into
+// scal[slot + q], then the scalar update op (none when op < 0)
+__global__ void __launch_bounds__(RED_BLOCK)
+k_dot_final(const double *__restrict__ part, int nb, double *scal, int slot, const int *ctl, int nd = 1,
+            int op = -1)
+{
+    __shared__ double sh[RED_BLOCK / 64];
+    if (ctl[0]) return;
+    for (int q = 0; q < nd; ++q) {
+        double acc = 0.;
+        for (int b = threadIdx.x; b < nb; b += RED_BLOCK) acc += part[q * nb + b];
+        acc = block_sum(acc, sh);
+        if (threadIdx.x == 0) scal[slot + q] = acc;
+    }
+    if (threadIdx.x == 0 && op >= 0) scal_apply(scal, op);
 }
 
 // p = r + beta (p - omega v)
@@ -1875,14 +1910,18 @@ void Poisson::newton_end()
 }
 
 template <typename R>
-void Poisson::dot(const R *a, const R *b, int slot, hipStream_t s)
+void Poisson::dot(const R *a, const R *b, int slot, hipStream_t s, int op, const R *c)
 {
     const long N = lv_[0].N;
     unsigned nb = nblocks(N, RED_BLOCK);
     if (nb > 1024) nb = 1024;
-    hipLaunchKernelGGL(k_dot_part<R>, dim3(nb), dim3(RED_BLOCK), 0, s, a, b, N, d_red_, (const int *)d_ctl_);
+    if (c)
+        hipLaunchKernelGGL(k_dot2_part<R>, dim3(nb), dim3(RED_BLOCK), 0, s, a, b, c, N, d_red_,
+                           (const int *)d_ctl_);
+    else
+        hipLaunchKernelGGL(k_dot_part<R>, dim3(nb), dim3(RED_BLOCK), 0, s, a, b, N, d_red_, (const int *)d_ctl_);
     hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nb, d_scal_,
-                       slot, (const int *)d_ctl_);
+                       slot, (const int *)d_ctl_, c ? 2 : 1, op);
 }
 
 // right-preconditioned BiCGStab (van der Vorst 1992) for J x = b, x0 = 0,
@@ -1926,21 +1965,34 @@ void Poisson::bicgstab(int iters, hipStream_t s)
         hipLaunchKernelGGL(k_copy<R>, dim3(nb), dim3(256), 0, s, out, (const R *)x0, N, ctl);
     };
     for (int k = 0; k < iters; ++k) {
-        dot<R>(rh, r, 8, s);
-        hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 1, ctl);
+        if (direct) {
+            dot<R>(rh, r, 8, s, 1);
+        } else {
+            dot<R>(rh, r, 8, s);
+            hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 1, ctl);
+        }
         hipLaunchKernelGGL(k_bicg_p<R>, dim3(nb), dim3(256), 0, s, p, (const R *)r, (const R *)v, N,
                            (const double *)d_scal_, ctl);
         precond(p, ph);
         hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)ph, (const R *)e0, v, ctl);
-        dot<R>(rh, v, 5, s);
-        hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 2, ctl);
+        if (direct) {
+            dot<R>(rh, v, 5, s, 2);
+        } else {
+            dot<R>(rh, v, 5, s);
+            hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 2, ctl);
+        }
         hipLaunchKernelGGL(k_bicg_s<R>, dim3(nb), dim3(256), 0, s, sv, (const R *)r, (const R *)v, N,
                            (const double *)d_scal_, ctl);
         precond(sv, shv);
         hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)shv, (const R *)e0, t, ctl);
-        dot<R>(t, sv, 6, s);
-        dot<R>(t, t, 7, s);
-        hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 3, ctl);
+        if (direct) {
+            // (t, s) and (t, t) from one pass over t, then omega
+            dot<R>(t, sv, 6, s, 3, t);
+        } else {
+            dot<R>(t, sv, 6, s);
+            dot<R>(t, t, 7, s);
+            hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 3, ctl);
+        }
         hipLaunchKernelGGL(k_bicg_xr<R>, dim3(nb), dim3(256), 0, s, X, r, (const R *)ph, (const R *)shv,
                            (const R *)sv, (const R *)t, N, (const double *)d_scal_, ctl);
     }
