@@ -219,6 +219,8 @@ class Poisson {
     // the BiCGStab scalar update op when op >= 0
     void dot(const R *a, const R *b, int slot, hipStream_t s, int op = -1, const R *c = nullptr);
     template <typename R>
+    void apply_J_dot(const R *x, R *y, const R *a, const R *c, int slot, int op, hipStream_t s);
+    template <typename R>
     CoarseSet<R> coarse_set(int l0) const;
     int first_coarse() const;
     template <typename R>

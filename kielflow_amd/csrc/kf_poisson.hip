@@ -1312,6 +1312,41 @@ __global__ void k_apply_J(LevelView<R> L, const R *__restrict__ x, const R *__re
     y[p] = A - D * x[p] - e[p] * x[p];
 }
 
+// y = J x, as k_apply_J, with the per-block partials of (a, y), or of (y, c)
+// and (y, y) when c is set, in the partition and order of k_dot_part /
+// k_dot2_part (grid-stride over the same blocks)
+template <typename R>
+__global__ void __launch_bounds__(RED_BLOCK)
+k_apply_J_dot(LevelView<R> L, const R *__restrict__ x, const R *__restrict__ e, R *__restrict__ y,
+              const R *__restrict__ a, const R *__restrict__ c, long n, double *__restrict__ part, const int *ctl)
+{
+    __shared__ double sh[RED_BLOCK / 64];
+    if (ctl[0]) return;
+    const int nz = L.n[2], ny = L.n[1];
+    double s1 = 0., s2 = 0.;
+    for (long p = (long)blockIdx.x * RED_BLOCK + threadIdx.x; p < n; p += (long)gridDim.x * RED_BLOCK) {
+        int i, j, k;
+        dec3(p, ny, nz, i, j, k);
+        R A, D;
+        stencil<R>(L, x, i, j, k, A, D);
+        const R yv = A - D * x[p] - e[p] * x[p];
+        y[p] = yv;
+        if (c) {
+            const double t = (double)yv;
+            s1 += t * (double)c[p];
+            s2 += t * (double)yv;
+        } else {
+            s1 += (double)a[p] * (double)yv;
+        }
+    }
+    s1 = block_sum(s1, sh);
+    if (c) s2 = block_sum(s2, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = s1;
+        if (c) part[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
 // ======================================================================== reference NLJ kernels
 // newton_method::solve (newton_method.hpp:17-63), f = A - D x - e^x
 // the local function of solver_poisson_jacobi_nlin.cpp:190-199; with
@@ -1924,6 +1959,23 @@ void Poisson::dot(const R *a, const R *b, int slot, hipStream_t s, int op, const
                        slot, (const int *)d_ctl_, c ? 2 : 1, op);
 }
 
+// y = J x with (a, y) into scalar slot, or (y, c), (y, y) into slot, slot + 1
+// when c is set; then the scalar update op
+template <typename R>
+void Poisson::apply_J_dot(const R *x, R *y, const R *a, const R *c, int slot, int op, hipStream_t s)
+{
+    const long N = lv_[0].N;
+    const R *e0;
+    if constexpr (std::is_same<R, double>::value) e0 = lv_[0].e64;
+    else e0 = lv_[0].e32;
+    unsigned nb = nblocks(N, RED_BLOCK);
+    if (nb > 1024) nb = 1024;
+    hipLaunchKernelGGL(k_apply_J_dot<R>, dim3(nb), dim3(RED_BLOCK), 0, s, view<R>(0), x, e0, y, a, c, N, d_red_,
+                       (const int *)d_ctl_);
+    hipLaunchKernelGGL(k_dot_final, dim3(1), dim3(RED_BLOCK), 0, s, (const double *)d_red_, (int)nb, d_scal_,
+                       slot, (const int *)d_ctl_, c ? 2 : 1, op);
+}
+
 // right-preconditioned BiCGStab (van der Vorst 1992) for J x = b, x0 = 0,
 // M = one V-cycle from zero; scalars stay on the device
 template <typename R>
@@ -1974,21 +2026,22 @@ void Poisson::bicgstab(int iters, hipStream_t s)
         hipLaunchKernelGGL(k_bicg_p<R>, dim3(nb), dim3(256), 0, s, p, (const R *)r, (const R *)v, N,
                            (const double *)d_scal_, ctl);
         precond(p, ph);
-        hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)ph, (const R *)e0, v, ctl);
         if (direct) {
-            dot<R>(rh, v, 5, s, 2);
+            // v = J ph with (rh, v), then alpha
+            apply_J_dot<R>(ph, v, rh, nullptr, 5, 2, s);
         } else {
+            hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)ph, (const R *)e0, v, ctl);
             dot<R>(rh, v, 5, s);
             hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 2, ctl);
         }
         hipLaunchKernelGGL(k_bicg_s<R>, dim3(nb), dim3(256), 0, s, sv, (const R *)r, (const R *)v, N,
                            (const double *)d_scal_, ctl);
         precond(sv, shv);
-        hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)shv, (const R *)e0, t, ctl);
         if (direct) {
-            // (t, s) and (t, t) from one pass over t, then omega
-            dot<R>(t, sv, 6, s, 3, t);
+            // t = J shv with (t, s) and (t, t), then omega
+            apply_J_dot<R>(shv, t, nullptr, sv, 6, 3, s);
         } else {
+            hipLaunchKernelGGL(k_apply_J<R>, dim3(nb), dim3(256), 0, s, v0, (const R *)shv, (const R *)e0, t, ctl);
             dot<R>(t, sv, 6, s);
             dot<R>(t, t, 7, s);
             hipLaunchKernelGGL(k_scal, dim3(1), dim3(1), 0, s, d_scal_, 3, ctl);
